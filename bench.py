@@ -1,0 +1,163 @@
+"""Throughput of the sigmaSPH particle time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
+                    [--workload tgv160|dambreak] [--no-cpu-baseline]
+
+Workload (default): the 3-D weakly compressible Taylor-Green block, 160^3 = 4 096 000
+particles at dx = 1 mm in a periodic cube, +-0.05 dx seeded jitter, Tait c0 = 20 m/s,
+Wendland C2 with support 2.6 dx, delta-SPH 0.1, shifting on, CFL 0.2.  A "step" is one
+full particle time step (sort + the delta-plus-SPH substep + boundaries), inputs resident
+in HBM.  Prints ONE JSON line on stdout; diagnostics go to stderr.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_scene(name):
+    from sigmasph_amd import scenes
+    if name == "tgv160":
+        return scenes.tgv_block(n=160), "taylor-green block 160^3, periodic, dx=1mm"
+    if name == "dambreak":
+        return scenes.dam_break_3d(), "3-D dam break 0.4x0.4x0.2 m column, dx=0.01 m, reflect box"
+    raise SystemExit("unknown workload %r" % name)
+
+
+def cpu_baseline(workload, precision, seconds=15.0):
+    """The FP64 CPU oracle (a C/OpenMP restatement of the reference step) timed on a bounded
+    sample of the same workload: a periodic 48^3 Taylor-Green block with identical dx,
+    jitter, EOS and numerics (per-particle work is that of the 160^3 block)."""
+    from oracle import oracle as O
+    from sigmasph_amd.config import SimConfig
+    from sigmasph_amd.particle_system import SceneLayout
+    from sigmasph_amd.DSPH import FluidSettings
+    from sigmasph_amd import scenes
+    if workload == "tgv160":
+        sc, sample = scenes.tgv_block(n=48), "periodic taylor-green block 48^3 (110 592 particles), same numerics"
+    else:
+        sc, sample = scenes.dam_break_3d(), "the full 32 000-particle dam break"
+    lay = SceneLayout(SimConfig(sc))
+    fs = FluidSettings(lay)
+    P = fs.params()
+    s = O.OracleState.from_snapshot(lay.snapshot())
+    s.scal[0] = fs.dt_initial
+    threads = min(os.cpu_count() or 1, 16)
+    O.set_threads(threads)
+    O.fluid_step(lay.dims, P, s)          # warm-up (page-in, first sort)
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        O.fluid_step(lay.dims, P, s)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > seconds or steps >= 50:
+            break
+    rate = s.n * steps / el
+    return {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "%s, %d steps in %.1f s" % (sample, steps, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
+    ap.add_argument("--workload", default="tgv160", choices=("tgv160", "dambreak"))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has "
+              "one MI355X); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from sigmasph_amd.config import SimConfig
+    from sigmasph_amd.particle_system import ParticleSystem
+    from sigmasph_amd.bytes_model import fluid_kernel_bytes
+    from sigmasph_amd import _native as N
+
+    scene, desc = make_scene(args.workload)
+    log("building %s (%s)" % (args.workload, args.precision))
+    ps = ParticleSystem(SimConfig(scene), precision=args.precision)
+    solver = ps.build_solver()
+    solver.initialize()
+    n = ps.particle_max_num
+    log("particles: %d" % n)
+    for _ in range(args.warmup):
+        solver.step()
+    torch.cuda.synchronize()
+    lib = ps._lib
+    N.check(lib.sph_timing(ps.ctx, 1), ps.ctx, "sph_timing")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solver.step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    kinds = ["pass_volume", "pass_force", "pass_shift", "sort"]
+    avg = {}
+    for k, name in enumerate(kinds):
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int64()
+        N.check(lib.sph_timing_read(ps.ctx, k, ctypes.byref(ms), ctypes.byref(cnt)), ps.ctx, "timing")
+        avg[name] = (ms.value, cnt.value)
+    lib.sph_timing(ps.ctx, 0)
+    log("kernel averages (ms): " + ", ".join("%s %.3f (%d)" % (k, v[0], v[1]) for k, v in avg.items()))
+    ms_step = 1e3 * el / args.steps
+    rate = n * args.steps / el
+    bm = fluid_kernel_bytes(args.precision, pst=bool(solver.pst_enabled),
+                            track_visc=bool(solver.track_visc_work))
+    dom = max(("pass_volume", "pass_force", "pass_shift"), key=lambda k: avg[k][0])
+    dur_s = avg[dom][0] * 1e-3
+    achieved = bm[dom] * n / dur_s / 1e9 if dur_s > 0 else 0.0
+    pairs = None
+    try:
+        pr = ps.neighbor_pairs()
+        pairs = len(pr) / n
+    except Exception as e:  # noqa: BLE001 - diagnostic only
+        log("pair count failed: %s" % e)
+    out = {
+        "metric": "particle-steps/s (1 GPU)",
+        "value": rate,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32 pair sums + f64 integration",
+        "data": "synthetic",
+        "config": {"workload": desc, "particles": n, "precision": args.precision,
+                   "support": "2.6 dx", "parallelism": "single GPU"},
+        "pair_interactions_per_s": (pairs * rate) if pairs else None,
+        "hbm_fraction_step": bm["step"] * rate / 1e9 / HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_particle": bm[dom], "avg_ms": avg[dom][0]},
+        "kernel_ms": {k: v[0] for k, v in avg.items()},
+        "cpu_baseline": None,
+    }
+    ps.close()
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(args.workload, args.precision)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

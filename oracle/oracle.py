@@ -1,0 +1,196 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/sph_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+bench.py's CPU baseline, as the checker.  The product (sigmasph_amd/) never imports it.
+
+The oracle restates the reference's particle step (see the header of sph_oracle.c for
+the file:line map).  Its state is plain float64 numpy arrays; `from_snapshot` builds one
+from `snapshot(ps)` of a GPU ParticleSystem so both start from identical particles.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+class OrState(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64),
+                ("pid", _ip), ("obj", _ip),
+                ("x", _dp), ("x0", _dp), ("u", _dp), ("v", _dp), ("shift", _dp),
+                ("V", _dp), ("m", _dp), ("p", _dp),
+                ("acc", _dp), ("lam_grad", _dp),
+                ("lam", _dp), ("div", _dp), ("de_visc", _dp),
+                ("L6", _dp),
+                ("scal", _dp)]
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        vp = ctypes.c_void_p
+        L.oracle_fluid_step.argtypes = [vp, vp, ctypes.POINTER(OrState)]
+        L.oracle_fluid_step.restype = None
+        L.oracle_sort_and_pairs.argtypes = [vp, ctypes.POINTER(OrState), _ip, ctypes.c_int64]
+        L.oracle_sort_and_pairs.restype = ctypes.c_int64
+        L.oracle_pairs_live.argtypes = [vp, ctypes.POINTER(OrState), _ip, ctypes.c_int64]
+        L.oracle_pairs_live.restype = ctypes.c_int64
+        L.oracle_reduce.argtypes = [ctypes.POINTER(OrState), ctypes.c_int, _dp, vp, _dp]
+        L.oracle_reduce.restype = None
+        L.oracle_wendland.argtypes = [vp, ctypes.c_double]
+        L.oracle_wendland.restype = ctypes.c_double
+        L.oracle_wendland_grad.argtypes = [vp, _dp, _dp]
+        L.oracle_wendland_grad.restype = None
+        L.oracle_spd_inverse3.argtypes = [_dp, ctypes.c_double, _dp, _dp]
+        L.oracle_spd_inverse3.restype = None
+        L.oracle_spd_inverse2.argtypes = [_dp, ctypes.c_double, _dp]
+        L.oracle_spd_inverse2.restype = None
+        L.oracle_spd_weight.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.oracle_spd_weight.restype = ctypes.c_double
+        L.oracle_tait_energy.argtypes = [ctypes.c_double] * 4
+        L.oracle_tait_energy.restype = ctypes.c_double
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_max_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+class OracleState:
+    """float64 state of the oracle; vectors are (n, 3)."""
+
+    FIELDS3 = ("x", "x0", "u", "v", "shift", "acc", "lam_grad")
+    FIELDS1 = ("V", "m", "p", "lam", "div", "de_visc")
+
+    def __init__(self, n):
+        self.n = n
+        for f in self.FIELDS3:
+            setattr(self, f, np.zeros((n, 3)))
+        for f in self.FIELDS1:
+            setattr(self, f, np.zeros(n))
+        self.L6 = np.zeros((n, 6))
+        self.pid = np.arange(n, dtype=np.int32)
+        self.obj = np.zeros(n, dtype=np.int32)
+        self.scal = np.zeros(32)
+
+    @classmethod
+    def from_snapshot(cls, snap):
+        s = cls(len(snap["x"]))
+        for k, v in snap.items():
+            if hasattr(s, k):
+                cur = getattr(s, k)
+                setattr(s, k, np.ascontiguousarray(np.asarray(v, dtype=cur.dtype).reshape(cur.shape)))
+        return s
+
+    def c(self):
+        st = OrState()
+        st.n = self.n
+        st.pid, st.obj = _ptr(self.pid, _ip), _ptr(self.obj, _ip)
+        for f in self.FIELDS3 + self.FIELDS1 + ("L6", "scal"):
+            setattr(st, f, _ptr(getattr(self, f)))
+        self._c = st
+        return st
+
+    def by_pid(self, name):
+        order = np.argsort(self.pid, kind="stable")
+        return getattr(self, name)[order]
+
+
+def fluid_step(dims, params, s):
+    st = s.c()
+    lib().oracle_fluid_step(ctypes.byref(dims), ctypes.byref(params), ctypes.byref(st))
+
+
+def sort_and_pairs(dims, s, cap=None):
+    cap = cap or 256 * s.n + 1024
+    out = np.zeros((cap, 3), dtype=np.int32)
+    st = s.c()
+    k = lib().oracle_sort_and_pairs(ctypes.byref(dims), ctypes.byref(st), _ptr(out, _ip), cap)
+    if k > cap:
+        raise RuntimeError("pair buffer too small")
+    return out[:k].copy()
+
+
+def pairs_live(dims, s, cap=None):
+    cap = cap or 256 * s.n + 1024
+    out = np.zeros((cap, 3), dtype=np.int32)
+    st = s.c()
+    k = lib().oracle_pairs_live(ctypes.byref(dims), ctypes.byref(st), _ptr(out, _ip), cap)
+    return out[:k].copy()
+
+
+def reduce(s, which, g=(0.0, 0.0, 0.0), params=None):
+    out = np.zeros(3)
+    gg = np.asarray(g, dtype=np.float64)
+    st = s.c()
+    lib().oracle_reduce(ctypes.byref(st), which, _ptr(gg),
+                        ctypes.byref(params) if params is not None else None, _ptr(out))
+    return out
+
+
+def wendland(dims, r):
+    return lib().oracle_wendland(ctypes.byref(dims), float(r))
+
+
+def wendland_grad(dims, r):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    g = np.zeros(3)
+    lib().oracle_wendland_grad(ctypes.byref(dims), _ptr(r), _ptr(g))
+    return g
+
+
+def spd_inverse3(E, tol):
+    E = np.ascontiguousarray(E, dtype=np.float64).reshape(9)
+    L6 = np.zeros(6)
+    me = np.zeros(1)
+    lib().oracle_spd_inverse3(_ptr(E), float(tol), _ptr(L6), _ptr(me))
+    return L6, float(me[0])
+
+
+def spd_inverse2(E2, tol):
+    E2 = np.ascontiguousarray(E2, dtype=np.float64).reshape(4)
+    L6 = np.zeros(6)
+    lib().oracle_spd_inverse2(_ptr(E2), float(tol), _ptr(L6))
+    return L6
+
+
+def spd_weight(e, tol):
+    return lib().oracle_spd_weight(float(e), float(tol))
+
+
+def tait_energy(rho, rho0, S, gamma):
+    return lib().oracle_tait_energy(float(rho), float(rho0), float(S), float(gamma))
+
+
+def set_threads(n):
+    lib().oracle_set_threads(int(n))
+
+
+def max_threads():
+    return lib().oracle_max_threads()
+
+
+def L6_to_mat(L6):
+    a = np.asarray(L6)
+    return np.array([[a[0], a[3], a[4]], [a[3], a[1], a[5]], [a[4], a[5], a[2]]])

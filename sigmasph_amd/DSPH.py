@@ -1,0 +1,174 @@
+"""DSPHSolver: the delta-plus-SPH fluid step on the GPU.
+
+Same construction, configuration reading and public methods as the reference's
+`DSPHSolver` (DSPH.py:50-1883): `initialize()`, `step()`, `substep()`, `dt[None]`,
+`CFL`, `c_signal`, and the energy budget.  `step()` is one call into libsigmasph.so
+(`sph_fluid_step`): the sort, the three fused neighbour passes and the streaming
+kernels, with dt kept on the device.
+
+Not built (each raises when a scene asks for it): gamma-SPH (`Diffusion.gamma`),
+`Shifting.normal: "eigvec"`, the initial relaxation (`Shifting.initialRelaxation`) and
+per-particle smoothing lengths.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+from .sph_base import SPHBase
+from .fields import Scalar
+
+
+class FluidSettings:
+    """The constructor half of DSPHSolver (DSPH.py:68-540) that reads the scene: every
+    constant of the step, and `params()`, the C struct the kernels take.  Host only, so
+    CPU tests and the oracle can build the same parameters without a GPU."""
+
+    def __init__(self, layout):
+        self.ps = ps = layout
+        c = ps.cfg.get_cfg
+        self.g = np.array(c("gravitation"), dtype=np.float64)
+        self.viscosity = float(c("viscosity"))
+        self.c0 = float(c("c0"))
+        self.density_0 = float(c("density0"))
+        self.exponent = float(c("exponent"))
+        self.stiffness = self.c0 ** 2 * self.density_0 / self.exponent      # DSPH.py:78
+        self.c_signal = self.c0
+        self.l_eig_tol_lam = 0.02 if c("L_EIG_TOL_LAM") is None else float(c("L_EIG_TOL_LAM"))
+        self.CFL = float(c("CFL"))
+        self.dt_initial = self.CFL * ps.h_min / self.c0                        # DSPH.py:120
+        dt_user = c("timeStepSize")
+        if dt_user is not None:
+            self.dt_initial = float(dt_user)
+        self.delta_sph = float(c("DELTA_SPH") or 0.0)
+        self.allow_negative_pressure = bool(c("allowNegativePressure") or False)
+        self.pst_enabled = bool(c("PST_ENABLED") or False)
+        self.pst_R = float(c("PST_R") if c("PST_R") is not None else 0.2)
+        self.pst_n = int(c("PST_N") if c("PST_N") is not None else 4)
+        surf = str(c("PST_SURFACE_SHIFT") or "tangential").lower()
+        self.pst_surface_none = surf == "none"
+        self.pst_max_shift = 0.05 if c("PST_MAX_SHIFT") is None else float(c("PST_MAX_SHIFT"))
+        if int(c("PST_RELAX_STEPS") or 0) > 0:
+            raise NotImplementedError("Shifting.initialRelaxation is not part of this build")
+        ale = c("PST_ALE_MOMENTUM")
+        self.pst_ale_momentum = True if ale is None else bool(ale)
+        ma = str(c("PST_MA_MODE") or "local").lower()
+        self.pst_ma_global = ma == "global"
+        self.pst_ma_fixed_mode = ma == "fixed"
+        self.pst_ma_fixed = 0.1 if c("PST_MA_FIXED") is None else float(c("PST_MA_FIXED"))
+        mode = str(c("PST_MODE") or "sun").lower()
+        if mode not in ("sun", "colle"):
+            raise ValueError('PST_MODE must be "sun" or "colle", got: %r' % mode)
+        self.pst_colle = mode == "colle"
+        lo, hi = c("PST_LAMBDA_LO"), c("PST_LAMBDA_HI")
+        self.pst_lambda_lo = (0.2 if self.pst_colle else 0.4) if lo is None else float(lo)
+        self.pst_lambda_hi = 0.75 if hi is None else float(hi)
+        self.pst_m_delta = 0.4 if c("PST_M_DELTA") is None else float(c("PST_M_DELTA"))
+        clamp_default = "both" if (self.pst_colle and not self.pst_ma_fixed_mode) else "displacement"
+        clamp = str(c("PST_CLAMP") or clamp_default).lower()
+        self.pst_clamp_disp = clamp in ("displacement", "both")
+        self.pst_clamp_vel = clamp in ("velocity", "both")
+        if str(c("PST_NORMAL") or "gradlam").lower() != "gradlam":
+            raise NotImplementedError('Shifting.normal "eigvec" is not part of this build')
+        self.dx0 = 2.0 * ps.particle_radius
+        track = c("TRACK_VISCOUS_WORK")
+        self.track_visc_work = (True if track is None else bool(track)) and self.viscosity > 0.0
+
+    def params(self):
+        ps = self.ps
+        P = N.SphFluidParams()
+        P.rho0, P.c0, P.gamma, P.stiffness = self.density_0, self.c0, self.exponent, self.stiffness
+        P.viscosity = self.viscosity
+        P.delta_sph = self.delta_sph
+        P.cfl = self.CFL
+        P.c_signal = self.c_signal
+        for k in range(3):
+            P.g[k] = float(self.g[k])
+            P.domain_size[k] = float(ps.domain_size[k])
+        P.dx0 = self.dx0
+        P.axi_r_min = ps.axi_r_min
+        P.pst_R, P.pst_max_shift, P.pst_ma_fixed = self.pst_R, self.pst_max_shift, self.pst_ma_fixed
+        P.pst_lambda_lo, P.pst_lambda_hi = self.pst_lambda_lo, self.pst_lambda_hi
+        P.pst_m_delta, P.l_eig_tol_lam = self.pst_m_delta, self.l_eig_tol_lam
+        P.restitution, P.padding = ps.wall_restitution, ps.padding
+        P.pst_n = self.pst_n
+        P.allow_negative_pressure = int(self.allow_negative_pressure)
+        P.track_visc_work = int(self.track_visc_work)
+        P.dt_pair = 1
+        P.pst_enabled = int(self.pst_enabled)
+        P.pst_colle = int(self.pst_colle)
+        P.pst_ma_mode = 2 if self.pst_ma_fixed_mode else (1 if self.pst_ma_global else 0)
+        P.pst_surface_none = int(self.pst_surface_none)
+        P.pst_clamp_disp, P.pst_clamp_vel = int(self.pst_clamp_disp), int(self.pst_clamp_vel)
+        P.pst_ale_momentum = int(self.pst_ale_momentum)
+        P.walls = int(ps.walls)
+        P.displacement_positions = int(ps.displacement_positions)
+        P.compensated_volume = int(ps.compensated_volume)
+        return P
+
+
+class DSPHSolver(SPHBase):
+    def __init__(self, particle_system):
+        super().__init__(particle_system)
+        ps = self.ps
+        self.settings = FluidSettings(ps)
+        for k, v in vars(self.settings).items():
+            if k not in ("ps", "g", "viscosity"):
+                setattr(self, k, v)
+        self.dt[None] = self.settings.dt_initial
+        self.visc_work = Scalar(ps.scal, N.SPH_S_VISC_WORK)
+        self.diff_work = Scalar(ps.scal, N.SPH_S_DIFF_WORK)
+        self.leftover_work = Scalar(ps.scal, N.SPH_S_LEFTOVER)
+        self.max_c_sq = Scalar(ps.scal, N.SPH_S_MAX_C2)
+        self.max_accel = Scalar(ps.scal, N.SPH_S_MAX_ACC)
+        self.max_v = Scalar(ps.scal, N.SPH_S_MAX_V)
+        self.lam = ps.lam
+        self.params = self.settings.params()
+
+    # ----------------------------------------------------------------------------------
+    def _call(self, fn, name):
+        ps = self.ps
+        N.check(fn(ps.ctx, ctypes.byref(ps.state), ctypes.byref(self.params), ps.stream()), ps.ctx, name)
+
+    def step(self):
+        """SPHBase.step (sph_base.py:622-655) with DSPHSolver.substep (DSPH.py:1854-1883)."""
+        self._call(self.ps._lib.sph_fluid_step, "sph_fluid_step")
+
+    def substep(self):
+        """The substep's phases on an already sorted state.  The last phase also runs the
+        boundary kernels that the reference runs in `step()` after `substep()`."""
+        L = self.ps._lib
+        self._call(L.sph_fluid_pass_volume, "sph_fluid_pass_volume")
+        self._call(L.sph_fluid_pass_force, "sph_fluid_pass_force")
+        self._call(L.sph_fluid_advect, "sph_fluid_advect")
+        self._call(L.sph_fluid_pass_shift, "sph_fluid_pass_shift")
+        self._call(L.sph_fluid_finish, "sph_fluid_finish")
+
+    # ---- energy budget (DSPH.py:1670-1852) -------------------------------------------
+    def compute_elastic_energy(self):
+        """Sum m e_tait(rho), zero below rho0 unless negative pressure is allowed."""
+        return float(self._reduce(N.SPH_R_TAIT_ENERGY, self.params)[0])
+
+    def compute_leftover_energy(self):
+        return float(self.leftover_work[None])
+
+    def compute_viscous_energy(self):
+        return float(self.visc_work[None])
+
+    def compute_diffusion_work(self):
+        return float(self.diff_work[None])
+
+    def compute_integrated_gap(self):
+        return self.compute_diffusion_work()
+
+    def energy_budget(self):
+        ke = self.compute_kinetic_energy()
+        se = self.compute_elastic_energy()
+        pe = self.compute_potential_energy()
+        lw = self.compute_leftover_energy()
+        wd = self.compute_diffusion_work()
+        return {"ke": ke, "se": se, "pe": pe, "ie": 0.0, "hg": 0.0,
+                "wall": self.compute_wall_energy(), "visc": self.compute_viscous_energy(),
+                "leftover": lw, "leftover_raw": lw, "comp": se, "comp_complete": True,
+                "e_neg": 0.0, "pw": 0.0, "wd": wd, "gap_integrated": wd,
+                "total": ke + se + pe}

@@ -1,0 +1,177 @@
+"""ctypes binding of libsigmasph.so (include/sigmasph.h).
+
+The structs below mirror the header field for field.  Loading fails loudly when the
+library has not been built: there is no CPU fallback for the step.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsigmasph.so")
+
+# return codes
+SPH_OK, SPH_EINVAL, SPH_EHIP, SPH_ENOMEM, SPH_ECAPACITY = 0, -1, -2, -3, -4
+# precision
+SPH_PREC_FP64, SPH_PREC_MIXED = 0, 1
+# geometry flags
+SPH_TWO_D = 1 << 0
+SPH_AXISYMMETRIC = 1 << 1
+SPH_MIRROR_Y = 1 << 2
+SPH_MIRROR_Z = 1 << 3
+SPH_PERIODIC_X = 1 << 4
+SPH_PERIODIC_Y = 1 << 5
+SPH_PERIODIC_Z = 1 << 6
+SPH_PLANE_STRAIN = 1 << 7
+# device scalar slots
+SPH_S_DT = 0
+SPH_S_DIFF_WORK = 1
+SPH_S_LEFTOVER = 2
+SPH_S_VISC_WORK = 3
+SPH_S_WALL_WORK = 4
+SPH_S_MAX_C2 = 5
+SPH_S_MAX_ACC = 6
+SPH_S_MAX_V = 7
+SPH_S_OFF_GRID_NUM = 8
+SPH_S_OFF_GRID_MASS = 9
+SPH_S_OUT_OF_BOX = 10
+SPH_NSCAL = 32
+# reductions
+SPH_R_KINETIC = 0
+SPH_R_POTENTIAL = 1
+SPH_R_MASS = 2
+SPH_R_MOMENTUM = 3
+SPH_R_GROSS_MOMENTUM = 4
+SPH_R_TAIT_ENERGY = 5
+SPH_R_KINETIC_OFFGRID = 6
+
+_d3 = ctypes.c_double * 3
+_i3 = ctypes.c_int32 * 3
+
+
+class SphDims(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int64),
+        ("precision", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+        ("h", ctypes.c_double),
+        ("cell", _d3),
+        ("origin", _d3),
+        ("period", _d3),
+        ("box_lo", _i3),
+        ("box_n", _i3),
+        ("hash_bits", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
+    ]
+
+
+class SphSorted(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in
+                ("pid", "obj", "x", "x0", "u", "v", "shift", "V", "V_lo", "m", "p")]
+
+
+class SphState(ctypes.Structure):
+    _fields_ = [
+        ("buf", SphSorted * 2),
+        ("cur", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
+        ("acc", ctypes.c_void_p),
+        ("lam", ctypes.c_void_p),
+        ("L6", ctypes.c_void_p),
+        ("lam_grad", ctypes.c_void_p),
+        ("div", ctypes.c_void_p),
+        ("de_visc", ctypes.c_void_p),
+        ("V_new", ctypes.c_void_p),
+        ("scal", ctypes.c_void_p),
+    ]
+
+
+class SphFluidParams(ctypes.Structure):
+    _fields_ = [
+        ("rho0", ctypes.c_double), ("c0", ctypes.c_double), ("gamma", ctypes.c_double),
+        ("stiffness", ctypes.c_double),
+        ("viscosity", ctypes.c_double),
+        ("delta_sph", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+        ("c_signal", ctypes.c_double),
+        ("g", _d3),
+        ("dx0", ctypes.c_double),
+        ("axi_r_min", ctypes.c_double),
+        ("pst_R", ctypes.c_double), ("pst_max_shift", ctypes.c_double),
+        ("pst_ma_fixed", ctypes.c_double), ("pst_lambda_lo", ctypes.c_double),
+        ("pst_lambda_hi", ctypes.c_double),
+        ("pst_m_delta", ctypes.c_double), ("l_eig_tol_lam", ctypes.c_double),
+        ("restitution", ctypes.c_double), ("padding", ctypes.c_double),
+        ("domain_size", _d3),
+        ("pst_n", ctypes.c_int32),
+        ("allow_negative_pressure", ctypes.c_int32),
+        ("track_visc_work", ctypes.c_int32),
+        ("dt_pair", ctypes.c_int32),
+        ("pst_enabled", ctypes.c_int32),
+        ("pst_colle", ctypes.c_int32),
+        ("pst_ma_mode", ctypes.c_int32),
+        ("pst_surface_none", ctypes.c_int32),
+        ("pst_clamp_disp", ctypes.c_int32), ("pst_clamp_vel", ctypes.c_int32),
+        ("pst_ale_momentum", ctypes.c_int32),
+        ("walls", ctypes.c_int32),
+        ("displacement_positions", ctypes.c_int32),
+        ("compensated_volume", ctypes.c_int32),
+    ]
+
+
+#: every symbol include/sigmasph.h declares
+EXPORTS = (
+    "sph_ctx_create", "sph_ctx_destroy", "sph_last_error", "sph_version",
+    "sph_sort", "sph_neighbor_pairs",
+    "sph_fluid_step", "sph_fluid_pass_volume", "sph_fluid_pass_force",
+    "sph_fluid_advect", "sph_fluid_pass_shift", "sph_fluid_finish",
+    "sph_reduce", "sph_timing", "sph_timing_read",
+)
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libsigmasph.so is not built (%s); run `python -m sigmasph_amd.build`. "
+            "There is no CPU fallback for the particle step." % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    L.sph_version.restype = ctypes.c_char_p
+    L.sph_version.argtypes = []
+    L.sph_last_error.restype = ctypes.c_char_p
+    L.sph_last_error.argtypes = [vp]
+    L.sph_ctx_create.restype = ctypes.c_int
+    L.sph_ctx_create.argtypes = [P(vp), P(SphDims)]
+    L.sph_ctx_destroy.restype = ctypes.c_int
+    L.sph_ctx_destroy.argtypes = [vp]
+    L.sph_sort.restype = ctypes.c_int
+    L.sph_sort.argtypes = [vp, P(SphState), vp]
+    L.sph_neighbor_pairs.restype = ctypes.c_int
+    L.sph_neighbor_pairs.argtypes = [vp, P(SphState), vp, ctypes.c_int64, vp, vp]
+    for name in ("sph_fluid_step", "sph_fluid_pass_volume", "sph_fluid_pass_force",
+                 "sph_fluid_advect", "sph_fluid_pass_shift", "sph_fluid_finish"):
+        f = getattr(L, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [vp, P(SphState), P(SphFluidParams), vp]
+    L.sph_reduce.restype = ctypes.c_int
+    L.sph_reduce.argtypes = [vp, P(SphState), ctypes.c_int32, vp, P(SphFluidParams), vp, vp]
+    L.sph_timing.restype = ctypes.c_int
+    L.sph_timing.argtypes = [vp, ctypes.c_int32]
+    L.sph_timing_read.restype = ctypes.c_int
+    L.sph_timing_read.argtypes = [vp, ctypes.c_int32, P(ctypes.c_double), P(ctypes.c_int64)]
+    _lib = L
+    return L
+
+
+def check(rc, ctx, what):
+    """Raise RuntimeError with the library's message for a non-zero return code."""
+    if rc != SPH_OK:
+        msg = lib().sph_last_error(ctx)
+        msg = msg.decode() if msg else ""
+        raise RuntimeError("%s failed (code %d): %s" % (what, rc, msg))

@@ -1,0 +1,39 @@
+"""Algorithmic HBM bytes per particle for each kernel of the fluid step.
+
+Counting rule: every per-particle field a kernel reads is counted once and every field
+it writes once, at its useful width (a 3-vector of doubles is 24 B even though it is
+stored in a 32 B slot); neighbour re-reads are free (ideal LDS/L2 reuse); scalar
+reductions are free.  `r` is the width of a Real field: 8 in fp64 mode, 4 in mixed mode.
+The cell table (one int per cell, about N / 17.6 cells at support 2.6 dx) is counted at
+its size per particle.
+"""
+
+
+def fluid_kernel_bytes(precision="fp64", pst=True, track_visc=False, cells_per_particle=1.0 / 17.6):
+    r = 8 if precision == "fp64" else 4
+    v3 = 24                      # double 3-vector (x, x0, u, v)
+    s3 = 3 * r                   # Real 3-vector (shift, acc, lam_grad)
+    cell = 4 * cells_per_particle
+    mixed = precision != "fp64"
+    out = {}
+    # pass A: reads x, v, shift, V, m, p(own), [V_lo]; writes lam, L6, div, V_new, p, [V_lo]
+    rd = v3 + v3 + (s3 if pst else 0) + r + r + r + (r if mixed else 0) + 2 * cell
+    wr = r + 6 * r + r + r + r + (r if mixed else 0)
+    out["pass_volume"] = rd + wr
+    # pass B: reads x, v, shift, V, V_new, m, p, lam, L6; writes acc, lam_grad, [de_visc]
+    rd = v3 + v3 + (s3 if pst else 0) + r + r + r + r + r + 6 * r + 2 * cell
+    wr = s3 + s3 + (r if track_visc else 0)
+    out["pass_force"] = rd + wr
+    # pass C: reads x, V, m, v(own), lam(own), lam_grad(own); writes shift
+    out["pass_shift"] = (v3 + r + r + v3 + r + s3 + 2 * cell + s3) if pst else 0
+    # sort: key pass reads x, writes key; histogram + scan over the cells; scatter reads
+    # key writes slot; rank reads slot/key/pid; permute reads perm and every sorted
+    # field and writes it (pid, obj, x, x0, u, v, shift, V, m, p, [V_lo])
+    fields = 4 + 4 + 4 * v3 + s3 + 3 * r + (r if mixed else 0)
+    out["sort"] = (v3 + 4) + 3 * 2 * cell + (4 + 4) + (4 + 4 + 4) + 4 + 2 * fields
+    # advect: reads acc, v, u, x0, m, V_new, [de_visc]; writes v, u, x, V
+    out["advect"] = (s3 + 3 * v3 + r + r + (r if track_visc else 0)) + (3 * v3 + r)
+    # finish: reads x, u, x0, v, shift, m; writes x, u, v
+    out["finish"] = (4 * v3 + (s3 if pst else 0) + r) + 3 * v3
+    out["step"] = sum(out.values())
+    return out

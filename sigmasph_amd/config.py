@@ -1,0 +1,343 @@
+"""Scene configuration: the flat canonical parameter names the solvers read.
+
+`SimConfig` keeps the reference's reader contract (config_builder.py:176-270): a scene is
+a JSON (with `#` comments) or TOML table of cards -- `Domain`, `Time`, `Solver`, `Load`,
+`Diffusion`, `Shifting` -- plus an optional flat `Configuration` block of canonical names
+applied on top, `Materials[]`, and `FluidBlocks` / `SolidBlocks`.  Everything downstream
+reads flat names through `get_cfg(name)`.  Unknown keys are errors with a suggestion, as
+in the reference (params.py:1479-1488).
+
+Only the parameters of the delta-plus-SPH particle step are registered here (the subset
+of params.py:253-1316 that the step reads); `Domain.periodic` is an addition of this
+build (periodic boxes do not exist in the reference).
+"""
+import difflib
+import json
+import os
+
+try:
+    import tomllib
+except ModuleNotFoundError:  # python 3.10
+    import tomli as tomllib
+
+
+class SceneError(ValueError):
+    """A scene that cannot be read as written; lists every problem found
+    (config_builder.py:56-64)."""
+
+
+class P:
+    __slots__ = ("name", "card", "field", "type", "default", "required", "choices")
+
+    def __init__(self, name, card, field, type_, default=None, required=False, choices=None):
+        self.name, self.card, self.field, self.type = name, card, field, type_
+        self.default, self.required, self.choices = default, required, choices
+
+
+def _colle(c):
+    return str(c.get("PST_MODE") or "sun").lower() == "colle"
+
+
+def _lambda_lo_default(c):       # params.py:234-237
+    return 0.2 if _colle(c) else 0.4
+
+
+def _clamp_default(c):           # params.py:240-247
+    if _colle(c) and str(c.get("PST_MA_MODE") or "local").lower() != "fixed":
+        return "both"
+    return "displacement"
+
+
+PARAMS = [
+    # Domain (params.py:256-393)
+    P("domainStart", "Domain", "start", "vec3", [0.0, 0.0, 0.0]),
+    P("domainEnd", "Domain", "end", "vec3", [1.0, 1.0, 1.0]),
+    P("particleRadius", "Domain", "particleRadius", "float", required=True),
+    P("supportRadiusFactor", "Domain", "supportRadiusFactor", "float", 4.0),
+    P("planeStrain", "Domain", "planeStrain", "bool", False),
+    P("axisymmetric", "Domain", "axisymmetric", "bool", False),
+    P("symmetryPlanes", "Domain", "symmetryPlanes", "str", ""),
+    P("AXI_R_MIN", "Domain", "axisRadiusMin", "float", 0.5),
+    P("DOMAIN_BOUNDARY", "Domain", "boundary", "str", "open", choices=("open", "reflect")),
+    P("DOMAIN_RESTITUTION", "Domain", "restitution", "float", 0.5),
+    P("PERIODIC", "Domain", "periodic", "str", ""),
+    # Time (params.py:394-417)
+    P("CFL", "Time", "CFL", "float", required=True),
+    P("timeStepSize", "Time", "stepSize", "float", None),
+    P("sim_duration", "Time", "duration", "float", None),
+    P("maxSteps", "Time", "maxSteps", "int", None),
+    P("numberOfStepsPerRenderUpdate", "Time", "stepsPerRenderUpdate", "int", None),
+    # Solver (params.py:418-495)
+    P("simulationMethod", "Solver", "type", "str", required=True,
+      choices=("deltaPlusSPH", "hypoElastic")),
+    P("viscosity", "Solver", "viscosity", "float", required=True),
+    P("allowNegativePressure", "Solver", "allowNegativePressure", "bool", False),
+    P("DISPLACEMENT_POSITIONS", "Solver", "displacementPositions", "bool", True),
+    P("COMPENSATED_VOLUME", "Solver", "compensatedVolume", "bool", True),
+    P("stiffness", "Solver", "stiffness", "float", 50000.0),
+    P("TRACK_VISCOUS_WORK", "Solver", "trackViscousWork", "bool", True),
+    # Load (params.py:977)
+    P("gravitation", "Load", "gravitation", "vec3", required=True),
+    # Diffusion (params.py:986-1013)
+    P("DELTA_SPH", "Diffusion", "delta", "float", 0.0),
+    P("DELTA_PAIRS", "Diffusion", "pairs", "str", "samematerial"),
+    # Shifting (params.py:1017-1131)
+    P("PST_ENABLED", "Shifting", "enabled", "bool", True),
+    P("PST_MODE", "Shifting", "mode", "str", "sun", choices=("sun", "colle")),
+    P("PST_MA_MODE", "Shifting", "maMode", "str", "local", choices=("local", "global", "fixed")),
+    P("PST_MA_FIXED", "Shifting", "maFixed", "float", 0.1),
+    P("PST_R", "Shifting", "R", "float", 0.2),
+    P("PST_N", "Shifting", "n", "int", 4),
+    P("PST_MAX_SHIFT", "Shifting", "maxShift", "float", 0.05),
+    P("PST_RELAX_STEPS", "Shifting", "initialRelaxation", "int", 0),
+    P("PST_SURFACE_SHIFT", "Shifting", "surfaceShift", "str", "tangential",
+      choices=("tangential", "none")),
+    P("PST_LAMBDA_LO", "Shifting", "lambdaLo", "float", _lambda_lo_default),
+    P("PST_LAMBDA_HI", "Shifting", "lambdaHi", "float", 0.75),
+    P("PST_NORMAL", "Shifting", "normal", "str", "gradlam", choices=("gradlam",)),
+    P("PST_CLAMP", "Shifting", "clamp", "str", _clamp_default,
+      choices=("displacement", "velocity", "both")),
+    P("PST_M_DELTA", "Shifting", "mDelta", "float", 0.4),
+    P("PST_ALE_MOMENTUM", "Shifting", "aleMomentum", "bool", True),
+    P("L_EIG_TOL_LAM", "Shifting", "lEigTolLam", "float", 0.02),
+    # legacy flat material constants (derived from Materials[] when it is given)
+    P("density0", "Materials", "density0", "float", 1000.0),
+    P("c0", "Materials", "c0", "float", None),
+    P("exponent", "Materials", "exponent", "float", 7.0),
+]
+BY_NAME = {p.name: p for p in PARAMS}
+CARDS = ("Domain", "Time", "Solver", "Load", "Diffusion", "Shifting")
+BLOCK_KEYS = {"objectId", "start", "end", "translation", "scale", "velocity", "density",
+              "color", "rotVelocity", "divVelocity", "shape", "material",
+              # build-side initial conditions (no reference counterpart)
+              "jitter", "seed", "taylorGreen"}
+TOP_LEVEL = set(CARDS) | {"Configuration", "Materials", "FluidBlocks", "SolidBlocks",
+                          "Constraints", "ParticlesFile", "Output"}
+
+
+def suggest(key, choices):
+    m = difflib.get_close_matches(key, list(choices), n=1)
+    return "  Did you mean %r?" % m[0] if m else ""
+
+
+def strip_json_comments(text):
+    """Drop `#` comments outside string literals (config_builder.py:130-172)."""
+    out = []
+    for line in text.splitlines():
+        in_str, esc, cut = False, False, None
+        for k, ch in enumerate(line):
+            if ch == "\\" and in_str:
+                esc = not esc
+                continue
+            if ch == '"' and not esc:
+                in_str = not in_str
+            esc = False
+            if ch == "#" and not in_str:
+                cut = k
+                break
+        out.append(line if cut is None else line[:cut])
+    return "\n".join(out)
+
+
+def _coerce(p, value, where, errors):
+    if value is None:
+        return None
+    t = p.type
+    try:
+        if t == "float":
+            if isinstance(value, bool) or not isinstance(value, (int, float)):
+                raise TypeError
+            return float(value)
+        if t == "int":
+            if isinstance(value, bool) or (isinstance(value, float) and value != int(value)):
+                raise TypeError
+            return int(value)
+        if t == "bool":
+            if isinstance(value, bool):
+                return value
+            if value in (0, 1):
+                return bool(value)
+            raise TypeError
+        if t == "vec3":
+            if not isinstance(value, (list, tuple)) or len(value) != 3:
+                raise TypeError
+            return [float(v) for v in value]
+        if t == "str":
+            if not isinstance(value, str):
+                raise TypeError
+            if p.choices:
+                canon = {c.lower(): c for c in p.choices}
+                if value.lower() not in canon:
+                    errors.append("%s: %r is not one of %s." % (where, value, ", ".join(p.choices)))
+                    return None
+                return canon[value.lower()]
+            return value
+    except (TypeError, ValueError):
+        errors.append("%s: expected %s, got %r." % (where, t, value))
+    return None
+
+
+class SimConfig:
+    """Resolved scene.  `scene` is a path (.json / .toml) or an already parsed dict."""
+
+    def __init__(self, scene, verbose=False):
+        if isinstance(scene, dict):
+            raw = scene
+            self.path = "<dict>"
+        else:
+            self.path = scene
+            try:
+                if str(scene).lower().endswith(".toml"):
+                    with open(scene, "rb") as fh:
+                        raw = tomllib.load(fh)
+                else:
+                    with open(scene, "r", encoding="utf-8") as fh:
+                        raw = json.loads(strip_json_comments(fh.read()))
+            except Exception as e:  # noqa: BLE001 - reported as a scene error
+                raise SceneError("%s: failed to load scene: %s" % (scene, e))
+        if not isinstance(raw, dict):
+            raise SceneError("%s: the scene must be a table/object" % self.path)
+        self.raw = raw
+        errors = []
+        given = {}
+        for section, body in raw.items():
+            if section not in TOP_LEVEL:
+                errors.append("unknown top-level section %r.%s" % (section, suggest(section, TOP_LEVEL)))
+                continue
+            if section in CARDS:
+                if not isinstance(body, dict):
+                    errors.append("card %s: expected an object" % section)
+                    continue
+                fields = {p.field: p for p in PARAMS if p.card == section}
+                for key, value in body.items():
+                    p = fields.get(key)
+                    if p is None:
+                        errors.append("card %s: unknown key %r.%s" % (section, key, suggest(key, fields)))
+                        continue
+                    v = _coerce(p, value, "%s.%s" % (section, key), errors)
+                    if v is not None:
+                        given[p.name] = v
+        flat = raw.get("Configuration") or {}
+        if not isinstance(flat, dict):
+            errors.append("Configuration: expected an object")
+            flat = {}
+        for key, value in flat.items():
+            p = BY_NAME.get(key)
+            if p is None:
+                errors.append("Configuration: unknown parameter %r.%s" % (key, suggest(key, BY_NAME)))
+                continue
+            v = _coerce(p, value, "Configuration.%s" % key, errors)
+            if v is not None:
+                given[key] = v
+        derived = self._materials(raw.get("Materials"), errors)
+        resolved = {}
+        shifting_present = "Shifting" in raw or "PST_ENABLED" in flat
+        deferred = []
+        for p in PARAMS:
+            if p.name in given:
+                resolved[p.name] = given[p.name]
+            elif callable(p.default):
+                deferred.append(p)
+            elif p.name == "PST_ENABLED" and not shifting_present:
+                resolved[p.name] = False
+            else:
+                resolved[p.name] = p.default
+        for p in deferred:
+            resolved[p.name] = p.default(resolved)
+        resolved.update(derived)
+        for p in PARAMS:
+            if p.required and resolved.get(p.name) is None:
+                errors.append("%s.%s (%s) is required" % (p.card, p.field, p.name))
+        for key in ("FluidBlocks", "SolidBlocks"):
+            for k, b in enumerate(raw.get(key) or []):
+                for bk in b:
+                    if bk not in BLOCK_KEYS:
+                        errors.append("%s[%d]: unknown key %r.%s" % (key, k, bk, suggest(bk, BLOCK_KEYS)))
+                for req in ("objectId", "start", "end"):
+                    if req not in b:
+                        errors.append("%s[%d]: %s is required" % (key, k, req))
+        if resolved.get("PERIODIC"):
+            if set(resolved["PERIODIC"].lower()) - set("xyz"):
+                errors.append("Domain.periodic: only the letters x, y, z are allowed")
+        if errors:
+            raise SceneError("%s: %d problem%s in the scene:\n  - %s" % (
+                os.path.basename(str(self.path)), len(errors), "" if len(errors) == 1 else "s",
+                "\n  - ".join(errors)))
+        self.config = {"Configuration": resolved}
+        if verbose:
+            print(self.summary())
+
+    @staticmethod
+    def _materials(mats, errors):
+        """derive_globals (materials.py:551-604) for strengthless Tait / linear fluids."""
+        if not mats:
+            return {}
+        if not isinstance(mats, list):
+            errors.append("Materials: expected a list")
+            return {}
+        c0s, rhos, gam = [], [], 1.0
+        for k, m in enumerate(mats):
+            rho0 = m.get("density0")
+            eos = m.get("eos") or {}
+            kind = str(eos.get("type", "linear")).lower()
+            if rho0 is None:
+                errors.append("Materials[%d]: density0 is required" % k)
+                continue
+            if kind == "tait":
+                c0, g = eos.get("c0"), eos.get("exponent")
+                if c0 is None or g is None:
+                    errors.append("Materials[%d].eos: a Tait equation of state needs c0 and exponent." % k)
+                    continue
+                c0s.append(float(c0)); gam = float(g)
+            elif kind == "linear":
+                K = eos.get("bulkModulus")
+                if K is None:
+                    errors.append("Materials[%d].eos: a linear EOS needs bulkModulus" % k)
+                    continue
+                c0s.append((float(K) / float(rho0)) ** 0.5); gam = 1.0
+            else:
+                errors.append("Materials[%d].eos: type %r is not part of the fluid path" % (k, kind))
+                continue
+            rhos.append(float(rho0))
+        if not c0s:
+            return {}
+        out = {"c0": max(c0s), "density0": max(rhos)}
+        out["exponent"] = gam if len(mats) == 1 else 1.0
+        return out
+
+    def get_cfg(self, name, enforce_exist=False):
+        c = self.config["Configuration"]
+        if enforce_exist and name not in c:
+            raise KeyError(name)
+        return c.get(name)
+
+    def set_cfg(self, name, value):
+        self.config["Configuration"][name] = value
+
+    def get_fluid_blocks(self):
+        return self._blocks("FluidBlocks")
+
+    def get_solid_blocks(self):
+        return self._blocks("SolidBlocks")
+
+    def _blocks(self, key):
+        out = []
+        for b in self.raw.get(key) or []:
+            b = dict(b)
+            b.setdefault("translation", [0.0, 0.0, 0.0])
+            b.setdefault("scale", [1.0, 1.0, 1.0])
+            b.setdefault("velocity", [0.0, 0.0, 0.0])
+            b.setdefault("density", self.get_cfg("density0"))
+            b.setdefault("color", [0, 0, 0])
+            out.append(b)
+        return out
+
+    def get_constraints(self):
+        return list(self.raw.get("Constraints") or [])
+
+    def summary(self):
+        c = self.config["Configuration"]
+        keys = ("simulationMethod", "particleRadius", "supportRadiusFactor", "CFL", "c0",
+                "density0", "exponent", "viscosity", "DELTA_SPH", "PST_ENABLED", "PST_MODE",
+                "DOMAIN_BOUNDARY", "PERIODIC")
+        return "scene %s: " % self.path + ", ".join("%s=%r" % (k, c.get(k)) for k in keys)

@@ -1,0 +1,1239 @@
+// sigmasph.hip -- libsigmasph.so: the delta-plus-SPH particle time step for MI355X.
+//
+// Kernels (per step, in launch order):
+//   sort      k_keys -> k_scan_blocks -> k_scan_sums -> k_scan_add -> k_scatter -> k_rank
+//             -> k_permute                          (particle_system.py:1300-1345)
+//   pass A    k_pass_volume : E_i, lambda, L_i, -div v, V update, EOS    (one neighbour pass)
+//   pass B    k_pass_force  : <grad lambda>, pressure + Morris + ALE momentum,
+//                             pair sound speed and |a| maxima            (one neighbour pass)
+//   k_dt      adaptive dt (one workgroup)
+//   k_advect  step work banking + symplectic Euler
+//   pass C    k_pass_shift  : PST disorder sum and shift                 (one neighbour pass)
+//   k_finish  displace by the shift + symmetry/axis fold-back + walls + plane-strain pin
+//   k_bank    deterministic f64 totals of the step's work partials
+// The reference runs six neighbour passes for the same step (DSPH.py:1854-1883); the
+// fusion keeps every data dependency: pass B reads the pre-update volume for
+// <grad lambda> and the updated one for the forces.
+#include "sph_device.h"
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+#define SPH_VERSION "sigmasph-amd 0.1 (gfx950)"
+
+// ------------------------------------------------------------------------------------
+// device views of the state
+// ------------------------------------------------------------------------------------
+template <typename R> struct DBuf {
+    int32_t* pid; int32_t* obj;
+    double4 *x, *x0, *u, *v;
+    typename R4t<R>::type* shift;
+    R *V, *V_lo, *m, *p;
+};
+
+template <typename R> struct DState {
+    DBuf<R> b;
+    typename R4t<R>::type* acc;
+    R* lam; R* L6; typename R4t<R>::type* lam_grad; R* div; R* de_visc; R* V_new;
+    double* scal;
+    int n;
+};
+
+// constants of the fluid step, folded on the host
+struct DFluid {
+    double h, inv_h, alpha, gfac, eta2;   // gfac = 20 alpha / h^2: grad W = -gfac (1-q)^3 r
+    double rho0, c0, gamma, S, visc, delta, cfl, csig2;
+    double g[3];
+    double dx0, Wdx0, axi_rmin;
+    double pst_R, pst_max_shift, pst_ma_fixed, lam_lo, lam_hi, m_delta, tol_lam;
+    double restitution, padding, dsize[3];
+    int pst_n, allow_neg, track_visc, dt_pair, pst_en, colle, ma_mode, surf_none;
+    int clamp_disp, clamp_vel, ale_mom, walls, disp_pos, comp_vol, flags;
+};
+
+// scratch owned by the context, device pointers
+struct DScratch {
+    int* key;        // [n] bucket of particle (unsorted order)
+    int4* cellw;     // [n] cell of out-of-box particles (unsorted order)
+    int4* cellid;    // [n] cell of out-of-box particles (sorted order)
+    int* count;      // [nbuck + 1] histogram (kept zero between sorts)
+    int* start;      // [nbuck + 1] exclusive scan
+    int* cursor;     // [nbuck + 1] scatter cursors
+    int* bsum;       // [scan blocks]
+    int* tmp;        // [n] slot -> unsorted index
+    int* perm;       // [n] sorted index -> unsorted index
+    double* part;    // [SPH_NPART][nblk] per-block partials
+    double* maxv;    // [4] atomic maxima: c2, |a|, |v|
+};
+
+enum { PART_WDIFF = 0, PART_LEFT = 1, PART_VISC = 2, PART_WALL = 3, PART_R0 = 4, PART_R1 = 5,
+       PART_R2 = 6, SPH_NPART = 7 };
+enum { MAX_C2 = 0, MAX_ACC = 1, MAX_V = 2 };
+
+// ------------------------------------------------------------------------------------
+// sort kernels
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SPH_BLOCK) void k_keys(DGrid g, int n, const double4* __restrict__ X,
+                                                    const void* __restrict__ mass, int mass_is_f64,
+                                                    DScratch s, double* scal) {
+    __shared__ double red[SPH_BLOCK / SPH_WAVE];
+    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    double noff = 0.0, moff = 0.0, nout = 0.0;
+    if (i < n) {
+        double4 x = X[i];
+        int key;
+        if (!finite3(x.x, x.y, x.z)) {
+            key = g.off_bucket;
+            noff = 1.0;
+            moff = mass_is_f64 ? ((const double*)mass)[i] : (double)((const float*)mass)[i];
+        } else {
+            int cx = cell1(g, 0, x.x), cy = cell1(g, 1, x.y), cz = cell1(g, 2, x.z);
+            if (in_box(g, cx, cy, cz)) {
+                key = dense_key(g, cx, cy, cz);
+            } else {
+                key = g.nbox + (int)hash_cell(cx, cy, cz, g.hash_bits);
+                s.cellw[i] = make_int4(cx, cy, cz, 0);
+                nout = 1.0;
+            }
+        }
+        s.key[i] = key;
+        atomicAdd(&s.count[key], 1);
+    }
+    double a = block_sum(noff, red), b = block_sum(moff, red), c = block_sum(nout, red);
+    if (threadIdx.x == 0) {
+        if (a > 0.0) { atomicAdd(&scal[SPH_S_OFF_GRID_NUM], a); atomicAdd(&scal[SPH_S_OFF_GRID_MASS], b); }
+        if (c > 0.0) atomicAdd(&scal[SPH_S_OUT_OF_BOX], c);
+    }
+}
+
+__global__ void k_zero_scal(double* scal) {
+    if (threadIdx.x == 0) {
+        scal[SPH_S_OFF_GRID_NUM] = 0.0;
+        scal[SPH_S_OFF_GRID_MASS] = 0.0;
+        scal[SPH_S_OUT_OF_BOX] = 0.0;
+    }
+}
+
+#define SCAN_PER_THREAD 16
+#define SCAN_TILE (SPH_BLOCK * SCAN_PER_THREAD)
+
+// block-local exclusive scan of count[] into start[]; zeroes count for the next sort
+__global__ __launch_bounds__(SPH_BLOCK) void k_scan_blocks(int total, int* count, int* start, int* bsum) {
+    __shared__ int tile[SCAN_TILE];
+    __shared__ int wsum[SPH_BLOCK / SPH_WAVE];
+    int base = blockIdx.x * SCAN_TILE;
+    for (int k = threadIdx.x; k < SCAN_TILE; k += SPH_BLOCK) {
+        int idx = base + k;
+        int c = 0;
+        if (idx < total) { c = count[idx]; count[idx] = 0; }
+        tile[k] = c;
+    }
+    __syncthreads();
+    int t0 = threadIdx.x * SCAN_PER_THREAD;
+    int run = 0;
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) run += tile[t0 + k];
+    // inclusive wave scan of the per-thread totals
+    int lane = threadIdx.x % SPH_WAVE, w = threadIdx.x / SPH_WAVE;
+    int inc = run;
+#pragma unroll
+    for (int o = 1; o < SPH_WAVE; o <<= 1) {
+        int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane == SPH_WAVE - 1) wsum[w] = inc;
+    __syncthreads();
+    int woff = 0;
+    for (int k = 0; k < w; ++k) woff += wsum[k];
+    int excl = woff + inc - run;
+    int acc = excl;
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+        int c = tile[t0 + k];
+        tile[t0 + k] = acc;
+        acc += c;
+    }
+    if (threadIdx.x == SPH_BLOCK - 1) bsum[blockIdx.x] = acc;
+    __syncthreads();
+    for (int k = threadIdx.x; k < SCAN_TILE; k += SPH_BLOCK) {
+        int idx = base + k;
+        if (idx < total) start[idx] = tile[k];
+    }
+}
+
+// exclusive scan of the block sums, one workgroup
+__global__ __launch_bounds__(1024) void k_scan_sums(int nb, int* bsum) {
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += 1024) {
+        int idx = base + threadIdx.x;
+        int v = idx < nb ? bsum[idx] : 0;
+        int lane = threadIdx.x % SPH_WAVE, w = threadIdx.x / SPH_WAVE;
+        int inc = v;
+#pragma unroll
+        for (int o = 1; o < SPH_WAVE; o <<= 1) {
+            int y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        if (lane == SPH_WAVE - 1) wsum[w] = inc;
+        __syncthreads();
+        int woff = 0;
+        for (int k = 0; k < w; ++k) woff += wsum[k];
+        int c0 = carry;
+        if (idx < nb) bsum[idx] = c0 + woff + inc - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = c0 + woff + inc;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(SPH_BLOCK) void k_scan_add(int total, int* start, int* cursor, const int* bsum) {
+    int idx = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    if (idx >= total) return;
+    int v = start[idx] + bsum[idx / SCAN_TILE];
+    start[idx] = v;
+    cursor[idx] = v;
+}
+
+__global__ __launch_bounds__(SPH_BLOCK) void k_scatter(int n, DScratch s) {
+    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    int slot = atomicAdd(&s.cursor[s.key[i]], 1);
+    s.tmp[slot] = i;
+}
+
+// order each bucket by ascending stable id: rank of slot = #{bucket members with smaller pid}
+__global__ __launch_bounds__(SPH_BLOCK) void k_rank(int n, DScratch s, const int32_t* __restrict__ pid) {
+    int t = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    if (t >= n) return;
+    int i = s.tmp[t];
+    int b = s.key[i];
+    int lo = s.start[b], hi = s.start[b + 1];
+    int my = pid[i];
+    int rank = 0;
+    for (int q = lo; q < hi; ++q) {
+        int iq = s.tmp[q];
+        int pq = pid[iq];
+        rank += (pq < my) || (pq == my && iq < i);   // unique ranks even for equal ids
+    }
+    s.perm[lo + rank] = i;
+}
+
+struct PermField { const void* src; void* dst; int bytes; };
+struct PermTable { PermField f[16]; int nf; };
+
+__global__ __launch_bounds__(SPH_BLOCK) void k_permute(int n, PermTable T, DScratch s, int off_bucket_lo) {
+    int k = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    if (k >= n) return;
+    int src = s.perm[k];
+    for (int f = 0; f < T.nf; ++f) {
+        const PermField& F = T.f[f];
+        switch (F.bytes) {
+        case 4: ((int*)F.dst)[k] = ((const int*)F.src)[src]; break;
+        case 8: ((double*)F.dst)[k] = ((const double*)F.src)[src]; break;
+        case 16: ((float4*)F.dst)[k] = ((const float4*)F.src)[src]; break;
+        case 32: ((double4*)F.dst)[k] = ((const double4*)F.src)[src]; break;
+        default: break;
+        }
+    }
+    if (s.key[src] >= off_bucket_lo) s.cellid[k] = s.cellw[src];
+}
+
+// ------------------------------------------------------------------------------------
+// small helpers for the pair passes
+// ------------------------------------------------------------------------------------
+template <typename R> __device__ __forceinline__ R wq(const DFluid& F, const DBuf<R>& b, const R* V, int p) {
+    // quadrature weight w (particle_system.py:990-995): V, or V / r in axisymmetry
+    R w = V[p];
+    if (F.flags & SPH_AXISYMMETRIC) w = w / (R)fmax(b.x[p].y, F.axi_rmin);
+    return w;
+}
+template <typename R> __device__ __forceinline__ R mwq(const DFluid& F, const DBuf<R>& b, int p) {
+    R m = b.m[p];
+    if (F.flags & SPH_AXISYMMETRIC) m = m / (R)fmax(b.x[p].y, F.axi_rmin);
+    return m;
+}
+
+template <typename R> __device__ __forceinline__ R tait(const DFluid& F, R rho) {
+    R p = (R)F.S * (pow(rho / (R)F.rho0, (R)F.gamma) - (R)1);
+    if (!F.allow_neg) p = p > (R)0 ? p : (R)0;
+    return p;
+}
+
+// grad W = c * r with c = -20 alpha / h^2 (1 - q)^3 for q < 1 and |r| > 1e-5
+template <typename R> __device__ __forceinline__ R gradw_c(const DFluid& F, R rn) {
+    R q = rn * (R)F.inv_h;
+    R c = (R)0;
+    if (q < (R)1 && rn > (R)1e-5) {
+        R om = (R)1 - q;
+        c = -(R)F.gfac * om * om * om;
+    }
+    return c;
+}
+
+// ------------------------------------------------------------------------------------
+// pass A: E_i (compute_L_task) + continuity (compute_depsvol_dt_task) per pair;
+//         lambda, L_i, -div v, the volume update and the EOS per particle
+// ------------------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(SPH_BLOCK) void k_pass_volume(DGrid g, DFluid F, DState<R> st, DScratch s) {
+    __shared__ double red[SPH_BLOCK / SPH_WAVE];
+    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    double wdiff = 0.0;
+    if (i < st.n) {
+        const DBuf<R>& b = st.b;
+        const double dt = st.scal[SPH_S_DT];
+        const R inv_dt = (R)(1.0 / dt);
+        double4 xi = b.x[i];
+        double4 vi4 = b.v[i];
+        R vix = (R)vi4.x, viy = (R)vi4.y, viz = (R)vi4.z;
+        R rho_i = b.m[i] / b.V[i];
+        R duix = 0, duiy = 0, duiz = 0;
+        if (F.pst_en) {
+            typename R4t<R>::type sh = b.shift[i];
+            duix = sh.x * inv_dt; duiy = sh.y * inv_dt; duiz = sh.z * inv_dt;
+            vix += duix; viy += duiy; viz += duiz;
+        }
+        const bool two_d = (F.flags & SPH_TWO_D) != 0;
+        R e00 = 0, e11 = 0, e22 = 0, e01 = 0, e02 = 0, e12 = 0;
+        R rate = 0, drate = 0;
+        const R dhc = (R)(F.delta * F.h * F.c0);
+        const R eta2 = (R)F.eta2;
+        auto task = [&](int j, int mir, double rxd, double ryd, double rzd) {
+            R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
+            R r2 = rx * rx + ry * ry + rz * rz;
+            R rn = sqrt(r2);
+            R c = gradw_c<R>(F, rn);
+            R wj = wq<R>(F, b, b.V, j);
+            // E_i += w_j (x_j - x_i) (x) grad W = -w_j c r (x) r
+            R sc = -wj * c;
+            e00 += sc * rx * rx; e11 += sc * ry * ry; e01 += sc * rx * ry;
+            if (!two_d) { e22 += sc * rz * rz; e02 += sc * rx * rz; e12 += sc * ry * rz; }
+            R gx = c * rx, gy = c * ry, gz = c * rz;
+            R rho_j = b.m[j] / b.V[j];
+            double4 vj4 = b.v[j];
+            R vjx = (R)vj4.x, vjy = (R)vj4.y, vjz = (R)vj4.z;
+            if (mir & 1) vjy = -vjy;
+            if (mir & 2) vjz = -vjz;
+            if (F.pst_en) {
+                typename R4t<R>::type sh = b.shift[j];
+                R djx = sh.x * inv_dt, djy = sh.y * inv_dt, djz = sh.z * inv_dt;
+                if (mir & 1) djy = -djy;
+                if (mir & 2) djz = -djz;
+                vjx += djx; vjy += djy; vjz += djz;
+                R fl = (rho_j * djx - rho_i * duix) * gx + (rho_j * djy - rho_i * duiy) * gy +
+                       (rho_j * djz - rho_i * duiz) * gz;
+                rate += wj * fl / rho_i;
+            }
+            rate += wj * ((vix - vjx) * gx + (viy - vjy) * gy + (viz - vjz) * gz);
+            if (F.delta > 0.0) {
+                // psi_ij . grad W with psi = 2 (rho_j - rho_i)(-r)/(r^2 + eta^2) and grad W = c r
+                R pg = (R)2 * (rho_j - rho_i) * (-c * r2) / (r2 + eta2);
+                R dr = dhc * wj * pg / rho_i;
+                rate += dr;
+                drate += dr;
+            }
+        };
+        Traverse tr{g, s.start, s.cellid, b.x, st.scal[SPH_S_OUT_OF_BOX] > 0.5};
+        tr(i, xi.x, xi.y, xi.z, task);
+
+        // lambda and L (DSPH.py:639-689)
+        double L6[6], lam;
+        if (two_d) {
+            double a = e00, bb = e01, d = e11;
+            double det = sqrt((a - d) * (a - d) + 4.0 * bb * bb);
+            double ev2 = 0.5 * (a + d - det);
+            lam = fmin(1.0, fmax(0.0, ev2));
+            spd_inverse2(e00, e01, e01, e11, F.tol_lam, L6);
+        } else {
+            double A6[6] = {(double)e00, (double)e11, (double)e22, (double)e01, (double)e02, (double)e12};
+            double me;
+            spd_inverse3(A6, F.tol_lam, L6, &me);
+            lam = fmin(1.0, fmax(0.0, me));
+        }
+        st.lam[i] = (R)lam;
+        R* Lo = st.L6 + 8 * (size_t)i;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) Lo[k] = (R)L6[k];
+
+        // -div v, with the hoop rate in axisymmetry (DSPH.py:1454-1476)
+        R dv = rate;
+        if (F.flags & SPH_AXISYMMETRIC) dv -= (R)(vi4.y / fmax(xi.y, F.axi_rmin));
+        st.div[i] = dv;
+        wdiff = -(double)b.V[i] * (double)b.p[i] * (double)drate;
+
+        // volume update (DSPH.py:1479-1494)
+        R Vi = b.V[i];
+        R dVdt = -Vi * dv;
+        R Vn;
+        if (sizeof(R) == 4 && F.comp_vol) {
+            double sum = (double)Vi + (double)b.V_lo[i] + (double)((R)dt * dVdt);
+            Vn = (R)sum;
+            b.V_lo[i] = (R)(sum - (double)Vn);
+        } else {
+            Vn = Vi + (R)dt * dVdt;
+        }
+        st.V_new[i] = Vn;
+        // EOS on the updated density (DSPH.py:1514-1516); p_i is read by no other
+        // particle in this pass, only by its own diffusion work above
+        b.p[i] = tait<R>(F, b.m[i] / Vn);
+    }
+    double bs = block_sum(wdiff, red);
+    if (threadIdx.x == 0) s.part[PART_WDIFF * gridDim.x + blockIdx.x] = bs;
+}
+
+// ------------------------------------------------------------------------------------
+// pass B: <grad lambda> (compute_lam_grad_task) + compute_pressure_accel_task +
+//         compute_dt_task per pair; acceleration and the dt maxima per particle
+// ------------------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(SPH_BLOCK) void k_pass_force(DGrid g, DFluid F, DState<R> st, DScratch s) {
+    __shared__ double red[SPH_BLOCK / SPH_WAVE];
+    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    double c2loc = F.csig2, amag = 0.0;
+    if (i < st.n) {
+        const DBuf<R>& b = st.b;
+        const R* Vn = st.V_new;
+        const double dt = st.scal[SPH_S_DT];
+        const R inv_dt = (R)(1.0 / dt);
+        double4 xi = b.x[i];
+        double4 vi4 = b.v[i];
+        R vix = (R)vi4.x, viy = (R)vi4.y, viz = (R)vi4.z;
+        R rho_i = b.m[i] / Vn[i];
+        R p_i = b.p[i];
+        R lam_i = st.lam[i];
+        R glx = 0, gly = 0, glz = 0;    // Sum w_old (lam_j - lam_i) grad W
+        R apx = 0, apy = 0, apz = 0;    // Sum w_j (p_i + p_j) grad W
+        R avx = 0, avy = 0, avz = 0;    // Sum w_j (v_ij . r)/(r^2 + eta^2) grad W
+        R aax = 0, aay = 0, aaz = 0;    // Sum w_j rho_j (du_j . grad W)(v_j - v_i)
+        R c2max = (R)F.csig2;
+        const R eta2 = (R)F.eta2;
+        const R drho_eps = (R)(1e-3 * F.rho0);
+        const bool ale = F.pst_en && F.ale_mom;
+        auto task = [&](int j, int mir, double rxd, double ryd, double rzd) {
+            R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
+            R r2 = rx * rx + ry * ry + rz * rz;
+            R rn = sqrt(r2);
+            R c = gradw_c<R>(F, rn);
+            R gx = c * rx, gy = c * ry, gz = c * rz;
+            R wo = wq<R>(F, b, b.V, j);
+            R wj = wq<R>(F, b, Vn, j);
+            R dl = st.lam[j] - lam_i;
+            glx += wo * dl * gx; gly += wo * dl * gy; glz += wo * dl * gz;
+            R p_j = b.p[j];
+            R pp = wj * (p_i + p_j);
+            apx += pp * gx; apy += pp * gy; apz += pp * gz;
+            double4 vj4 = b.v[j];
+            R vjx = (R)vj4.x, vjy = (R)vj4.y, vjz = (R)vj4.z;
+            if (mir & 1) vjy = -vjy;
+            if (mir & 2) vjz = -vjz;
+            if (F.visc > 0.0) {
+                R vr = (vix - vjx) * rx + (viy - vjy) * ry + (viz - vjz) * rz;
+                R fv = wj * vr / (r2 + eta2);
+                avx += fv * gx; avy += fv * gy; avz += fv * gz;
+            }
+            R rho_j = b.m[j] / Vn[j];
+            if (ale) {
+                typename R4t<R>::type sh = b.shift[j];
+                R djx = sh.x * inv_dt, djy = sh.y * inv_dt, djz = sh.z * inv_dt;
+                if (mir & 1) djy = -djy;
+                if (mir & 2) djz = -djz;
+                R t = wj * rho_j * (djx * gx + djy * gy + djz * gz);
+                aax += t * (vjx - vix); aay += t * (vjy - viy); aaz += t * (vjz - viz);
+            }
+            if (F.dt_pair) {
+                R drho = fabs(rho_i - rho_j);
+                if (drho > drho_eps) {
+                    R c2 = fabs(p_i - p_j) / drho;
+                    c2max = c2 > c2max ? c2 : c2max;
+                }
+            }
+        };
+        Traverse tr{g, s.start, s.cellid, b.x, st.scal[SPH_S_OUT_OF_BOX] > 0.5};
+        tr(i, xi.x, xi.y, xi.z, task);
+
+        // <grad lambda> = L_i Sum(...) (DSPH.py:690-702)
+        const R* Li = st.L6 + 8 * (size_t)i;
+        R lgx = Li[0] * glx + Li[3] * gly + Li[4] * glz;
+        R lgy = Li[3] * glx + Li[1] * gly + Li[5] * glz;
+        R lgz = Li[4] * glx + Li[5] * gly + Li[2] * glz;
+        if (F.flags & SPH_TWO_D) lgz = 0;
+        typename R4t<R>::type lg; lg.x = lgx; lg.y = lgy; lg.z = lgz; lg.w = 0;
+        st.lam_grad[i] = lg;
+
+        // acceleration: g + pressure + Morris (d = 2(3 + 2)) + ALE
+        R wi = wq<R>(F, b, Vn, i), mwi = mwq<R>(F, b, i);
+        R kp = -wi / mwi;
+        R kv = (R)(10.0 * F.visc);
+        R ax = (R)F.g[0] + kp * apx + kv * avx;
+        R ay = (R)F.g[1] + kp * apy + kv * avy;
+        R az = (R)F.g[2] + kp * apz + kv * avz;
+        if (ale) { ax += aax / rho_i; ay += aay / rho_i; az += aaz / rho_i; }
+        typename R4t<R>::type a4; a4.x = ax; a4.y = ay; a4.z = az; a4.w = 0;
+        st.acc[i] = a4;
+        if (F.track_visc) st.de_visc[i] = vix * kv * avx + viy * kv * avy + viz * kv * avz;
+        c2loc = (double)c2max;
+        if (finite3(xi.x, xi.y, xi.z)) amag = sqrt((double)ax * ax + (double)ay * ay + (double)az * az);
+    }
+    double bc = block_max(c2loc, red);
+    double ba = block_max(amag, red);
+    if (threadIdx.x == 0) {
+        atomic_max_pos(&s.maxv[MAX_C2], bc);
+        atomic_max_pos(&s.maxv[MAX_ACC], ba);
+    }
+}
+
+// adaptive dt (DSPH.py:1418-1430) and banking of the diffusion work (1495)
+__global__ __launch_bounds__(1024) void k_dt(DFluid F, double* scal, DScratch s, int nblk) {
+    __shared__ double red[1024 / SPH_WAVE];
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < nblk; k += blockDim.x) acc += s.part[PART_WDIFF * nblk + k];
+    double w = block_sum(acc, red);
+    if (threadIdx.x == 0) {
+        double dt_prev = scal[SPH_S_DT];
+        scal[SPH_S_DIFF_WORK] += dt_prev * w;
+        double c2 = fmax(s.maxv[MAX_C2], F.csig2);
+        double am = fmax(s.maxv[MAX_ACC], 1e-10);
+        double dt_ac = F.cfl * F.h / sqrt(c2);
+        double dt_f = F.cfl * sqrt(F.h / am);
+        scal[SPH_S_DT] = fmin(dt_ac, dt_f);
+        scal[SPH_S_MAX_C2] = c2;
+        scal[SPH_S_MAX_ACC] = am;
+        s.maxv[MAX_C2] = 0.0;
+        s.maxv[MAX_ACC] = 0.0;
+    }
+}
+
+__device__ __forceinline__ double wrap_pos(const DGrid& g, int k, double x) {
+    if (g.periodic & (1 << k)) x = x - g.period[k] * floor((x - g.origin[k]) / g.period[k]);
+    return x;
+}
+
+// accumulate_step_work (DSPH.py:1714-1763) + advect (1652-1665) + max |v| (712-723)
+template <typename R>
+__global__ __launch_bounds__(SPH_BLOCK) void k_advect(DGrid g, DFluid F, DState<R> st, DScratch s) {
+    __shared__ double red[SPH_BLOCK / SPH_WAVE];
+    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    double wl = 0.0, wv = 0.0, vmax = 0.0;
+    if (i < st.n) {
+        const DBuf<R>& b = st.b;
+        double dt = st.scal[SPH_S_DT];
+        typename R4t<R>::type a = st.acc[i];
+        double ax = a.x, ay = a.y, az = a.z;
+        double m = b.m[i];
+        double fx = ax - F.g[0], fy = ay - F.g[1], fz = az - F.g[2];
+        double g2 = F.g[0] * F.g[0] + F.g[1] * F.g[1] + F.g[2] * F.g[2];
+        wl = 0.5 * m * dt * dt * (fx * fx + fy * fy + fz * fz - g2);
+        if (F.track_visc) wv = m * (double)st.de_visc[i];
+        double4 v = b.v[i];
+        v.x = fma(dt, ax, v.x); v.y = fma(dt, ay, v.y); v.z = fma(dt, az, v.z);
+        b.v[i] = v;
+        double4 x = b.x[i];
+        if (F.disp_pos) {
+            double4 u = b.u[i], x0 = b.x0[i];
+            u.x = fma(dt, v.x, u.x); u.y = fma(dt, v.y, u.y); u.z = fma(dt, v.z, u.z);
+            b.u[i] = u;
+            x.x = x0.x + u.x; x.y = x0.y + u.y; x.z = x0.z + u.z;
+        } else {
+            x.x += dt * v.x; x.y += dt * v.y; x.z += dt * v.z;
+        }
+        x.x = wrap_pos(g, 0, x.x); x.y = wrap_pos(g, 1, x.y); x.z = wrap_pos(g, 2, x.z);
+        b.x[i] = x;
+        b.V[i] = st.V_new[i];
+        if (F.ma_mode == 1 && finite3(x.x, x.y, x.z)) vmax = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    }
+    double s1 = block_sum(wl, red);
+    double s2 = block_sum(wv, red);
+    if (threadIdx.x == 0) {
+        s.part[PART_LEFT * gridDim.x + blockIdx.x] = s1;
+        s.part[PART_VISC * gridDim.x + blockIdx.x] = s2;
+    }
+    if (F.ma_mode == 1) {
+        double bm = block_max(vmax, red);
+        if (threadIdx.x == 0) atomic_max_pos(&s.maxv[MAX_V], bm);
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// pass C: apply_pst first pass (DSPH.py:975-1043) over the advected positions
+// ------------------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(SPH_BLOCK) void k_pass_shift(DGrid g, DFluid F, DState<R> st, DScratch s) {
+    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    if (i >= st.n) return;
+    const DBuf<R>& b = st.b;
+    double4 xi = b.x[i];
+    R rho_i = b.m[i] / b.V[i];
+    R sx = 0, sy = 0, sz = 0;
+    const R inv_wdx0 = F.Wdx0 > 1e-10 ? (R)(1.0 / F.Wdx0) : (R)0;
+    const R alpha = (R)F.alpha;
+    auto task = [&](int j, int mir, double rxd, double ryd, double rzd) {
+        R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
+        R r2 = rx * rx + ry * ry + rz * rz;
+        R rn = sqrt(r2);
+        R c = gradw_c<R>(F, rn);
+        R q = rn * (R)F.inv_h;
+        R W = 0;
+        if (q < (R)1) { R om = (R)1 - q; W = alpha * om * om * om * om * ((R)4 * q + (R)1); }
+        R f = 0;
+        if (F.Wdx0 > 1e-10) {
+            R t = W * inv_wdx0, tp = 1;
+            for (int k = 0; k < F.pst_n; ++k) tp *= t;
+            f = (R)F.pst_R * tp;
+        }
+        R wgt = 0;
+        if (F.colle) {
+            R Vj = wq<R>(F, b, b.V, j);
+            if (Vj > (R)1e-30) wgt = Vj;
+        } else {
+            R mj = mwq<R>(F, b, j);
+            if (mj > (R)1e-10) {
+                R rho_j = b.m[j] / b.V[j];
+                R ra = (R)0.5 * (rho_i + rho_j);
+                if (ra > (R)1e-10) wgt = mj / ra;
+            }
+        }
+        R k = ((R)1 + f) * c * wgt;
+        sx += k * rx; sy += k * ry; sz += k * rz;
+    };
+    Traverse tr{g, s.start, s.cellid, b.x, st.scal[SPH_S_OUT_OF_BOX] > 0.5};
+    tr(i, xi.x, xi.y, xi.z, task);
+
+    double4 v = b.v[i];
+    double vn = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    double Ma;
+    if (F.ma_mode == 2) Ma = F.pst_ma_fixed;
+    else if (F.ma_mode == 1) Ma = st.scal[SPH_S_MAX_V] / F.c0;
+    else Ma = vn / F.c0;
+    double drx, dry, drz;
+    if (F.colle) {
+        double kk = -2.0 * F.h * F.c0 * Ma;
+        double dvx = kk * sx, dvy = kk * sy, dvz = kk * sz;
+        if (F.clamp_vel) {
+            double cap = F.m_delta * vn;
+            double dn = sqrt(dvx * dvx + dvy * dvy + dvz * dvz);
+            if (dn > cap) { double r = cap / fmax(dn, 1e-30); dvx *= r; dvy *= r; dvz *= r; }
+        }
+        double dt = st.scal[SPH_S_DT];
+        drx = dvx * dt; dry = dvy * dt; drz = dvz * dt;
+    } else {
+        double kk = -F.cfl * Ma * F.h * F.h;
+        drx = kk * sx; dry = kk * sy; drz = kk * sz;
+    }
+    if (F.clamp_disp) {
+        double mx = F.pst_max_shift * F.dx0;
+        double dn = sqrt(drx * drx + dry * dry + drz * drz);
+        if (dn > mx) { double r = mx / dn; drx *= r; dry *= r; drz *= r; }
+    }
+    // pst_constrain (DSPH.py:888-941)
+    double lam = st.lam[i];
+    if (F.surf_none) {
+        if (lam < F.lam_hi) drx = dry = drz = 0.0;
+    } else if (lam < F.lam_hi) {
+        if (lam < F.lam_lo) {
+            drx = dry = drz = 0.0;
+        } else {
+            typename R4t<R>::type ng = st.lam_grad[i];
+            double nx = ng.x, ny = ng.y, nz = (F.flags & SPH_TWO_D) ? 0.0 : (double)ng.z;
+            double nn = sqrt(nx * nx + ny * ny + nz * nz);
+            if (nn > 1e-10) {
+                nx /= nn; ny /= nn; nz /= nn;
+                double d = drx * nx + dry * ny + drz * nz;
+                drx -= d * nx; dry -= d * ny; drz -= d * nz;
+            }
+        }
+    }
+    if (F.flags & SPH_TWO_D) drz = 0.0;
+    typename R4t<R>::type o; o.x = (R)drx; o.y = (R)dry; o.z = (R)drz; o.w = 0;
+    b.shift[i] = o;
+}
+
+// ------------------------------------------------------------------------------------
+// apply_pst second pass (DSPH.py:1045-1049) and the boundary kernels (sph_base.py:622-655)
+// ------------------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(SPH_BLOCK) void k_finish(DGrid g, DFluid F, DState<R> st, DScratch s) {
+    __shared__ double red[SPH_BLOCK / SPH_WAVE];
+    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    double ww = 0.0;
+    if (i < st.n) {
+        const DBuf<R>& b = st.b;
+        double4 x = b.x[i], u = b.u[i], x0 = b.x0[i], v = b.v[i];
+        typename R4t<R>::type a = st.acc[i];
+        bool a_changed = false;
+        double xs[3] = {x.x, x.y, x.z}, us[3] = {u.x, u.y, u.z}, x0s[3] = {x0.x, x0.y, x0.z};
+        double vs[3] = {v.x, v.y, v.z};
+        double as[3] = {(double)a.x, (double)a.y, (double)a.z};
+        if (F.pst_en) {
+            typename R4t<R>::type sh = b.shift[i];
+            double d[3] = {(double)sh.x, (double)sh.y, (double)sh.z};
+            for (int k = 0; k < 3; ++k) {
+                if (F.disp_pos) { us[k] += d[k]; xs[k] = x0s[k] + us[k]; }
+                else xs[k] += d[k];
+                xs[k] = wrap_pos(g, k, xs[k]);
+            }
+        }
+        const bool axi = (F.flags & SPH_AXISYMMETRIC) != 0;
+        const bool two_d = (F.flags & SPH_TWO_D) != 0;
+        auto place = [&](int k, double val) { xs[k] = val; if (F.disp_pos) us[k] = val - x0s[k]; };
+        auto reflect = [&](int k) {
+            xs[k] = -xs[k];
+            if (F.disp_pos) us[k] = -us[k] - 2.0 * x0s[k];
+            vs[k] = -vs[k]; as[k] = -as[k]; a_changed = true;
+        };
+        if (axi) {
+            vs[2] = 0.0; as[2] = 0.0; a_changed = true;
+            place(2, x0s[2]);
+            if (xs[1] < 0.0) reflect(1);
+        } else if (!two_d) {
+            if ((F.flags & SPH_MIRROR_Y) && xs[1] < 0.0) reflect(1);
+            if ((F.flags & SPH_MIRROR_Z) && xs[2] < 0.0) reflect(2);
+        }
+        if (F.walls) {
+            int nd = two_d ? 2 : 3;
+            double pos[3] = {xs[0], xs[1], xs[2]};
+            double nr[3] = {0, 0, 0};
+            for (int k = 0; k < nd; ++k) {
+                double hi = F.dsize[k] - F.padding;
+                if (pos[k] > hi) { nr[k] += 1.0; place(k, hi); }
+                if (axi && k == 1) continue;
+                if (pos[k] <= F.padding) { nr[k] -= 1.0; place(k, F.padding); }
+            }
+            double nl = sqrt(nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2]);
+            if (nl > 1e-6) {
+                double nx = nr[0] / nl, ny = nr[1] / nl, nz = nr[2] / nl;
+                double vnn = vs[0] * nx + vs[1] * ny + vs[2] * nz;
+                if (vnn > 0.0) {
+                    ww = 0.5 * (double)b.m[i] * vnn * vnn * (1.0 - F.restitution * F.restitution);
+                    double kq = (1.0 + F.restitution) * vnn;
+                    vs[0] -= kq * nx; vs[1] -= kq * ny; vs[2] -= kq * nz;
+                }
+            }
+        }
+        if (!axi && (F.flags & SPH_PLANE_STRAIN)) {
+            vs[2] = 0.0; as[2] = 0.0; a_changed = true;
+            place(2, x0s[2]);
+        }
+        b.x[i] = make_double4(xs[0], xs[1], xs[2], 0.0);
+        if (F.disp_pos) b.u[i] = make_double4(us[0], us[1], us[2], 0.0);
+        b.v[i] = make_double4(vs[0], vs[1], vs[2], 0.0);
+        if (a_changed) {
+            typename R4t<R>::type ao; ao.x = (R)as[0]; ao.y = (R)as[1]; ao.z = (R)as[2]; ao.w = 0;
+            st.acc[i] = ao;
+        }
+    }
+    double bs = block_sum(ww, red);
+    if (threadIdx.x == 0) s.part[PART_WALL * gridDim.x + blockIdx.x] = bs;
+}
+
+__global__ __launch_bounds__(1024) void k_bank(double* scal, DScratch s, int nblk, int ma_global) {
+    __shared__ double red[1024 / SPH_WAVE];
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int k = threadIdx.x; k < nblk; k += blockDim.x) {
+        a += s.part[PART_LEFT * nblk + k];
+        b += s.part[PART_VISC * nblk + k];
+        c += s.part[PART_WALL * nblk + k];
+    }
+    double sa = block_sum(a, red), sb = block_sum(b, red), sc = block_sum(c, red);
+    if (threadIdx.x == 0) {
+        double dt = scal[SPH_S_DT];
+        scal[SPH_S_LEFTOVER] += sa;
+        scal[SPH_S_VISC_WORK] += dt * sb;
+        scal[SPH_S_WALL_WORK] += sc;
+        if (ma_global) s.maxv[MAX_V] = 0.0;
+    }
+}
+
+// max |v| is needed before pass C: publish it into the scalar slot
+__global__ void k_publish_vmax(double* scal, DScratch s) {
+    if (threadIdx.x == 0) scal[SPH_S_MAX_V] = s.maxv[MAX_V];
+}
+
+// ------------------------------------------------------------------------------------
+// neighbour pairs for the bit-exact test
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SPH_BLOCK) void k_pairs(DGrid g, int n, const double4* __restrict__ X,
+                                                     const int32_t* __restrict__ pid, DScratch s, const double* scal,
+                                                     int32_t* out, long long cap, unsigned long long* cnt) {
+    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double4 xi = X[i];
+    int pi = pid[i];
+    auto task = [&](int j, int mir, double, double, double) {
+        unsigned long long k = atomicAdd(cnt, 1ull);
+        if ((long long)k < cap) {
+            out[3 * k + 0] = pi;
+            out[3 * k + 1] = pid[j];
+            out[3 * k + 2] = mir;
+        }
+    };
+    Traverse tr{g, s.start, s.cellid, X, scal[SPH_S_OUT_OF_BOX] > 0.5};
+    tr(i, xi.x, xi.y, xi.z, task);
+}
+
+// ------------------------------------------------------------------------------------
+// f64 diagnostics
+// ------------------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(SPH_BLOCK) void k_reduce(int n, DBuf<R> b, int which, double g0, double g1, double g2,
+                                                      DFluid F, DScratch s) {
+    __shared__ double red[SPH_BLOCK / SPH_WAVE];
+    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    double r0 = 0, r1 = 0, r2 = 0;
+    if (i < n) {
+        double4 v = b.v[i], x = b.x[i];
+        double m = (double)b.m[i];
+        double v2 = v.x * v.x + v.y * v.y + v.z * v.z;
+        switch (which) {
+        case SPH_R_KINETIC: r0 = 0.5 * m * v2; break;
+        case SPH_R_KINETIC_OFFGRID: if (!finite3(x.x, x.y, x.z)) r0 = 0.5 * m * v2; break;
+        case SPH_R_POTENTIAL: r0 = -m * (g0 * x.x + g1 * x.y + g2 * x.z); break;
+        case SPH_R_MASS: r0 = m; break;
+        case SPH_R_MOMENTUM: r0 = m * v.x; r1 = m * v.y; r2 = m * v.z; break;
+        case SPH_R_GROSS_MOMENTUM: r0 = m * sqrt(v2); break;
+        case SPH_R_TAIT_ENERGY: {
+            double rho = m / (double)b.V[i];
+            double xr = rho / F.rho0, e;
+            if (fabs(F.gamma - 1.0) < 1e-6) e = (F.S / F.rho0) * (log(xr) + 1.0 / xr - 1.0);
+            else e = (F.S / F.rho0) * ((pow(xr, F.gamma - 1.0) - 1.0) / (F.gamma - 1.0) + 1.0 / xr - 1.0);
+            if (!F.allow_neg && rho < F.rho0) e = 0.0;
+            r0 = m * e;
+        } break;
+        default: break;
+        }
+    }
+    double a = block_sum(r0, red), bb = block_sum(r1, red), c = block_sum(r2, red);
+    if (threadIdx.x == 0) {
+        s.part[PART_R0 * gridDim.x + blockIdx.x] = a;
+        s.part[PART_R1 * gridDim.x + blockIdx.x] = bb;
+        s.part[PART_R2 * gridDim.x + blockIdx.x] = c;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_reduce_final(DScratch s, int nblk, double* out) {
+    __shared__ double red[1024 / SPH_WAVE];
+    double a = 0, b = 0, c = 0;
+    for (int k = threadIdx.x; k < nblk; k += blockDim.x) {
+        a += s.part[PART_R0 * nblk + k];
+        b += s.part[PART_R1 * nblk + k];
+        c += s.part[PART_R2 * nblk + k];
+    }
+    double sa = block_sum(a, red), sb = block_sum(b, red), sc = block_sum(c, red);
+    if (threadIdx.x == 0) { out[0] = sa; out[1] = sb; out[2] = sc; }
+}
+
+// ====================================================================================
+// host side
+// ====================================================================================
+struct TimerKind {
+    std::vector<hipEvent_t> ev;   // pairs
+    int used = 0;
+};
+
+struct sph_ctx {
+    sph_dims dims;
+    DGrid grid;
+    DScratch s;
+    int64_t nbuck;        // dense + hashed + off-grid
+    int nblk;             // particle blocks
+    int scan_blocks;
+    std::string err;
+    bool timing = false;
+    TimerKind tk[4];
+};
+
+static int set_err(sph_ctx* c, int code, const char* fmt, const char* a = "", long long b = 0) {
+    if (c) {
+        char buf[512];
+        snprintf(buf, sizeof buf, fmt, a, b);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHECK(ctx, call)                                                               \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return set_err(ctx, SPH_EHIP, "HIP error %s at line %lld", hipGetErrorString(e_), \
+                           (long long)__LINE__);                                          \
+    } while (0)
+
+static int check_launch(sph_ctx* c, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        char buf[512];
+        snprintf(buf, sizeof buf, "kernel launch %s failed: %s", what, hipGetErrorString(e));
+        c->err = buf;
+        return SPH_EHIP;
+    }
+    return SPH_OK;
+}
+
+static const int TIMER_SLOTS = 64;
+
+static void timer_begin(sph_ctx* c, int k, hipStream_t st) {
+    if (!c->timing) return;
+    TimerKind& t = c->tk[k];
+    if (t.used >= TIMER_SLOTS) return;
+    (void)hipEventRecord(t.ev[2 * t.used], st);
+}
+static void timer_end(sph_ctx* c, int k, hipStream_t st) {
+    if (!c->timing) return;
+    TimerKind& t = c->tk[k];
+    if (t.used >= TIMER_SLOTS) return;
+    (void)hipEventRecord(t.ev[2 * t.used + 1], st);
+    t.used++;
+}
+
+extern "C" const char* sph_version(void) { return SPH_VERSION; }
+
+extern "C" const char* sph_last_error(const sph_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+extern "C" int sph_ctx_create(sph_ctx** out, const sph_dims* d) {
+    if (!out || !d) return SPH_EINVAL;
+    *out = nullptr;
+    if (d->n <= 0 || d->n > (int64_t)1 << 30) return SPH_EINVAL;
+    if (d->precision != SPH_PREC_FP64 && d->precision != SPH_PREC_MIXED) return SPH_EINVAL;
+    if (!(d->h > 0.0)) return SPH_EINVAL;
+    for (int k = 0; k < 3; ++k) {
+        if (!(d->cell[k] >= d->h)) return SPH_EINVAL;
+        if (d->box_n[k] <= 0) return SPH_EINVAL;
+        if (d->flags & (SPH_PERIODIC_X << k)) {
+            if (d->box_n[k] < 3 || d->box_lo[k] != 0) return SPH_EINVAL;
+            if (k > 0 && (d->flags & (SPH_MIRROR_Y << (k - 1)))) return SPH_EINVAL;
+        }
+    }
+    if (d->hash_bits < 4 || d->hash_bits > 26) return SPH_EINVAL;
+    int64_t nbox = (int64_t)d->box_n[0] * d->box_n[1] * d->box_n[2];
+    int64_t T = (int64_t)1 << d->hash_bits;
+    if (nbox + T + 2 > (int64_t)1 << 31) return SPH_EINVAL;
+
+    sph_ctx* c = new sph_ctx();
+    c->dims = *d;
+    DGrid& g = c->grid;
+    for (int k = 0; k < 3; ++k) {
+        g.cell[k] = d->cell[k]; g.origin[k] = d->origin[k];
+        g.period[k] = d->period[k]; g.half_period[k] = 0.5 * d->period[k];
+        g.lo[k] = d->box_lo[k]; g.nb[k] = d->box_n[k];
+    }
+    g.periodic = (d->flags >> 4) & 7;
+    g.flags = d->flags;
+    g.hash_bits = d->hash_bits;
+    g.nbox = (int)nbox;
+    g.off_bucket = (int)(nbox + T);
+    g.h = d->h;
+    g.h2 = d->h * d->h;
+    c->nbuck = nbox + T + 1;
+    c->nblk = (int)((d->n + SPH_BLOCK - 1) / SPH_BLOCK);
+    c->scan_blocks = (int)((c->nbuck + 1 + SCAN_TILE - 1) / SCAN_TILE);
+
+    size_t n = (size_t)d->n;
+    size_t nb1 = (size_t)c->nbuck + 1;
+    DScratch& s = c->s;
+    hipError_t e = hipSuccess;
+#define ALLOC(p, bytes) if (e == hipSuccess) e = hipMalloc((void**)&(p), (bytes))
+    ALLOC(s.key, n * sizeof(int));
+    ALLOC(s.cellw, n * sizeof(int4));
+    ALLOC(s.cellid, n * sizeof(int4));
+    ALLOC(s.count, nb1 * sizeof(int));
+    ALLOC(s.start, nb1 * sizeof(int));
+    ALLOC(s.cursor, nb1 * sizeof(int));
+    ALLOC(s.bsum, (size_t)c->scan_blocks * sizeof(int));
+    ALLOC(s.tmp, n * sizeof(int));
+    ALLOC(s.perm, n * sizeof(int));
+    ALLOC(s.part, (size_t)SPH_NPART * c->nblk * sizeof(double));
+    ALLOC(s.maxv, 4 * sizeof(double));
+#undef ALLOC
+    if (e == hipSuccess) e = hipMemset(s.count, 0, nb1 * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(s.maxv, 0, 4 * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(s.start, 0, nb1 * sizeof(int));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        sph_ctx_destroy(c);
+        return SPH_ENOMEM;
+    }
+    *out = c;
+    return SPH_OK;
+}
+
+extern "C" int sph_ctx_destroy(sph_ctx* c) {
+    if (!c) return SPH_OK;
+    DScratch& s = c->s;
+    void* ptrs[] = {s.key, s.cellw, s.cellid, s.count, s.start, s.cursor, s.bsum, s.tmp, s.perm, s.part, s.maxv};
+    for (void* p : ptrs) if (p) (void)hipFree(p);
+    for (auto& t : c->tk) for (auto ev : t.ev) (void)hipEventDestroy(ev);
+    delete c;
+    return SPH_OK;
+}
+
+extern "C" int sph_timing(sph_ctx* c, int32_t enable) {
+    if (!c) return SPH_EINVAL;
+    if (enable && c->tk[0].ev.empty()) {
+        for (auto& t : c->tk) {
+            t.ev.resize(2 * TIMER_SLOTS);
+            for (auto& ev : t.ev) HIPCHECK(c, hipEventCreate(&ev));
+        }
+    }
+    for (auto& t : c->tk) t.used = 0;
+    c->timing = enable != 0;
+    return SPH_OK;
+}
+
+extern "C" int sph_timing_read(sph_ctx* c, int32_t which, double* avg_ms, int64_t* launches) {
+    if (!c || which < 0 || which > 3 || !avg_ms || !launches) return SPH_EINVAL;
+    TimerKind& t = c->tk[which];
+    double tot = 0.0;
+    for (int k = 0; k < t.used; ++k) {
+        HIPCHECK(c, hipEventSynchronize(t.ev[2 * k + 1]));
+        float ms = 0.f;
+        HIPCHECK(c, hipEventElapsedTime(&ms, t.ev[2 * k], t.ev[2 * k + 1]));
+        tot += ms;
+    }
+    *launches = t.used;
+    *avg_ms = t.used ? tot / t.used : 0.0;
+    return SPH_OK;
+}
+
+template <typename R> static DBuf<R> dbuf(const sph_sorted& b) {
+    DBuf<R> d;
+    d.pid = b.pid; d.obj = b.obj;
+    d.x = (double4*)b.x; d.x0 = (double4*)b.x0; d.u = (double4*)b.u; d.v = (double4*)b.v;
+    d.shift = (typename R4t<R>::type*)b.shift;
+    d.V = (R*)b.V; d.V_lo = (R*)b.V_lo; d.m = (R*)b.m; d.p = (R*)b.p;
+    return d;
+}
+
+template <typename R> static DState<R> dstate(const sph_ctx* c, const sph_state* st) {
+    DState<R> d;
+    d.b = dbuf<R>(st->buf[st->cur]);
+    d.acc = (typename R4t<R>::type*)st->acc;
+    d.lam = (R*)st->lam; d.L6 = (R*)st->L6;
+    d.lam_grad = (typename R4t<R>::type*)st->lam_grad;
+    d.div = (R*)st->div; d.de_visc = (R*)st->de_visc; d.V_new = (R*)st->V_new;
+    d.scal = st->scal;
+    d.n = (int)c->dims.n;
+    return d;
+}
+
+static bool buf_ok(const sph_sorted& b, bool mixed) {
+    return b.pid && b.obj && b.x && b.x0 && b.u && b.v && b.shift && b.V && b.m && b.p && (!mixed || b.V_lo);
+}
+
+static int check_state(sph_ctx* c, const sph_state* st, bool full) {
+    if (!c) return SPH_EINVAL;
+    if (!st || (st->cur != 0 && st->cur != 1) || !st->scal)
+        return set_err(c, SPH_EINVAL, "invalid state%s", "", 0);
+    bool mixed = c->dims.precision == SPH_PREC_MIXED;
+    if (!buf_ok(st->buf[st->cur], mixed)) return set_err(c, SPH_EINVAL, "null field in live buffer%s", "", 0);
+    if (full) {
+        if (!buf_ok(st->buf[1 - st->cur], mixed))
+            return set_err(c, SPH_EINVAL, "null field in alternate buffer%s", "", 0);
+        if (!st->acc || !st->lam || !st->L6 || !st->lam_grad || !st->div || !st->de_visc || !st->V_new)
+            return set_err(c, SPH_EINVAL, "null derived field%s", "", 0);
+    }
+    return SPH_OK;
+}
+
+extern "C" int sph_sort(sph_ctx* c, sph_state* st, void* stream) {
+    int rc = check_state(c, st, false);
+    if (rc) return rc;
+    if (!buf_ok(st->buf[1 - st->cur], c->dims.precision == SPH_PREC_MIXED))
+        return set_err(c, SPH_EINVAL, "null field in alternate buffer%s", "", 0);
+    hipStream_t hs = (hipStream_t)stream;
+    int n = (int)c->dims.n;
+    int nb = c->nblk;
+    bool mixed = c->dims.precision == SPH_PREC_MIXED;
+    const sph_sorted& A = st->buf[st->cur];
+    const sph_sorted& B = st->buf[1 - st->cur];
+    timer_begin(c, 3, hs);
+    hipLaunchKernelGGL(k_zero_scal, dim3(1), dim3(64), 0, hs, st->scal);
+    hipLaunchKernelGGL(k_keys, dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, n, (const double4*)A.x,
+                       (const void*)A.m, mixed ? 0 : 1, c->s, st->scal);
+    int total = (int)c->nbuck + 1;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(c->scan_blocks), dim3(SPH_BLOCK), 0, hs, total, c->s.count, c->s.start,
+                       c->s.bsum);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, hs, c->scan_blocks, c->s.bsum);
+    hipLaunchKernelGGL(k_scan_add, dim3((total + SPH_BLOCK - 1) / SPH_BLOCK), dim3(SPH_BLOCK), 0, hs, total,
+                       c->s.start, c->s.cursor, (const int*)c->s.bsum);
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(SPH_BLOCK), 0, hs, n, c->s);
+    hipLaunchKernelGGL(k_rank, dim3(nb), dim3(SPH_BLOCK), 0, hs, n, c->s, (const int32_t*)A.pid);
+    PermTable T;
+    int rb = mixed ? 4 : 8, r4 = mixed ? 16 : 32;
+    int nf = 0;
+    auto add = [&](const void* s_, void* d_, int bytes) { T.f[nf].src = s_; T.f[nf].dst = d_; T.f[nf].bytes = bytes; ++nf; };
+    add(A.pid, B.pid, 4); add(A.obj, B.obj, 4);
+    add(A.x, B.x, 32); add(A.x0, B.x0, 32); add(A.u, B.u, 32); add(A.v, B.v, 32);
+    add(A.shift, B.shift, r4);
+    add(A.V, B.V, rb); add(A.m, B.m, rb); add(A.p, B.p, rb);
+    if (mixed) add(A.V_lo, B.V_lo, rb);
+    T.nf = nf;
+    hipLaunchKernelGGL(k_permute, dim3(nb), dim3(SPH_BLOCK), 0, hs, n, T, c->s, c->grid.nbox);
+    timer_end(c, 3, hs);
+    rc = check_launch(c, "sort");
+    if (rc) return rc;
+    st->cur = 1 - st->cur;
+    return SPH_OK;
+}
+
+static DFluid make_fluid(const sph_ctx* c, const sph_fluid_params* P) {
+    DFluid F;
+    memset(&F, 0, sizeof F);
+    const sph_dims& d = c->dims;
+    double h = d.h;
+    F.h = h; F.inv_h = 1.0 / h;
+    F.alpha = (d.flags & SPH_TWO_D) ? 7.0 / (M_PI * h * h) : 21.0 / (2.0 * M_PI * h * h * h);
+    F.gfac = 20.0 * F.alpha / (h * h);
+    F.eta2 = 0.01 * h * h;
+    F.rho0 = P->rho0; F.c0 = P->c0; F.gamma = P->gamma; F.S = P->stiffness;
+    F.visc = P->viscosity; F.delta = P->delta_sph; F.cfl = P->cfl;
+    F.csig2 = P->c_signal * P->c_signal;
+    for (int k = 0; k < 3; ++k) { F.g[k] = P->g[k]; F.dsize[k] = P->domain_size[k]; }
+    F.dx0 = P->dx0;
+    double q0 = P->dx0 / h;
+    F.Wdx0 = q0 < 1.0 ? F.alpha * pow(1.0 - q0, 4) * (4.0 * q0 + 1.0) : 0.0;
+    F.axi_rmin = P->axi_r_min;
+    F.pst_R = P->pst_R; F.pst_max_shift = P->pst_max_shift; F.pst_ma_fixed = P->pst_ma_fixed;
+    F.lam_lo = P->pst_lambda_lo; F.lam_hi = P->pst_lambda_hi; F.m_delta = P->pst_m_delta;
+    F.tol_lam = P->l_eig_tol_lam; F.restitution = P->restitution; F.padding = P->padding;
+    F.pst_n = P->pst_n; F.allow_neg = P->allow_negative_pressure;
+    F.track_visc = P->track_visc_work && P->viscosity > 0.0;
+    F.dt_pair = P->dt_pair; F.pst_en = P->pst_enabled; F.colle = P->pst_colle;
+    F.ma_mode = P->pst_ma_mode; F.surf_none = P->pst_surface_none;
+    F.clamp_disp = P->pst_clamp_disp; F.clamp_vel = P->pst_clamp_vel;
+    F.ale_mom = P->pst_ale_momentum; F.walls = P->walls;
+    F.disp_pos = P->displacement_positions; F.comp_vol = P->compensated_volume;
+    F.flags = d.flags;
+    return F;
+}
+
+template <typename R> static int pass_volume(sph_ctx* c, sph_state* st, const DFluid& F, hipStream_t hs) {
+    timer_begin(c, 0, hs);
+    hipLaunchKernelGGL((k_pass_volume<R>), dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st),
+                       c->s);
+    timer_end(c, 0, hs);
+    return check_launch(c, "pass_volume");
+}
+
+template <typename R> static int pass_force(sph_ctx* c, sph_state* st, const DFluid& F, hipStream_t hs) {
+    timer_begin(c, 1, hs);
+    hipLaunchKernelGGL((k_pass_force<R>), dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st),
+                       c->s);
+    timer_end(c, 1, hs);
+    hipLaunchKernelGGL(k_dt, dim3(1), dim3(1024), 0, hs, F, st->scal, c->s, c->nblk);
+    return check_launch(c, "pass_force");
+}
+
+template <typename R> static int advect(sph_ctx* c, sph_state* st, const DFluid& F, hipStream_t hs) {
+    hipLaunchKernelGGL((k_advect<R>), dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st), c->s);
+    if (F.ma_mode == 1) hipLaunchKernelGGL(k_publish_vmax, dim3(1), dim3(64), 0, hs, st->scal, c->s);
+    return check_launch(c, "advect");
+}
+
+template <typename R> static int pass_shift(sph_ctx* c, sph_state* st, const DFluid& F, hipStream_t hs) {
+    if (!F.pst_en) return SPH_OK;
+    timer_begin(c, 2, hs);
+    hipLaunchKernelGGL((k_pass_shift<R>), dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st),
+                       c->s);
+    timer_end(c, 2, hs);
+    return check_launch(c, "pass_shift");
+}
+
+template <typename R> static int finish(sph_ctx* c, sph_state* st, const DFluid& F, hipStream_t hs) {
+    hipLaunchKernelGGL((k_finish<R>), dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st), c->s);
+    hipLaunchKernelGGL(k_bank, dim3(1), dim3(1024), 0, hs, st->scal, c->s, c->nblk, F.ma_mode == 1 ? 1 : 0);
+    return check_launch(c, "finish");
+}
+
+static int check_fluid(sph_ctx* c, sph_state* st, const sph_fluid_params* P) {
+    int rc = check_state(c, st, true);
+    if (rc) return rc;
+    if (!P) return set_err(c, SPH_EINVAL, "null params%s", "", 0);
+    if (!(P->rho0 > 0.0) || !(P->cfl > 0.0) || !(P->gamma > 0.0))
+        return set_err(c, SPH_EINVAL, "invalid fluid parameters (rho0, CFL, exponent)%s", "", 0);
+    if (P->pst_n < 0 || P->pst_n > 16) return set_err(c, SPH_EINVAL, "PST_N out of range: %s%lld", "", P->pst_n);
+    return SPH_OK;
+}
+
+#define DISPATCH(fn)                                                                          \
+    extern "C" int sph_fluid_##fn(sph_ctx* c, sph_state* st, const sph_fluid_params* P, void* stream) { \
+        int rc = check_fluid(c, st, P);                                                       \
+        if (rc) return rc;                                                                    \
+        DFluid F = make_fluid(c, P);                                                          \
+        if (c->dims.precision == SPH_PREC_MIXED) return fn<float>(c, st, F, (hipStream_t)stream); \
+        return fn<double>(c, st, F, (hipStream_t)stream);                                     \
+    }
+
+// sph_fluid_pass_volume etc.
+DISPATCH(pass_volume)
+DISPATCH(pass_force)
+DISPATCH(advect)
+DISPATCH(pass_shift)
+DISPATCH(finish)
+
+extern "C" int sph_fluid_step(sph_ctx* c, sph_state* st, const sph_fluid_params* P, void* stream) {
+    int rc = check_fluid(c, st, P);
+    if (rc) return rc;
+    hipStream_t hs = (hipStream_t)stream;
+    rc = sph_sort(c, st, stream);
+    if (rc) return rc;
+    DFluid F = make_fluid(c, P);
+    if (c->dims.precision == SPH_PREC_MIXED) {
+        if ((rc = pass_volume<float>(c, st, F, hs))) return rc;
+        if ((rc = pass_force<float>(c, st, F, hs))) return rc;
+        if ((rc = advect<float>(c, st, F, hs))) return rc;
+        if ((rc = pass_shift<float>(c, st, F, hs))) return rc;
+        return finish<float>(c, st, F, hs);
+    }
+    if ((rc = pass_volume<double>(c, st, F, hs))) return rc;
+    if ((rc = pass_force<double>(c, st, F, hs))) return rc;
+    if ((rc = advect<double>(c, st, F, hs))) return rc;
+    if ((rc = pass_shift<double>(c, st, F, hs))) return rc;
+    return finish<double>(c, st, F, hs);
+}
+
+extern "C" int sph_neighbor_pairs(sph_ctx* c, const sph_state* st, int32_t* pairs, int64_t cap, int64_t* count,
+                                  void* stream) {
+    int rc = check_state(c, st, false);
+    if (rc) return rc;
+    if (!pairs || !count || cap < 0) return set_err(c, SPH_EINVAL, "null output%s", "", 0);
+    hipStream_t hs = (hipStream_t)stream;
+    const sph_sorted& A = st->buf[st->cur];
+    HIPCHECK(c, hipMemsetAsync(count, 0, sizeof(int64_t), hs));
+    hipLaunchKernelGGL(k_pairs, dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, (int)c->dims.n,
+                       (const double4*)A.x, (const int32_t*)A.pid, c->s, (const double*)st->scal, pairs, (long long)cap,
+                       (unsigned long long*)count);
+    rc = check_launch(c, "neighbor_pairs");
+    if (rc) return rc;
+    int64_t h = 0;
+    HIPCHECK(c, hipMemcpyAsync(&h, count, sizeof h, hipMemcpyDeviceToHost, hs));
+    HIPCHECK(c, hipStreamSynchronize(hs));
+    if (h > cap) return set_err(c, SPH_ECAPACITY, "pair buffer too small%s: %lld pairs", "", (long long)h);
+    return SPH_OK;
+}
+
+extern "C" int sph_reduce(sph_ctx* c, const sph_state* st, int32_t which, const double* g,
+                          const sph_fluid_params* P, double* out, void* stream) {
+    int rc = check_state(c, st, false);
+    if (rc) return rc;
+    if (!out || which < 0 || which > SPH_R_KINETIC_OFFGRID) return set_err(c, SPH_EINVAL, "bad reduce%s", "", 0);
+    if (which == SPH_R_TAIT_ENERGY && !P) return set_err(c, SPH_EINVAL, "TAIT_ENERGY needs params%s", "", 0);
+    if (which == SPH_R_POTENTIAL && !g) return set_err(c, SPH_EINVAL, "POTENTIAL needs g%s", "", 0);
+    hipStream_t hs = (hipStream_t)stream;
+    DFluid F;
+    memset(&F, 0, sizeof F);
+    if (P) F = make_fluid(c, P);
+    double g0 = g ? g[0] : 0, g1 = g ? g[1] : 0, g2 = g ? g[2] : 0;
+    int n = (int)c->dims.n;
+    if (c->dims.precision == SPH_PREC_MIXED)
+        hipLaunchKernelGGL(k_reduce<float>, dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, n, dbuf<float>(st->buf[st->cur]),
+                           which, g0, g1, g2, F, c->s);
+    else
+        hipLaunchKernelGGL(k_reduce<double>, dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, n,
+                           dbuf<double>(st->buf[st->cur]), which, g0, g1, g2, F, c->s);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, hs, c->s, c->nblk, out);
+    return check_launch(c, "reduce");
+}

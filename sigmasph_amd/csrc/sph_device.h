@@ -1,0 +1,305 @@
+// sph_device.h -- device-side building blocks shared by the kernels of libsigmasph.so:
+// the cell lattice and its buckets, the neighbour traversal, the Wendland C2 kernel,
+// the symmetric eigen solver, and wave/block reductions.  gfx950 only (wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/sigmasph.h"
+
+#define SPH_BLOCK 256
+#define SPH_WAVE 64
+
+template <typename R> struct R4t;
+template <> struct R4t<double> { typedef double4 type; };
+template <> struct R4t<float> { typedef float4 type; };
+
+// ------------------------------------------------------------------------------------
+// lattice description passed to every kernel by value
+// ------------------------------------------------------------------------------------
+struct DGrid {
+    double cell[3], origin[3], period[3], half_period[3];
+    int lo[3], nb[3];
+    int periodic;      // bit k: axis k periodic
+    int flags;         // SPH_* geometry flags
+    int hash_bits;
+    int nbox;          // dense buckets
+    int off_bucket;    // nbox + 2^hash_bits: quarantine bucket of non-finite particles
+    double h, h2;      // support radius and its square (the predicate is r2 < h2)
+};
+
+__device__ __forceinline__ bool finite3(double a, double b, double c) {
+    return isfinite(a) && isfinite(b) && isfinite(c);
+}
+
+// floor((x - origin) / cell), clamped to +-2^30; periodic axes clamp into the box
+__device__ __forceinline__ int cell1(const DGrid& g, int k, double x) {
+    double c = floor((x - g.origin[k]) / g.cell[k]);
+    c = fmin(fmax(c, -1073741823.0), 1073741823.0);
+    int ci = (int)c;
+    if (g.periodic & (1 << k)) ci = max(0, min(ci, g.nb[k] - 1));
+    return ci;
+}
+
+// the XOR cell hash of the reference (particle_system.py:1283-1293)
+__device__ __forceinline__ unsigned hash_cell(int cx, int cy, int cz, int bits) {
+    unsigned h = ((unsigned)cx * 73856093u) ^ ((unsigned)cy * 19349663u) ^ ((unsigned)cz * 83492791u);
+    return h & ((1u << bits) - 1u);
+}
+
+__device__ __forceinline__ bool in_box(const DGrid& g, int cx, int cy, int cz) {
+    return cx >= g.lo[0] && cx < g.lo[0] + g.nb[0] && cy >= g.lo[1] && cy < g.lo[1] + g.nb[1] &&
+           cz >= g.lo[2] && cz < g.lo[2] + g.nb[2];
+}
+
+__device__ __forceinline__ int dense_key(const DGrid& g, int cx, int cy, int cz) {
+    return ((cz - g.lo[2]) * g.nb[1] + (cy - g.lo[1])) * g.nb[0] + (cx - g.lo[0]);
+}
+
+__device__ __forceinline__ int bucket_of(const DGrid& g, int cx, int cy, int cz) {
+    if (in_box(g, cx, cy, cz)) return dense_key(g, cx, cy, cz);
+    return g.nbox + (int)hash_cell(cx, cy, cz, g.hash_bits);
+}
+
+__device__ __forceinline__ int wrap_cell(const DGrid& g, int k, int c) {
+    if (g.periodic & (1 << k)) {
+        int n = g.nb[k];
+        c %= n;
+        if (c < 0) c += n;
+    }
+    return c;
+}
+
+__device__ __forceinline__ double min_image(const DGrid& g, int k, double r) {
+    if (g.periodic & (1 << k)) {
+        if (r > g.half_period[k]) r -= g.period[k];
+        else if (r < -g.half_period[k]) r += g.period[k];
+    }
+    return r;
+}
+
+// r.r with no fused multiply-add, so the neighbour predicate is the same bit decision
+// as the oracle's (built with -ffp-contract=off)
+__device__ __forceinline__ double dist2_exact(double a, double b, double c) {
+#pragma clang fp contract(off)
+    double aa = a * a;
+    double bb = b * b;
+    double cc = c * c;
+    return (aa + bb) + cc;
+}
+
+// ------------------------------------------------------------------------------------
+// neighbour traversal (for_all_neighbors, particle_system.py:1348-1490)
+//
+// Visits, for particle i at live position xi, every j binned in the 27 cells around
+// floor(xi / cell) with |xi - xj| < h (j != i), then the images across y = 0, z = 0 and
+// the corner.  Cells are visited z-outer, y, x-inner; inside the dense box the three x
+// cells of a row are one contiguous range of the sorted order.  F(j, mir, rx, ry, rz)
+// receives r = x_i - M x_j.
+// ------------------------------------------------------------------------------------
+struct Traverse {
+    const DGrid& g;
+    const int* __restrict__ start;
+    const int4* __restrict__ cellid;
+    const double4* __restrict__ X;
+    bool any_out;
+
+    template <class F>
+    __device__ __forceinline__ void range(int b, int e, bool check, int cx, int cy, int cz, int i,
+                                          int mir, double tx, double ty, double tz, F& f) const {
+        for (int j = b; j < e; ++j) {
+            if (check) {
+                int4 cj = cellid[j];
+                if (cj.x != cx || cj.y != cy || cj.z != cz) continue;
+            }
+            if (mir == 0 && j == i) continue;
+            double4 xj = X[j];
+            double dx = min_image(g, 0, tx - xj.x);
+            double dy = min_image(g, 1, ty - xj.y);
+            double dz = min_image(g, 2, tz - xj.z);
+            if (!(dist2_exact(dx, dy, dz) < g.h2)) continue;
+            // r = x_i - M x_j = M (M x_i - x_j)
+            if (mir & 1) dy = -dy;
+            if (mir & 2) dz = -dz;
+            f(j, mir, dx, dy, dz);
+        }
+    }
+
+    template <class F>
+    __device__ __forceinline__ void cell(int cx, int cy, int cz, int i, int mir, double tx, double ty,
+                                         double tz, F& f) const {
+        if (in_box(g, cx, cy, cz)) {
+            int k = dense_key(g, cx, cy, cz);
+            range(start[k], start[k + 1], false, cx, cy, cz, i, mir, tx, ty, tz, f);
+        } else if (any_out) {
+            int k = g.nbox + (int)hash_cell(cx, cy, cz, g.hash_bits);
+            range(start[k], start[k + 1], true, cx, cy, cz, i, mir, tx, ty, tz, f);
+        }
+    }
+
+    template <class F>
+    __device__ __forceinline__ void operator()(int i, double xi, double yi, double zi, F& f) const {
+        if (!finite3(xi, yi, zi)) return;
+        int cx = cell1(g, 0, xi), cy = cell1(g, 1, yi), cz = cell1(g, 2, zi);
+        const bool two_d = (g.flags & SPH_TWO_D) != 0;
+        const bool px = (g.periodic & 1) != 0;
+        for (int oz = -1; oz <= 1; ++oz) {
+            if (two_d && oz != 0 && cz == 0 && g.nb[2] == 1 && g.lo[2] == 0 && !any_out) continue;
+            int zz = wrap_cell(g, 2, cz + oz);
+            for (int oy = -1; oy <= 1; ++oy) {
+                int yy = wrap_cell(g, 1, cy + oy);
+                bool yz_in = yy >= g.lo[1] && yy < g.lo[1] + g.nb[1] && zz >= g.lo[2] &&
+                             zz < g.lo[2] + g.nb[2];
+                if (yz_in && !px && cx - 1 >= g.lo[0] && cx + 1 < g.lo[0] + g.nb[0]) {
+                    int k = dense_key(g, cx - 1, yy, zz);
+                    range(start[k], start[k + 3], false, 0, 0, 0, i, 0, xi, yi, zi, f);
+                } else {
+                    for (int ox = -1; ox <= 1; ++ox)
+                        cell(wrap_cell(g, 0, cx + ox), yy, zz, i, 0, xi, yi, zi, f);
+                }
+            }
+        }
+        const bool my = (g.flags & SPH_MIRROR_Y) != 0, mz = (g.flags & SPH_MIRROR_Z) != 0;
+        if (my && yi < g.h) {
+            for (int oz = -1; oz <= 1; ++oz)
+                for (int ox = -1; ox <= 1; ++ox)
+                    cell(cx + ox, 0, cz + oz, i, 1, xi, -yi, zi, f);
+        }
+        if (mz && zi < g.h) {
+            for (int oy = -1; oy <= 1; ++oy)
+                for (int ox = -1; ox <= 1; ++ox)
+                    cell(cx + ox, cy + oy, 0, i, 2, xi, yi, -zi, f);
+        }
+        if (my && mz && yi < g.h && zi < g.h) {
+            for (int ox = -1; ox <= 1; ++ox) cell(cx + ox, 0, 0, i, 3, xi, -yi, -zi, f);
+        }
+    }
+};
+
+// ------------------------------------------------------------------------------------
+// symmetric 3x3 eigenproblem by cyclic Jacobi rotations (stands in for ti.sym_eig) and
+// the regularised inverse of spd_inverse.py
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ double spd_weight(double e, double tol) {
+    double u = fmin(1.0, fmax(0.0, (e - tol) / tol));
+    double beta = u * u * (3.0 - 2.0 * u);
+    return beta / fmax(e, tol) + (1.0 - beta);
+}
+
+// A = (a00,a11,a22,a01,a02,a12) symmetric; returns L (same layout) and min eigenvalue
+__device__ inline void spd_inverse3(const double* A6, double tol, double* L6, double* min_eig) {
+    double A[3][3] = {{A6[0], A6[3], A6[4]}, {A6[3], A6[1], A6[5]}, {A6[4], A6[5], A6[2]}};
+    double Vm[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    const int P[3] = {0, 0, 1}, Q[3] = {1, 2, 2};
+    for (int sweep = 0; sweep < 32; ++sweep) {
+        double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+        double dia = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+        if (off <= 1.0e-32 * dia || off == 0.0) break;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            int p = P[r], q = Q[r];
+            double apq = A[p][q];
+            if (apq == 0.0) continue;
+            double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
+            double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double akp = A[k][p], akq = A[k][q];
+                A[k][p] = c * akp - sn * akq;
+                A[k][q] = sn * akp + c * akq;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double apk = A[p][k], aqk = A[q][k];
+                A[p][k] = c * apk - sn * aqk;
+                A[q][k] = sn * apk + c * aqk;
+            }
+            A[p][q] = A[q][p] = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double vkp = Vm[k][p], vkq = Vm[k][q];
+                Vm[k][p] = c * vkp - sn * vkq;
+                Vm[k][q] = sn * vkp + c * vkq;
+            }
+        }
+    }
+    double me = fmin(A[0][0], fmin(A[1][1], A[2][2]));
+    double w0 = spd_weight(A[0][0], tol), w1 = spd_weight(A[1][1], tol), w2 = spd_weight(A[2][2], tol);
+    L6[0] = w0 * Vm[0][0] * Vm[0][0] + w1 * Vm[0][1] * Vm[0][1] + w2 * Vm[0][2] * Vm[0][2];
+    L6[1] = w0 * Vm[1][0] * Vm[1][0] + w1 * Vm[1][1] * Vm[1][1] + w2 * Vm[1][2] * Vm[1][2];
+    L6[2] = w0 * Vm[2][0] * Vm[2][0] + w1 * Vm[2][1] * Vm[2][1] + w2 * Vm[2][2] * Vm[2][2];
+    L6[3] = w0 * Vm[0][0] * Vm[1][0] + w1 * Vm[0][1] * Vm[1][1] + w2 * Vm[0][2] * Vm[1][2];
+    L6[4] = w0 * Vm[0][0] * Vm[2][0] + w1 * Vm[0][1] * Vm[2][1] + w2 * Vm[0][2] * Vm[2][2];
+    L6[5] = w0 * Vm[1][0] * Vm[2][0] + w1 * Vm[1][1] * Vm[2][1] + w2 * Vm[1][2] * Vm[2][2];
+    *min_eig = me;
+}
+
+// spd_inverse_2x2 (spd_inverse.py:53-84) embedded with L22 = 1; E = (e00, e01, e10, e11)
+__device__ inline void spd_inverse2(double e00, double e01, double e10, double e11, double tol,
+                                    double* L6) {
+    double a = e00, b = 0.5 * (e01 + e10), d = e11;
+    double disc = sqrt(fmax((a - d) * (a - d) + 4.0 * b * b, 0.0));
+    double e1 = 0.5 * (a + d + disc), e2 = 0.5 * (a + d - disc);
+    double n0 = 1.0, n1 = 0.0;
+    double scale = fabs(a) + fabs(d) + 1e-30;
+    if (disc > 1e-9 * scale) {
+        double v10 = b, v11 = e1 - a, v20 = e1 - d, v21 = b;
+        double l1 = sqrt(v10 * v10 + v11 * v11), l2 = sqrt(v20 * v20 + v21 * v21);
+        if (l1 >= l2) { n0 = v10 / l1; n1 = v11 / l1; }
+        else          { n0 = v20 / l2; n1 = v21 / l2; }
+    }
+    double m0 = -n1, m1 = n0;
+    double w1 = spd_weight(e1, tol), w2 = spd_weight(e2, tol);
+    L6[0] = w1 * n0 * n0 + w2 * m0 * m0;
+    L6[1] = w1 * n1 * n1 + w2 * m1 * m1;
+    L6[2] = 1.0;
+    L6[3] = w1 * n0 * n1 + w2 * m0 * m1;
+    L6[4] = 0.0;
+    L6[5] = 0.0;
+}
+
+// ------------------------------------------------------------------------------------
+// reductions: wave64 shuffles, then LDS across the block's waves
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// deterministic block sum of one value per thread; result valid in thread 0
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+    v = wave_sum(v);
+    int w = threadIdx.x / SPH_WAVE, l = threadIdx.x % SPH_WAVE;
+    if (l == 0) lds[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < (int)(blockDim.x / SPH_WAVE); ++k) r += lds[k];
+    }
+    __syncthreads();
+    return r;
+}
+__device__ __forceinline__ double block_max(double v, double* lds) {
+    v = wave_max(v);
+    int w = threadIdx.x / SPH_WAVE, l = threadIdx.x % SPH_WAVE;
+    if (l == 0) lds[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        r = lds[0];
+        for (int k = 1; k < (int)(blockDim.x / SPH_WAVE); ++k) r = fmax(r, lds[k]);
+    }
+    __syncthreads();
+    return r;
+}
+
+// atomic max of a non-negative double through its bit pattern
+__device__ __forceinline__ void atomic_max_pos(double* addr, double v) {
+    atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
+}

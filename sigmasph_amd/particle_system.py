@@ -1,0 +1,417 @@
+"""ParticleSystem: particle state on the GPU and the neighbour search.
+
+Mirrors the reference's `ParticleSystem` (particle_system.py:24-1849) for the particle
+time step: the same constructor argument (a `SimConfig`), the same attribute names
+(`x`, `x_0`, `u`, `v`, `acceleration`, `V`, `V_lo`, `m`, `pressure`, `pst_shift`,
+`object_id`, `lam`, `divergence`, `particle_num[None]`, ...), `build_solver()` and
+`initialize_particle_system()`.  The state lives in torch tensors on the GPU; the sort
+and every kernel run in libsigmasph.so through the C ABI.
+
+Differences from the reference, all deliberate:
+  - positions, velocities and displacements are float64 in both precision modes;
+    the other per-particle fields follow `precision` ("fp64" or "mixed");
+  - every sorted field is double-buffered and the counting sort gathers into the other
+    buffer (the reference scatters into a scratch buffer and copies back);
+  - within a cell particles are ordered by a stable particle id `pid`;
+  - the lattice is laid in float64 (the reference's `lattice_points` is float32);
+  - `Domain.periodic` and the block keys `jitter`/`seed`/`taylorGreen` are additions.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .fields import Field, Scalar
+
+
+def lattice_points(lower, size, diameter, dim=3):
+    """The lattice `add_cube` lays on a box (geometry.py:552-561): `np.arange` per axis,
+    meshgrid 'ij', one row per point -- in float64 here."""
+    axes = [np.arange(lower[i], lower[i] + size[i], diameter) for i in range(dim)]
+    grid = np.array(np.meshgrid(*axes, sparse=False, indexing="ij"), dtype=np.float64)
+    return grid.reshape(dim, -1).T
+
+
+class SceneLayout:
+    """Host-side part of ParticleSystem.__init__ (particle_system.py:25-592): the mode
+    flags, the particle lattice and the cell lattice, in float64, without a GPU."""
+
+    def __init__(self, config, precision="fp64"):
+        self.cfg = config
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed', got %r" % (precision,))
+        self.precision = precision
+        self.real = torch.float64 if precision == "fp64" else torch.float32
+
+        c = self.cfg.get_cfg
+        ds, de = c("domainStart"), c("domainEnd")
+        self.domain_start = np.array(ds if ds is not None else [0.0, 0.0, 0.0], dtype=np.float64)
+        self.domain_end = np.array(de if de is not None else [1.0, 1.0, 1.0], dtype=np.float64)
+        self.domain_size = self.domain_end - self.domain_start
+
+        self.plane_strain = bool(c("planeStrain") or False)
+        self.axisymmetric = bool(c("axisymmetric") or False)
+        if self.axisymmetric and self.plane_strain:
+            raise SystemExit("planeStrain and axisymmetric cannot both be on")
+        self.two_d = self.plane_strain or self.axisymmetric
+        self.dim = 3
+        sym = str(c("symmetryPlanes") or "").strip().lower()
+        self.sym_y = "y" in sym
+        self.sym_z = "z" in sym
+        if sym and self.two_d:
+            raise SystemExit("Domain.symmetryPlanes needs a full 3D domain")
+        self.mirror_y = self.axisymmetric or self.sym_y
+        self.mirror_z = self.sym_z
+        per = str(c("PERIODIC") or "").lower()
+        self.periodic = [ax in per for ax in "xyz"]
+        if (self.periodic[1] and self.mirror_y) or (self.periodic[2] and self.mirror_z):
+            raise SystemExit("an axis cannot be both periodic and a symmetry plane")
+        if self.two_d and self.periodic[2]:
+            raise SystemExit("a single-layer deck cannot be periodic in z")
+
+        dp = c("DISPLACEMENT_POSITIONS")
+        self.displacement_positions = True if dp is None else bool(dp)
+        cv = c("COMPENSATED_VOLUME")
+        self.compensated_volume = True if cv is None else bool(cv)
+        self.simulation_method = c("simulationMethod")
+        self.is_solid = self.simulation_method == "hypoElastic"
+        self.walls = str(c("DOMAIN_BOUNDARY") or "open").lower() == "reflect"
+        cf = c("DOMAIN_RESTITUTION")
+        self.wall_restitution = 0.5 if cf is None else float(cf)
+        if not 0.0 <= self.wall_restitution <= 1.0:
+            raise ValueError("Domain.restitution must lie in [0, 1]; got %r" % (self.wall_restitution,))
+
+        self.material_fluid, self.material_solid = 1, 2
+        self.particle_radius = float(c("particleRadius"))
+        self.particle_diameter = 2.0 * self.particle_radius
+        srf = c("supportRadiusFactor")
+        self.support_radius_factor = 4.0 if srf is None else float(srf)
+        self.support_radius = self.particle_radius * self.support_radius_factor
+        self.variable_h = False
+        self.dx_per_h = 2.0 / self.support_radius_factor
+        self.h_max = self.h_min = self.support_radius
+        self.dt_length = self.support_radius
+        self.V0 = self.particle_diameter ** 2 if self.two_d else self.particle_diameter ** 3
+        rmin = c("AXI_R_MIN")
+        self.axi_r_min = (0.5 if rmin is None else float(rmin)) * self.particle_diameter
+        self.grid_size = self.support_radius
+        self.padding = self.grid_size
+        self.grid_num = np.ceil(self.domain_size / self.grid_size).astype(int)
+        self.object_collection = {}
+
+        # ---- lay the particles (host, float64) ------------------------------------
+        blocks = ([(b, self.material_fluid) for b in self.cfg.get_fluid_blocks()] +
+                  [(b, self.material_solid) for b in self.cfg.get_solid_blocks()])
+        if c("ParticlesFile"):
+            raise NotImplementedError("ParticlesFile input is not part of this build")
+        xs, vs, rhos, objs = [], [], [], []
+        for blk, _mat in blocks:
+            x, v, rho = self._lay_block(blk)
+            if len(x) == 0:
+                raise SystemExit("block %s is empty" % blk["objectId"])
+            blk["particleNum"] = len(x)
+            self.object_collection[blk["objectId"]] = blk
+            xs.append(x); vs.append(v); rhos.append(rho)
+            objs.append(np.full(len(x), int(blk["objectId"]), dtype=np.int32))
+        if not xs:
+            raise SystemExit("the scene lays no particles")
+        x = np.concatenate(xs)
+        v = np.concatenate(vs)
+        rho = np.concatenate(rhos)
+        obj = np.concatenate(objs)
+        n = len(x)
+        self.particle_max_num = n
+        self.fluid_particle_num = n
+        if self.two_d:
+            if np.ptp(x[:, 2]) > 1e-8:
+                raise SystemExit("non-zero z-range of particles is not allowed for a single layer")
+            if self.axisymmetric and x[:, 1].min() < 0.0:
+                raise SystemExit("axisymmetric: every particle must have y >= 0")
+        for axis, on, name in ((1, self.sym_y, "y"), (2, self.sym_z, "z")):
+            if on and x[:, axis].min() < 0.0:
+                raise SystemExit("symmetryPlanes '%s': every particle must have %s >= 0" % (name, name))
+        if any(self.periodic):
+            for k in range(3):
+                if self.periodic[k]:
+                    L = self.domain_size[k]
+                    x[:, k] = x[:, k] - L * np.floor((x[:, k] - self.domain_start[k]) / L)
+
+        self._init_arrays(x, v, rho, obj)
+
+    def _init_arrays(self, x, v, rho, obj):
+        n = len(x)
+        vol = np.full(n, self.V0)
+        if self.axisymmetric:
+            vol = self.V0 * np.maximum(x[:, 1], self.axi_r_min)
+        m = vol * rho
+        V = m / rho
+        self.x_init, self.v_init, self.m_init, self.V_init, self.obj_init = x, v, m, V, obj
+        self.dims = self._make_dims(x)
+
+
+    def _lay_block(self, blk):
+        """add_cube (particle_system.py:1771-1850) for one block, plus the build-side
+        initial conditions `jitter` (uniform +-jitter*dx per component, seeded) and
+        `taylorGreen` ({"U": speed, "L": box length}, the 3-D TGV field)."""
+        offset = np.array(blk.get("translation", [0, 0, 0]), dtype=np.float64)
+        start = np.array(blk["start"], dtype=np.float64) + offset
+        end = np.array(blk["end"], dtype=np.float64) + offset
+        scale = np.array(blk.get("scale", [1, 1, 1]), dtype=np.float64)
+        size = (end - start) * scale
+        dx = self.particle_diameter
+        x = lattice_points(start, size, dx, 3)
+        if blk.get("shape"):
+            raise NotImplementedError("block shapes are not part of this build")
+        n = len(x)
+        rho = np.full(n, float(blk.get("density") or self.cfg.get_cfg("density0") or 1000.0))
+        v = np.zeros((n, 3))
+        if blk.get("velocity") is not None:
+            v[:] = np.array(blk["velocity"], dtype=np.float64)
+        if blk.get("rotVelocity") is not None:
+            om = float(blk["rotVelocity"][2])
+            ctr = start + size / 2.0
+            r = x - ctr
+            v[:, 0] = om * r[:, 1]
+            v[:, 1] = -om * r[:, 0]
+            v[:, 2] = 0.0
+        if blk.get("divVelocity") is not None:
+            dv = blk["divVelocity"]
+            ctr = x.mean(axis=0)
+            r = x - ctr
+            v[:] = 0.0
+            if isinstance(dv, (list, tuple)):
+                for d in range(min(len(dv), 3)):
+                    v[:, d] = float(dv[d]) * r[:, d]
+            else:
+                v[:, 0], v[:, 1] = r[:, 0], r[:, 1]
+        tg = blk.get("taylorGreen")
+        if tg:
+            U = float(tg.get("U", 1.0))
+            Lb = float(tg.get("L", size[0]))
+            o = np.array(tg.get("origin", start), dtype=np.float64)
+            k = 2.0 * math.pi / Lb
+            X, Y, Z = (x[:, 0] - o[0]) * k, (x[:, 1] - o[1]) * k, (x[:, 2] - o[2]) * k
+            v[:, 0] = U * np.sin(X) * np.cos(Y) * np.cos(Z)
+            v[:, 1] = -U * np.cos(X) * np.sin(Y) * np.cos(Z)
+            v[:, 2] = 0.0
+        jit = float(blk.get("jitter") or 0.0)
+        if jit > 0.0:
+            rng = np.random.default_rng(int(blk.get("seed", 0)))
+            d = rng.uniform(-1.0, 1.0, size=(n, 3)) * (jit * dx)
+            if self.two_d:
+                d[:, 2] = 0.0
+            x = x + d
+        return x, v, rho
+
+    def _make_dims(self, x):
+        d = N.SphDims()
+        n = len(x)
+        d.n = n
+        d.precision = N.SPH_PREC_FP64 if self.precision == "fp64" else N.SPH_PREC_MIXED
+        f = 0
+        if self.two_d: f |= N.SPH_TWO_D
+        if self.axisymmetric: f |= N.SPH_AXISYMMETRIC
+        if self.plane_strain: f |= N.SPH_PLANE_STRAIN
+        if self.mirror_y: f |= N.SPH_MIRROR_Y
+        if self.mirror_z: f |= N.SPH_MIRROR_Z
+        for k, flag in enumerate((N.SPH_PERIODIC_X, N.SPH_PERIODIC_Y, N.SPH_PERIODIC_Z)):
+            if self.periodic[k]: f |= flag
+        d.flags = f
+        h = self.support_radius
+        d.h = h
+        lo_pos = x.min(axis=0)
+        hi_pos = x.max(axis=0)
+        extent = hi_pos - lo_pos
+        cap = max(16 * n, 1 << 20)
+        los, nbs = [0, 0, 0], [1, 1, 1]
+        for k in range(3):
+            if self.periodic[k]:
+                L = float(self.domain_size[k])
+                nc = int(math.floor(L / h))
+                if nc < 3:
+                    raise SystemExit("periodic axis %s must be at least three support radii long" % "xyz"[k])
+                d.cell[k] = L / nc
+                d.origin[k] = float(self.domain_start[k])
+                d.period[k] = L
+                los[k], nbs[k] = 0, nc
+                continue
+            d.cell[k] = h
+            d.origin[k] = 0.0
+            d.period[k] = 0.0
+            if self.two_d and k == 2:
+                los[k], nbs[k] = 0, 1
+                continue
+            a, b = lo_pos[k], hi_pos[k]
+            if self.walls:
+                a, b = min(a, 0.0), max(b, float(self.domain_size[k]))
+            else:
+                margin = 0.25 * extent[k] + 2.0 * h
+                a, b = a - margin, b + margin
+            los[k] = int(math.floor(a / h)) - 1
+            nbs[k] = int(math.floor(b / h)) + 2 - los[k]
+        if nbs[0] * nbs[1] * nbs[2] > cap:
+            # a sparse cloud over a large region: the dense box covers the particle
+            # bounding box only, or nothing, and the hashed table takes the rest
+            for k in range(3):
+                if self.periodic[k] or (self.two_d and k == 2):
+                    continue
+                los[k] = int(math.floor(lo_pos[k] / h)) - 1
+                nbs[k] = int(math.floor(hi_pos[k] / h)) + 2 - los[k]
+            if nbs[0] * nbs[1] * nbs[2] > cap:
+                for k in range(3):
+                    if not self.periodic[k]:
+                        los[k], nbs[k] = 0, 1
+        for k in range(3):
+            d.box_lo[k] = los[k]
+            d.box_n[k] = nbs[k]
+        d.hash_bits = int(min(24, max(12, math.ceil(math.log2(max(2 * n if nbs == [1, 1, 1] else n // 8, 4096))))))
+        return d
+
+    def snapshot(self):
+        """The initial particles as float64 numpy arrays in pid order."""
+        n = len(self.x_init)
+        z3 = np.zeros((n, 3))
+        return {"pid": np.arange(n, dtype=np.int32), "obj": self.obj_init, "x": self.x_init.copy(),
+                "x0": self.x_init.copy(), "u": z3.copy(), "v": self.v_init.copy(), "shift": z3.copy(),
+                "V": self.V_init.copy(), "m": self.m_init.copy(), "p": np.zeros(n)}
+
+
+class ParticleSystem(SceneLayout):
+    def __init__(self, config, GGUI=False, precision="fp64", device="cuda"):
+        if GGUI:
+            raise ValueError("the GGUI viewer is not part of this build; run headless")
+        self.GGUI = False
+        self.device = torch.device(device)
+        super().__init__(config, precision)
+        x, v, m, V, obj = self.x_init, self.v_init, self.m_init, self.V_init, self.obj_init
+        n = len(x)
+        # ---- device state -----------------------------------------------------------
+        dev, R = self.device, self.real
+
+        def buf():
+            return {
+                "pid": torch.zeros(n, dtype=torch.int32, device=dev),
+                "obj": torch.zeros(n, dtype=torch.int32, device=dev),
+                "x": torch.zeros(n, 4, dtype=torch.float64, device=dev),
+                "x0": torch.zeros(n, 4, dtype=torch.float64, device=dev),
+                "u": torch.zeros(n, 4, dtype=torch.float64, device=dev),
+                "v": torch.zeros(n, 4, dtype=torch.float64, device=dev),
+                "shift": torch.zeros(n, 4, dtype=R, device=dev),
+                "V": torch.zeros(n, dtype=R, device=dev),
+                "V_lo": torch.zeros(n, dtype=R, device=dev),
+                "m": torch.zeros(n, dtype=R, device=dev),
+                "p": torch.zeros(n, dtype=R, device=dev),
+            }
+        self._bufs = [buf(), buf()]
+        self._derived = {
+            "acc": torch.zeros(n, 4, dtype=R, device=dev),
+            "lam": torch.zeros(n, dtype=R, device=dev),
+            "L6": torch.zeros(n, 8, dtype=R, device=dev),
+            "lam_grad": torch.zeros(n, 4, dtype=R, device=dev),
+            "div": torch.zeros(n, dtype=R, device=dev),
+            "de_visc": torch.zeros(n, dtype=R, device=dev),
+            "V_new": torch.zeros(n, dtype=R, device=dev),
+        }
+        self.scal = torch.zeros(N.SPH_NSCAL, dtype=torch.float64, device=dev)
+        b0 = self._bufs[0]
+        b0["pid"].copy_(torch.arange(n, dtype=torch.int32))
+        b0["obj"].copy_(torch.from_numpy(obj))
+        b0["x"][:, :3].copy_(torch.from_numpy(x))
+        b0["x0"][:, :3].copy_(torch.from_numpy(x))
+        b0["v"][:, :3].copy_(torch.from_numpy(v))
+        b0["V"].copy_(torch.from_numpy(V))
+        b0["m"].copy_(torch.from_numpy(m))
+
+        self.state = N.SphState()
+        for k, b in enumerate(self._bufs):
+            s = self.state.buf[k]
+            for name in ("pid", "obj", "x", "x0", "u", "v", "shift", "V", "V_lo", "m", "p"):
+                setattr(s, name, b[name].data_ptr())
+        self.state.cur = 0
+        for name, t in self._derived.items():
+            setattr(self.state, name, t.data_ptr())
+        self.state.scal = self.scal.data_ptr()
+
+        self._lib = N.lib()
+        self.ctx = ctypes.c_void_p()
+        N.check(self._lib.sph_ctx_create(ctypes.byref(self.ctx), ctypes.byref(self.dims)), None,
+                "sph_ctx_create")
+
+        # ---- reference-named views ----------------------------------------------------
+        self.particle_num = Scalar(value=n, dtype=int)
+        self.off_grid_num = Scalar(self.scal, N.SPH_S_OFF_GRID_NUM, dtype=int)
+        self.off_grid_mass = Scalar(self.scal, N.SPH_S_OFF_GRID_MASS)
+        L = self._live
+        self.object_id = Field(lambda: L()["obj"], 1, "object_id")
+        self.pid = Field(lambda: L()["pid"], 1, "pid")
+        self.x = Field(lambda: L()["x"], 3, "x")
+        self.x_0 = Field(lambda: L()["x0"], 3, "x_0")
+        self.u = Field(lambda: L()["u"], 3, "u")
+        self.v = Field(lambda: L()["v"], 3, "v")
+        self.pst_shift = Field(lambda: L()["shift"], 3, "pst_shift")
+        self.V = Field(lambda: L()["V"], 1, "V")
+        self.V_lo = Field(lambda: L()["V_lo"], 1, "V_lo")
+        self.m = Field(lambda: L()["m"], 1, "m")
+        self.pressure = Field(lambda: L()["p"], 1, "pressure")
+        self.acceleration = Field(lambda: self._derived["acc"], 3, "acceleration")
+        self.lam = Field(lambda: self._derived["lam"], 1, "lam")
+        self.divergence = Field(lambda: self._derived["div"], 1, "divergence")
+
+    # ---------------------------------------------------------------------------------
+    def _live(self):
+        return self._bufs[self.state.cur]
+
+    def stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    # ---------------------------------------------------------------------------------
+    def initialize_particle_system(self):
+        """update_grid_id + prefix sum + counting_sort (particle_system.py:1342-1345)."""
+        N.check(self._lib.sph_sort(self.ctx, ctypes.byref(self.state), self.stream()), self.ctx, "sph_sort")
+
+    def build_solver(self):
+        method = self.cfg.get_cfg("simulationMethod")
+        if method == "deltaPlusSPH":
+            from .DSPH import DSPHSolver
+            return DSPHSolver(self)
+        if method == "hypoElastic":
+            raise NotImplementedError("the hypoelastic solid step is not built yet")
+        raise ValueError("simulationMethod must be 'deltaPlusSPH' or 'hypoElastic', got: %r" % (method,))
+
+    def neighbor_pairs(self, cap=None):
+        """Directed neighbour pairs (pid_i, pid_j, mirror) of the current state, as the
+        pair passes enumerate them; a (k, 3) int32 numpy array."""
+        n = self.particle_max_num
+        cap = cap or 128 * n
+        out = torch.empty(cap, 3, dtype=torch.int32, device=self.device)
+        cnt = torch.zeros(1, dtype=torch.int64, device=self.device)
+        rc = self._lib.sph_neighbor_pairs(self.ctx, ctypes.byref(self.state), out.data_ptr(), cap,
+                                          cnt.data_ptr(), self.stream())
+        if rc == N.SPH_ECAPACITY:
+            return self.neighbor_pairs(int(cnt.item()) + 16)
+        N.check(rc, self.ctx, "sph_neighbor_pairs")
+        k = int(cnt.item())
+        return out[:k].cpu().numpy()
+
+    def sorted_by_pid(self, field):
+        """`field.to_numpy()` reordered by stable particle id."""
+        order = np.argsort(self.pid.to_numpy(), kind="stable")
+        return field.to_numpy()[order]
+
+    def resync_displacement(self):
+        """u = x - x_0 (particle_system.py:1135-1140)."""
+        b = self._live()
+        b["u"][:, :3] = b["x"][:, :3] - b["x0"][:, :3]
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self._lib.sph_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass

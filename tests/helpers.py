@@ -1,0 +1,65 @@
+"""Shared builders for the tests: host-side layouts, oracle states, brute-force pairs."""
+import numpy as np
+
+from sigmasph_amd.config import SimConfig
+from sigmasph_amd.particle_system import SceneLayout
+from sigmasph_amd.DSPH import FluidSettings
+from oracle import oracle as O
+
+
+def layout(scene, precision="fp64"):
+    lay = SceneLayout(SimConfig(scene), precision)
+    fs = FluidSettings(lay)
+    return lay, fs, fs.params()
+
+
+def oracle_state(lay, fs):
+    s = O.OracleState.from_snapshot(lay.snapshot())
+    s.scal[0] = fs.dt_initial
+    return s
+
+
+def brute_pairs(x, h, mirror_y=False, mirror_z=False, period=None):
+    """All directed pairs (i, j, mir) with |x_i - M x_j| < h by O(N^2) enumeration, in the
+    canonical predicate r2 < h*h (r2 summed as (dx^2 + dy^2) + dz^2)."""
+    x = np.asarray(x, dtype=np.float64)
+    n = len(x)
+    h2 = h * h
+    out = []
+    codes = [0]
+    if mirror_y:
+        codes.append(1)
+    if mirror_z:
+        codes.append(2)
+    if mirror_y and mirror_z:
+        codes.append(3)
+    for mir in codes:
+        xm = x.copy()
+        if mir & 1:
+            xm[:, 1] = -xm[:, 1]
+        if mir & 2:
+            xm[:, 2] = -xm[:, 2]
+        # image sweeps only run for particles within h of the plane (as the traversal)
+        for i in range(n):
+            if mir & 1 and not x[i, 1] < h:
+                continue
+            if mir & 2 and not x[i, 2] < h:
+                continue
+            t = xm[i][None, :] - x          # M x_i - x_j
+            if period is not None:
+                for k in range(3):
+                    if period[k]:
+                        L = period[k]
+                        t[:, k] = np.where(t[:, k] > 0.5 * L, t[:, k] - L,
+                                           np.where(t[:, k] < -0.5 * L, t[:, k] + L, t[:, k]))
+            r2 = (t[:, 0] * t[:, 0] + t[:, 1] * t[:, 1]) + t[:, 2] * t[:, 2]
+            js = np.nonzero(r2 < h2)[0]
+            for j in js:
+                if mir == 0 and j == i:
+                    continue
+                out.append((i, j, mir))
+    return np.array(out, dtype=np.int64).reshape(-1, 3)
+
+
+def pair_set(pairs):
+    return set(map(tuple, np.asarray(pairs).tolist()))

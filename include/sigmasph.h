@@ -119,6 +119,8 @@ typedef struct {
     void*   div;            /* [n]    Real -div v (the volumetric strain rate)       */
     void*   de_visc;        /* [n]    Real viscous specific power v.a_visc           */
     void*   V_new;          /* [n]    Real volume after this step's update           */
+    void*   rho;            /* [n]    Real m / V at the sort (written by sph_sort)   */
+    void*   rho_new;        /* [n]    Real m / V after the volume update             */
     double* scal;           /* [SPH_NSCAL] device scalars                            */
 } sph_state;
 

@@ -81,6 +81,8 @@ class SphState(ctypes.Structure):
         ("div", ctypes.c_void_p),
         ("de_visc", ctypes.c_void_p),
         ("V_new", ctypes.c_void_p),
+        ("rho", ctypes.c_void_p),
+        ("rho_new", ctypes.c_void_p),
         ("scal", ctypes.c_void_p),
     ]
 

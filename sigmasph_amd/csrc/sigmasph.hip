@@ -36,6 +36,7 @@ template <typename R> struct DState {
     DBuf<R> b;
     typename R4t<R>::type* acc;
     R* lam; R* L6; typename R4t<R>::type* lam_grad; R* div; R* de_visc; R* V_new;
+    R* rho; R* rho_new;
     double* scal;
     int n;
 };
@@ -221,12 +222,26 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_rank(int n, DScratch s, const int
 }
 
 struct PermField { const void* src; void* dst; int bytes; };
-struct PermTable { PermField f[16]; int nf; };
+struct PermTable {
+    PermField f[16];
+    int nf;
+    int real_bytes;                        // 8: fp64, 4: mixed
+    const void *m, *V, *V_lo;              // source mass / volume (V_lo: mixed only)
+    void* rho;                             // out: m / (V + V_lo) in the sorted order
+};
 
 __global__ __launch_bounds__(SPH_BLOCK) void k_permute(int n, PermTable T, DScratch s, int off_bucket_lo) {
     int k = blockIdx.x * SPH_BLOCK + threadIdx.x;
     if (k >= n) return;
     int src = s.perm[k];
+    if (T.rho) {
+        if (T.real_bytes == 8) {
+            ((double*)T.rho)[k] = ((const double*)T.m)[src] / ((const double*)T.V)[src];
+        } else {
+            double Vf = (double)((const float*)T.V)[src] + (double)((const float*)T.V_lo)[src];
+            ((float*)T.rho)[k] = (float)((double)((const float*)T.m)[src] / Vf);
+        }
+    }
     for (int f = 0; f < T.nf; ++f) {
         const PermField& F = T.f[f];
         switch (F.bytes) {
@@ -255,9 +270,26 @@ template <typename R> __device__ __forceinline__ R mwq(const DFluid& F, const DB
     return m;
 }
 
-template <typename R> __device__ __forceinline__ R tait(const DFluid& F, R rho) {
-    R p = (R)F.S * (pow(rho / (R)F.rho0, (R)F.gamma) - (R)1);
-    if (!F.allow_neg) p = p > (R)0 ? p : (R)0;
+// fast square root / reciprocal for the float pair arithmetic (1 ulp); exact in f64
+template <typename R> __device__ __forceinline__ R fsqrt(R x);
+template <> __device__ __forceinline__ float fsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+template <> __device__ __forceinline__ double fsqrt<double>(double x) { return sqrt(x); }
+template <typename R> __device__ __forceinline__ R frcp(R x);
+template <> __device__ __forceinline__ float frcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ __forceinline__ double frcp<double>(double x) { return 1.0 / x; }
+
+// Blocks are dispatched round-robin over the 8 XCDs; renumber them so that each XCD
+// works on one contiguous slab of the sorted particles and its L2 sees the slab's
+// neighbours, not every eighth block's.
+__device__ __forceinline__ int xcd_block() {
+    int nb = gridDim.x, b = blockIdx.x;
+    int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x * q + min(x, r) + k;
+}
+
+__device__ __forceinline__ double tait_d(const DFluid& F, double rho) {
+    double p = F.S * (pow(rho / F.rho0, F.gamma) - 1.0);
+    if (!F.allow_neg) p = p > 0.0 ? p : 0.0;
     return p;
 }
 
@@ -279,7 +311,7 @@ template <typename R> __device__ __forceinline__ R gradw_c(const DFluid& F, R rn
 template <typename R>
 __global__ __launch_bounds__(SPH_BLOCK) void k_pass_volume(DGrid g, DFluid F, DState<R> st, DScratch s) {
     __shared__ double red[SPH_BLOCK / SPH_WAVE];
-    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    int i = xcd_block() * SPH_BLOCK + threadIdx.x;
     double wdiff = 0.0;
     if (i < st.n) {
         const DBuf<R>& b = st.b;
@@ -287,13 +319,12 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_pass_volume(DGrid g, DFluid F, DS
         const R inv_dt = (R)(1.0 / dt);
         double4 xi = b.x[i];
         double4 vi4 = b.v[i];
-        R vix = (R)vi4.x, viy = (R)vi4.y, viz = (R)vi4.z;
-        R rho_i = b.m[i] / b.V[i];
+        const R rho_i = st.rho[i];
+        const R inv_rho_i = frcp<R>(rho_i);
         R duix = 0, duiy = 0, duiz = 0;
         if (F.pst_en) {
             typename R4t<R>::type sh = b.shift[i];
             duix = sh.x * inv_dt; duiy = sh.y * inv_dt; duiz = sh.z * inv_dt;
-            vix += duix; viy += duiy; viz += duiz;
         }
         const bool two_d = (F.flags & SPH_TWO_D) != 0;
         R e00 = 0, e11 = 0, e22 = 0, e01 = 0, e02 = 0, e12 = 0;
@@ -303,7 +334,7 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_pass_volume(DGrid g, DFluid F, DS
         auto task = [&](int j, int mir, double rxd, double ryd, double rzd) {
             R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
             R r2 = rx * rx + ry * ry + rz * rz;
-            R rn = sqrt(r2);
+            R rn = fsqrt<R>(r2);
             R c = gradw_c<R>(F, rn);
             R wj = wq<R>(F, b, b.V, j);
             // E_i += w_j (x_j - x_i) (x) grad W = -w_j c r (x) r
@@ -311,26 +342,27 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_pass_volume(DGrid g, DFluid F, DS
             e00 += sc * rx * rx; e11 += sc * ry * ry; e01 += sc * rx * ry;
             if (!two_d) { e22 += sc * rz * rz; e02 += sc * rx * rz; e12 += sc * ry * rz; }
             R gx = c * rx, gy = c * ry, gz = c * rz;
-            R rho_j = b.m[j] / b.V[j];
+            R rho_j = st.rho[j];
             double4 vj4 = b.v[j];
-            R vjx = (R)vj4.x, vjy = (R)vj4.y, vjz = (R)vj4.z;
-            if (mir & 1) vjy = -vjy;
-            if (mir & 2) vjz = -vjz;
+            if (mir & 1) vj4.y = -vj4.y;
+            if (mir & 2) vj4.z = -vj4.z;
+            // v_i - v_j in f64 before narrowing (the difference is what the sum needs)
+            R dvx = (R)(vi4.x - vj4.x), dvy = (R)(vi4.y - vj4.y), dvz = (R)(vi4.z - vj4.z);
             if (F.pst_en) {
                 typename R4t<R>::type sh = b.shift[j];
                 R djx = sh.x * inv_dt, djy = sh.y * inv_dt, djz = sh.z * inv_dt;
                 if (mir & 1) djy = -djy;
                 if (mir & 2) djz = -djz;
-                vjx += djx; vjy += djy; vjz += djz;
+                dvx += duix - djx; dvy += duiy - djy; dvz += duiz - djz;
                 R fl = (rho_j * djx - rho_i * duix) * gx + (rho_j * djy - rho_i * duiy) * gy +
                        (rho_j * djz - rho_i * duiz) * gz;
-                rate += wj * fl / rho_i;
+                rate += wj * fl * inv_rho_i;
             }
-            rate += wj * ((vix - vjx) * gx + (viy - vjy) * gy + (viz - vjz) * gz);
+            rate += wj * (dvx * gx + dvy * gy + dvz * gz);
             if (F.delta > 0.0) {
                 // psi_ij . grad W with psi = 2 (rho_j - rho_i)(-r)/(r^2 + eta^2) and grad W = c r
-                R pg = (R)2 * (rho_j - rho_i) * (-c * r2) / (r2 + eta2);
-                R dr = dhc * wj * pg / rho_i;
+                R pg = (R)2 * (rho_j - rho_i) * (-c * r2) * frcp<R>(r2 + eta2);
+                R dr = dhc * wj * pg * inv_rho_i;
                 rate += dr;
                 drate += dr;
             }
@@ -367,17 +399,22 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_pass_volume(DGrid g, DFluid F, DS
         R Vi = b.V[i];
         R dVdt = -Vi * dv;
         R Vn;
+        double Vfull;
         if (sizeof(R) == 4 && F.comp_vol) {
             double sum = (double)Vi + (double)b.V_lo[i] + (double)((R)dt * dVdt);
             Vn = (R)sum;
             b.V_lo[i] = (R)(sum - (double)Vn);
+            Vfull = sum;
         } else {
             Vn = Vi + (R)dt * dVdt;
+            Vfull = (double)Vn;
         }
         st.V_new[i] = Vn;
-        // EOS on the updated density (DSPH.py:1514-1516); p_i is read by no other
+        st.rho_new[i] = (R)((double)b.m[i] / Vfull);
+        // EOS on the updated density (DSPH.py:1514-1516), in f64 from the compensated
+        // volume: (rho/rho0)^gamma - 1 cancels near rho0; p_i is read by no other
         // particle in this pass, only by its own diffusion work above
-        b.p[i] = tait<R>(F, b.m[i] / Vn);
+        b.p[i] = (R)tait_d(F, (double)b.m[i] / Vfull);
     }
     double bs = block_sum(wdiff, red);
     if (threadIdx.x == 0) s.part[PART_WDIFF * gridDim.x + blockIdx.x] = bs;
@@ -390,7 +427,7 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_pass_volume(DGrid g, DFluid F, DS
 template <typename R>
 __global__ __launch_bounds__(SPH_BLOCK) void k_pass_force(DGrid g, DFluid F, DState<R> st, DScratch s) {
     __shared__ double red[SPH_BLOCK / SPH_WAVE];
-    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    int i = xcd_block() * SPH_BLOCK + threadIdx.x;
     double c2loc = F.csig2, amag = 0.0;
     if (i < st.n) {
         const DBuf<R>& b = st.b;
@@ -400,7 +437,7 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_pass_force(DGrid g, DFluid F, DSt
         double4 xi = b.x[i];
         double4 vi4 = b.v[i];
         R vix = (R)vi4.x, viy = (R)vi4.y, viz = (R)vi4.z;
-        R rho_i = b.m[i] / Vn[i];
+        const R rho_i = st.rho_new[i];
         R p_i = b.p[i];
         R lam_i = st.lam[i];
         R glx = 0, gly = 0, glz = 0;    // Sum w_old (lam_j - lam_i) grad W
@@ -414,7 +451,7 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_pass_force(DGrid g, DFluid F, DSt
         auto task = [&](int j, int mir, double rxd, double ryd, double rzd) {
             R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
             R r2 = rx * rx + ry * ry + rz * rz;
-            R rn = sqrt(r2);
+            R rn = fsqrt<R>(r2);
             R c = gradw_c<R>(F, rn);
             R gx = c * rx, gy = c * ry, gz = c * rz;
             R wo = wq<R>(F, b, b.V, j);
@@ -425,28 +462,28 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_pass_force(DGrid g, DFluid F, DSt
             R pp = wj * (p_i + p_j);
             apx += pp * gx; apy += pp * gy; apz += pp * gz;
             double4 vj4 = b.v[j];
-            R vjx = (R)vj4.x, vjy = (R)vj4.y, vjz = (R)vj4.z;
-            if (mir & 1) vjy = -vjy;
-            if (mir & 2) vjz = -vjz;
+            if (mir & 1) vj4.y = -vj4.y;
+            if (mir & 2) vj4.z = -vj4.z;
+            R dvx = (R)(vi4.x - vj4.x), dvy = (R)(vi4.y - vj4.y), dvz = (R)(vi4.z - vj4.z);
             if (F.visc > 0.0) {
-                R vr = (vix - vjx) * rx + (viy - vjy) * ry + (viz - vjz) * rz;
-                R fv = wj * vr / (r2 + eta2);
+                R vr = dvx * rx + dvy * ry + dvz * rz;
+                R fv = wj * vr * frcp<R>(r2 + eta2);
                 avx += fv * gx; avy += fv * gy; avz += fv * gz;
             }
-            R rho_j = b.m[j] / Vn[j];
+            R rho_j = st.rho_new[j];
             if (ale) {
                 typename R4t<R>::type sh = b.shift[j];
                 R djx = sh.x * inv_dt, djy = sh.y * inv_dt, djz = sh.z * inv_dt;
                 if (mir & 1) djy = -djy;
                 if (mir & 2) djz = -djz;
                 R t = wj * rho_j * (djx * gx + djy * gy + djz * gz);
-                aax += t * (vjx - vix); aay += t * (vjy - viy); aaz += t * (vjz - viz);
+                aax -= t * dvx; aay -= t * dvy; aaz -= t * dvz;
             }
             if (F.dt_pair) {
                 R drho = fabs(rho_i - rho_j);
                 if (drho > drho_eps) {
-                    R c2 = fabs(p_i - p_j) / drho;
-                    c2max = c2 > c2max ? c2 : c2max;
+                    R dp = fabs(p_i - p_j);
+                    if (dp > c2max * drho) c2max = dp / drho;   // divides only when the max grows
                 }
             }
         };
@@ -469,7 +506,7 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_pass_force(DGrid g, DFluid F, DSt
         R ax = (R)F.g[0] + kp * apx + kv * avx;
         R ay = (R)F.g[1] + kp * apy + kv * avy;
         R az = (R)F.g[2] + kp * apz + kv * avz;
-        if (ale) { ax += aax / rho_i; ay += aay / rho_i; az += aaz / rho_i; }
+        if (ale) { R ir = (R)1 / rho_i; ax += aax * ir; ay += aay * ir; az += aaz * ir; }
         typename R4t<R>::type a4; a4.x = ax; a4.y = ay; a4.z = az; a4.w = 0;
         st.acc[i] = a4;
         if (F.track_visc) st.de_visc[i] = vix * kv * avx + viy * kv * avy + viz * kv * avz;
@@ -560,18 +597,18 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_advect(DGrid g, DFluid F, DState<
 // ------------------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(SPH_BLOCK) void k_pass_shift(DGrid g, DFluid F, DState<R> st, DScratch s) {
-    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    int i = xcd_block() * SPH_BLOCK + threadIdx.x;
     if (i >= st.n) return;
     const DBuf<R>& b = st.b;
     double4 xi = b.x[i];
-    R rho_i = b.m[i] / b.V[i];
+    R rho_i = st.rho_new[i];
     R sx = 0, sy = 0, sz = 0;
     const R inv_wdx0 = F.Wdx0 > 1e-10 ? (R)(1.0 / F.Wdx0) : (R)0;
     const R alpha = (R)F.alpha;
     auto task = [&](int j, int mir, double rxd, double ryd, double rzd) {
         R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
         R r2 = rx * rx + ry * ry + rz * rz;
-        R rn = sqrt(r2);
+        R rn = fsqrt<R>(r2);
         R c = gradw_c<R>(F, rn);
         R q = rn * (R)F.inv_h;
         R W = 0;
@@ -589,7 +626,7 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_pass_shift(DGrid g, DFluid F, DSt
         } else {
             R mj = mwq<R>(F, b, j);
             if (mj > (R)1e-10) {
-                R rho_j = b.m[j] / b.V[j];
+                R rho_j = st.rho_new[j];
                 R ra = (R)0.5 * (rho_i + rho_j);
                 if (ra > (R)1e-10) wgt = mj / ra;
             }
@@ -1014,6 +1051,7 @@ template <typename R> static DState<R> dstate(const sph_ctx* c, const sph_state*
     d.lam = (R*)st->lam; d.L6 = (R*)st->L6;
     d.lam_grad = (typename R4t<R>::type*)st->lam_grad;
     d.div = (R*)st->div; d.de_visc = (R*)st->de_visc; d.V_new = (R*)st->V_new;
+    d.rho = (R*)st->rho; d.rho_new = (R*)st->rho_new;
     d.scal = st->scal;
     d.n = (int)c->dims.n;
     return d;
@@ -1032,7 +1070,8 @@ static int check_state(sph_ctx* c, const sph_state* st, bool full) {
     if (full) {
         if (!buf_ok(st->buf[1 - st->cur], mixed))
             return set_err(c, SPH_EINVAL, "null field in alternate buffer%s", "", 0);
-        if (!st->acc || !st->lam || !st->L6 || !st->lam_grad || !st->div || !st->de_visc || !st->V_new)
+        if (!st->acc || !st->lam || !st->L6 || !st->lam_grad || !st->div || !st->de_visc || !st->V_new ||
+            !st->rho || !st->rho_new)
             return set_err(c, SPH_EINVAL, "null derived field%s", "", 0);
     }
     return SPH_OK;
@@ -1043,6 +1082,7 @@ extern "C" int sph_sort(sph_ctx* c, sph_state* st, void* stream) {
     if (rc) return rc;
     if (!buf_ok(st->buf[1 - st->cur], c->dims.precision == SPH_PREC_MIXED))
         return set_err(c, SPH_EINVAL, "null field in alternate buffer%s", "", 0);
+    if (!st->rho) return set_err(c, SPH_EINVAL, "null rho%s", "", 0);
     hipStream_t hs = (hipStream_t)stream;
     int n = (int)c->dims.n;
     int nb = c->nblk;
@@ -1071,6 +1111,9 @@ extern "C" int sph_sort(sph_ctx* c, sph_state* st, void* stream) {
     add(A.V, B.V, rb); add(A.m, B.m, rb); add(A.p, B.p, rb);
     if (mixed) add(A.V_lo, B.V_lo, rb);
     T.nf = nf;
+    T.real_bytes = rb;
+    T.m = A.m; T.V = A.V; T.V_lo = A.V_lo;
+    T.rho = st->rho;
     hipLaunchKernelGGL(k_permute, dim3(nb), dim3(SPH_BLOCK), 0, hs, n, T, c->s, c->grid.nbox);
     timer_end(c, 3, hs);
     rc = check_launch(c, "sort");

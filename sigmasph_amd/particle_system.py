@@ -313,6 +313,8 @@ class ParticleSystem(SceneLayout):
             "div": torch.zeros(n, dtype=R, device=dev),
             "de_visc": torch.zeros(n, dtype=R, device=dev),
             "V_new": torch.zeros(n, dtype=R, device=dev),
+            "rho": torch.zeros(n, dtype=R, device=dev),
+            "rho_new": torch.zeros(n, dtype=R, device=dev),
         }
         self.scal = torch.zeros(N.SPH_NSCAL, dtype=torch.float64, device=dev)
         b0 = self._bufs[0]

@@ -44,7 +44,7 @@ STRUCTS = {
                              "box_n", "hash_bits"]),
     "sph_sorted": (N.SphSorted, ["pid", "obj", "x", "x0", "u", "v", "shift", "V", "V_lo", "m", "p"]),
     "sph_state": (N.SphState, ["buf", "cur", "acc", "lam", "L6", "lam_grad", "div", "de_visc", "V_new",
-                               "scal"]),
+                               "rho", "rho_new", "scal"]),
     "sph_fluid_params": (N.SphFluidParams, [f[0] for f in N.SphFluidParams._fields_]),
 }
 

@@ -36,21 +36,35 @@ def run_pair(scene, steps=1, precision="fp64"):
     return ps, solver, s
 
 
-def scaled_err(gpu, ora):
+def scaled_err(gpu, ora, floor=0.0):
+    """max |gpu - oracle| over max(max |oracle|, floor)."""
     gpu = np.asarray(gpu, dtype=np.float64)
     ora = np.asarray(ora, dtype=np.float64)
-    scale = np.abs(ora).max()
+    scale = max(np.abs(ora).max(), floor)
     if scale == 0.0:
         return np.abs(gpu).max()
     return np.abs(gpu - ora).max() / scale
 
 
+def floors(ps, s):
+    """Physical scales below which a field is round-off around zero: -div v against the
+    transport speed over h, the pressure against 1e-6 of the EOS stiffness, the
+    acceleration against that pressure over rho0 h."""
+    c = ps.cfg.get_cfg
+    S = float(c("c0")) ** 2 * float(c("density0")) / float(c("exponent"))
+    h = ps.support_radius
+    U = np.abs(s.v).max() + np.abs(s.shift).max() / s.scal[0]
+    return {"divergence": U / h, "pressure": 1e-6 * S,
+            "acceleration": 1e-6 * S / (float(c("density0")) * h)}
+
+
 def compare(ps, s, tol, fields=FIELDS):
     errs = {}
+    fl = floors(ps, s)
     for fname, oname in fields:
         g = ps.sorted_by_pid(getattr(ps, fname))
         o = s.by_pid(oname)
-        errs[fname] = scaled_err(g, o)
+        errs[fname] = scaled_err(g, o, fl.get(fname, 0.0))
     bad = {k: v for k, v in errs.items() if not v <= tol}
     assert not bad, "fields beyond %g: %s (all: %s)" % (tol, bad, errs)
     return errs

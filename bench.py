@@ -123,12 +123,7 @@ def main():
     dom = max(("pass_volume", "pass_force", "pass_shift"), key=lambda k: avg[k][0])
     dur_s = avg[dom][0] * 1e-3
     achieved = bm[dom] * n / dur_s / 1e9 if dur_s > 0 else 0.0
-    pairs = None
-    try:
-        pr = ps.neighbor_pairs()
-        pairs = len(pr) / n
-    except Exception as e:  # noqa: BLE001 - diagnostic only
-        log("pair count failed: %s" % e)
+    pairs = ps.neighbor_count() / n
     out = {
         "metric": "particle-steps/s (1 GPU)",
         "value": rate,

@@ -88,6 +88,7 @@ typedef struct {
     int32_t box_n[3];       /* dense cells per axis                                  */
     int32_t hash_bits;      /* log2 of the overflow table size                       */
     int32_t pad_;
+    double  dx0;            /* lattice spacing: sizes the cell tiles (0 = h / 2.6)    */
 } sph_dims;
 
 /* One of the two buffers every sorted (persistent) field is double-buffered in.  The
@@ -176,7 +177,8 @@ int  sph_sort(sph_ctx* ctx, sph_state* st, void* stream);
 /* Directed neighbour pairs of the live buffer as (pid_i, pid_j, mirror code) triples,
  * enumerated exactly as every pair pass visits them (for_all_neighbors,
  * particle_system.py:1348-1490).  `pairs` is device int32[cap][3]; `count` is a device
- * int64 that receives the total (which may exceed cap: SPH_ECAPACITY). */
+ * int64 that receives the total (which may exceed cap: SPH_ECAPACITY).  With pairs NULL
+ * and cap 0 only the count is produced. */
 int  sph_neighbor_pairs(sph_ctx* ctx, const sph_state* st, int32_t* pairs, int64_t cap,
                         int64_t* count, void* stream);
 

@@ -251,7 +251,7 @@ void or_for_all_neighbors(const sph_dims* d, const or_state* s, const or_grid* g
         double xm[3] = { xi[0], -xi[1], xi[2] };
         for (int dz = -1; dz <= 1; ++dz)
             for (int dx = -1; dx <= 1; ++dx) {
-                int32_t c[3] = { cc[0] + dx, 0, cc[2] + dz };
+                int32_t c[3] = { or_wrap_cell(d, 0, cc[0] + dx), 0, or_wrap_cell(d, 2, cc[2] + dz) };
                 or_visit_cell(d, s, g, i, c, 1, xm, h2, task, ctx);
             }
     }
@@ -259,7 +259,7 @@ void or_for_all_neighbors(const sph_dims* d, const or_state* s, const or_grid* g
         double xm[3] = { xi[0], xi[1], -xi[2] };
         for (int dy = -1; dy <= 1; ++dy)
             for (int dx = -1; dx <= 1; ++dx) {
-                int32_t c[3] = { cc[0] + dx, cc[1] + dy, 0 };
+                int32_t c[3] = { or_wrap_cell(d, 0, cc[0] + dx), or_wrap_cell(d, 1, cc[1] + dy), 0 };
                 or_visit_cell(d, s, g, i, c, 2, xm, h2, task, ctx);
             }
     }
@@ -267,7 +267,7 @@ void or_for_all_neighbors(const sph_dims* d, const or_state* s, const or_grid* g
         xi[1] < d->h && xi[2] < d->h) {
         double xm[3] = { xi[0], -xi[1], -xi[2] };
         for (int dx = -1; dx <= 1; ++dx) {
-            int32_t c[3] = { cc[0] + dx, 0, 0 };
+            int32_t c[3] = { or_wrap_cell(d, 0, cc[0] + dx), 0, 0 };
             or_visit_cell(d, s, g, i, c, 3, xm, h2, task, ctx);
         }
     }

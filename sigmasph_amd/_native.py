@@ -61,6 +61,7 @@ class SphDims(ctypes.Structure):
         ("box_n", _i3),
         ("hash_bits", ctypes.c_int32),
         ("pad_", ctypes.c_int32),
+        ("dx0", ctypes.c_double),
     ]
 
 

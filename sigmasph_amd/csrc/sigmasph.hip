@@ -19,6 +19,7 @@
 #include <string.h>
 #include <string>
 #include <vector>
+#include <algorithm>
 
 #define SPH_VERSION "sigmasph-amd 0.1 (gfx950)"
 
@@ -305,78 +306,316 @@ template <typename R> __device__ __forceinline__ R gradw_c(const DFluid& F, R rn
 }
 
 // ------------------------------------------------------------------------------------
+// cell tiles
+//
+// One workgroup serves a tile: B consecutive x-cells of one (y, z) row of the dense box,
+// whose particles are one contiguous range of the sorted order.  For each of the nine
+// stencil rows (oy, oz) the positions of the row segment [cx_a - 1, cx_b] -- B + 2 cells,
+// again one contiguous range unless a periodic x wrap splits it -- are staged in LDS, and
+// every thread scans only the three cells around its own.  The images across y = 0 and
+// z = 0 are tested against the same staged rows.  Particles a tile cannot serve go to a
+// slow list that the generic traversal finishes in a second launch: one that left its
+// binned cell since the sort (only after advection), a row segment larger than the LDS
+// buffer, tiles on the box edge while particles sit in the hashed table, and every
+// particle of the hashed table itself.
+// ------------------------------------------------------------------------------------
+#define TILE_MAXSEG 512
+#define TILE_MAXB 62
+
+struct TileGeo {
+    int B;        // cells per tile along x
+    int ntx;      // tiles per row
+    int ntiles;   // ntx * nb[1] * nb[2]
+};
+
+struct TileShared {
+    double sx[TILE_MAXSEG], sy[TILE_MAXSEG], sz[TILE_MAXSEG];
+    int sg[TILE_MAXSEG];
+    int off[TILE_MAXB + 4];   // staged offset of local cell k, k = 0 .. nseg
+    int gs[TILE_MAXB + 4];    // global start of local cell k
+    int contiguous;           // every non-empty cell at gbase + off[k]
+    int gbase;
+    double red[SPH_BLOCK / SPH_WAVE];
+};
+
+struct SlowList {
+    int* idx;
+    int* count;
+};
+
+// Runs `body` over every particle of the tile that the tile can serve; the rest are
+// appended to the slow list.  Every thread of the block executes the same barriers.
+template <class Body>
+__device__ void tile_pass(const DGrid& g, const TileGeo& tg, const int* __restrict__ start,
+                          const double4* __restrict__ X, bool any_out, SlowList sl, TileShared& sh,
+                          Body& body) {
+    const int t = xcd_block();
+    const int tx = t % tg.ntx, row = t / tg.ntx;
+    const int cy = g.lo[1] + row % g.nb[1], cz = g.lo[2] + row / g.nb[1];
+    const int cxa = g.lo[0] + tx * tg.B;
+    const int cxb = min(cxa + tg.B, g.lo[0] + g.nb[0]);
+    const int k0 = dense_key(g, cxa, cy, cz);
+    const int ib = start[k0], ie = start[k0 + (cxb - cxa)];
+    if (ib == ie) return;
+    const bool px = (g.periodic & 1) != 0, py = (g.periodic & 2) != 0, pz = (g.periodic & 4) != 0;
+    const bool edge = any_out && ((!px && (cxa == g.lo[0] || cxb == g.lo[0] + g.nb[0])) ||
+                                  (!py && (cy == g.lo[1] || cy == g.lo[1] + g.nb[1] - 1)) ||
+                                  (!pz && (cz == g.lo[2] || cz == g.lo[2] + g.nb[2] - 1)));
+    const bool my = (g.flags & SPH_MIRROR_Y) != 0, mz = (g.flags & SPH_MIRROR_Z) != 0;
+    const bool two_d_skip = (g.flags & SPH_TWO_D) && g.nb[2] == 1;
+    const int nseg = cxb - cxa + 2;
+    for (int base = ib; base < ie; base += blockDim.x) {
+        const int i = base + threadIdx.x;
+        const bool act = i < ie;
+        double xi = 0.0, yi = 0.0, zi = 0.0;
+        bool fast = false;
+        int li = 1;
+        if (act) {
+            body.begin(i, xi, yi, zi);
+            if (!edge && finite3(xi, yi, zi)) {
+                int ccx = cell1(g, 0, xi), ccy = cell1(g, 1, yi), ccz = cell1(g, 2, zi);
+                fast = ccx >= cxa && ccx < cxb && ccy == cy && ccz == cz;
+                li = ccx - cxa + 1;
+                // the image sweeps read the cell rows at y = 0 / z = 0: staged only if adjacent
+                if (my && yi < g.h && (cy < -1 || cy > 1)) fast = false;
+                if (mz && zi < g.h && (cz < -1 || cz > 1)) fast = false;
+            }
+        }
+        if (!edge) {
+            for (int oz = -1; oz <= 1; ++oz) {
+                if (two_d_skip && oz != 0) continue;
+                int zz = pz ? wrap_cell(g, 2, cz + oz) : cz + oz;
+                for (int oy = -1; oy <= 1; ++oy) {
+                    int yy = py ? wrap_cell(g, 1, cy + oy) : cy + oy;
+                    if (yy < g.lo[1] || yy >= g.lo[1] + g.nb[1] || zz < g.lo[2] || zz >= g.lo[2] + g.nb[2])
+                        continue;   // a row outside the box is empty while nothing is outside it
+                    // --- stage the row segment: offsets (wave 0), then positions
+                    if (threadIdx.x < SPH_WAVE) {
+                        int k = threadIdx.x;
+                        int cnt = 0, gst = 0;
+                        if (k < nseg) {
+                            int xx = cxa - 1 + k;
+                            if (px) xx = wrap_cell(g, 0, xx);
+                            if (xx >= g.lo[0] && xx < g.lo[0] + g.nb[0]) {
+                                int key = dense_key(g, xx, yy, zz);
+                                gst = start[key];
+                                cnt = start[key + 1] - gst;
+                            }
+                        }
+                        int inc = cnt;
+#pragma unroll
+                        for (int o = 1; o < SPH_WAVE; o <<= 1) {
+                            int y = __shfl_up(inc, o, 64);
+                            if (k >= o) inc += y;
+                        }
+                        if (k < nseg) { sh.gs[k] = gst; sh.off[k + 1] = inc; }
+                        if (k == 0) sh.off[0] = 0;
+                        // contiguous iff every non-empty cell starts at gbase + its offset
+                        int base_k = gst - (inc - cnt);
+                        unsigned long long ne = __ballot(cnt > 0);
+                        int first = ne ? __ffsll((long long)ne) - 1 : 0;
+                        int gbase = __shfl(base_k, first, 64);
+                        unsigned long long ok = __ballot(cnt == 0 || base_k == gbase);
+                        if (k == 0) { sh.contiguous = (ok == ~0ull); sh.gbase = gbase; }
+                    }
+                    __syncthreads();
+                    const int tot = sh.off[nseg];
+                    const bool ovf = tot > TILE_MAXSEG;
+                    if (!ovf) {
+                        const bool cont = sh.contiguous != 0;
+                        const int gbase = sh.gbase;
+                        for (int s = threadIdx.x; s < tot; s += blockDim.x) {
+                            int gj;
+                            if (cont) {
+                                gj = gbase + s;
+                            } else {
+                                int k = 0;
+                                while (sh.off[k + 1] <= s) ++k;
+                                gj = sh.gs[k] + (s - sh.off[k]);
+                            }
+                            double4 xj = X[gj];
+                            sh.sx[s] = xj.x; sh.sy[s] = xj.y; sh.sz[s] = xj.z;
+                            sh.sg[s] = gj;
+                        }
+                    }
+                    __syncthreads();
+                    if (ovf) {
+                        fast = false;
+                    } else if (fast) {
+                        const int s0 = sh.off[li - 1], s1 = sh.off[li + 2];
+                        const bool imy = my && yi < g.h && yy == 0;
+                        const bool imz = mz && zi < g.h && zz == 0;
+                        const bool imc = imy && imz;
+                        for (int s = s0; s < s1; ++s) {
+                            const double xj = sh.sx[s], yj = sh.sy[s], zj = sh.sz[s];
+                            const int j = sh.sg[s];
+                            const double dx = min_image(g, 0, xi - xj);
+                            if (j != i) {
+                                double dy = min_image(g, 1, yi - yj), dz = min_image(g, 2, zi - zj);
+                                if (dist2_exact(dx, dy, dz) < g.h2) body.pair(j, 0, dx, dy, dz);
+                            }
+                            if (imy) {
+                                double dy = -yi - yj, dz = min_image(g, 2, zi - zj);
+                                if (dist2_exact(dx, dy, dz) < g.h2) body.pair(j, 1, dx, -dy, dz);
+                            }
+                            if (imz) {
+                                double dy = min_image(g, 1, yi - yj), dz = -zi - zj;
+                                if (dist2_exact(dx, dy, dz) < g.h2) body.pair(j, 2, dx, dy, -dz);
+                            }
+                            if (imc) {
+                                double dy = -yi - yj, dz = -zi - zj;
+                                if (dist2_exact(dx, dy, dz) < g.h2) body.pair(j, 3, dx, -dy, -dz);
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+        if (act) {
+            if (fast) {
+                body.end(i);
+            } else {
+                int slot = atomicAdd(sl.count, 1);
+                sl.idx[slot] = i;
+            }
+        }
+    }
+}
+
+// The particles the tiles left: the slow list, then the hashed table and the quarantine
+// bucket (the sorted tail from start[nbox]), each through the generic traversal.
+template <class Body>
+__device__ void slow_pass(const DGrid& g, int n, const int* __restrict__ start, const int4* __restrict__ cellid,
+                          const double4* __restrict__ X, bool any_out, SlowList sl, Body& body) {
+    const int t = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    const int cnt = *sl.count;
+    const int tail0 = start[g.nbox];
+    int i = -1;
+    if (t < cnt) i = sl.idx[t];
+    else if (t - cnt < n - tail0) i = tail0 + (t - cnt);
+    if (i < 0) return;
+    double xi, yi, zi;
+    body.begin(i, xi, yi, zi);
+    auto f = [&](int j, int mir, double rx, double ry, double rz) { body.pair(j, mir, rx, ry, rz); };
+    Traverse tr{g, start, cellid, X, any_out};
+    tr(i, xi, yi, zi, f);
+    body.end(i);
+}
+
+template <class Body>
+__global__ __launch_bounds__(SPH_BLOCK) void k_tiled(DGrid g, TileGeo tg, typename Body::Args a, SlowList sl) {
+    __shared__ TileShared sh;
+    Body body(a);
+    tile_pass(g, tg, a.s.start, a.X, a.any_out(), sl, sh, body);
+    body.reduce(sh.red, blockIdx.x);
+}
+
+template <class Body>
+__global__ __launch_bounds__(SPH_BLOCK) void k_slow(DGrid g, int n, int tile_blocks, typename Body::Args a,
+                                                    SlowList sl) {
+    __shared__ double red[SPH_BLOCK / SPH_WAVE];
+    Body body(a);
+    slow_pass(g, n, a.s.start, a.s.cellid, a.X, a.any_out(), sl, body);
+    body.reduce(red, tile_blocks + blockIdx.x);
+}
+
+__global__ void k_zero_int(int* p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
+// arguments every pair body receives
+template <typename R> struct PassArgs {
+    DFluid F;
+    DState<R> st;
+    DScratch s;
+    const double4* X;
+    int pstride;     // stride of one partial-sum slot
+    __device__ bool any_out() const { return st.scal[SPH_S_OUT_OF_BOX] > 0.5; }
+};
+
+// ------------------------------------------------------------------------------------
 // pass A: E_i (compute_L_task) + continuity (compute_depsvol_dt_task) per pair;
 //         lambda, L_i, -div v, the volume update and the EOS per particle
 // ------------------------------------------------------------------------------------
-template <typename R>
-__global__ __launch_bounds__(SPH_BLOCK) void k_pass_volume(DGrid g, DFluid F, DState<R> st, DScratch s) {
-    __shared__ double red[SPH_BLOCK / SPH_WAVE];
-    int i = xcd_block() * SPH_BLOCK + threadIdx.x;
-    double wdiff = 0.0;
-    if (i < st.n) {
-        const DBuf<R>& b = st.b;
-        const double dt = st.scal[SPH_S_DT];
-        const R inv_dt = (R)(1.0 / dt);
+template <typename R> struct VolumeBody {
+    typedef PassArgs<R> Args;
+    const Args& a;
+    double wdiff_acc;
+    double4 vi4;
+    double xiy;
+    R rho_i, inv_rho_i, duix, duiy, duiz, inv_dt;
+    R e00, e11, e22, e01, e02, e12, rate, drate;
+
+    __device__ VolumeBody(const Args& a_) : a(a_), wdiff_acc(0.0) {}
+
+    __device__ void begin(int i, double& x, double& y, double& z) {
+        const DBuf<R>& b = a.st.b;
         double4 xi = b.x[i];
-        double4 vi4 = b.v[i];
-        const R rho_i = st.rho[i];
-        const R inv_rho_i = frcp<R>(rho_i);
-        R duix = 0, duiy = 0, duiz = 0;
-        if (F.pst_en) {
+        x = xi.x; y = xi.y; z = xi.z;
+        xiy = xi.y;
+        vi4 = b.v[i];
+        rho_i = a.st.rho[i];
+        inv_rho_i = frcp<R>(rho_i);
+        inv_dt = (R)(1.0 / a.st.scal[SPH_S_DT]);
+        duix = duiy = duiz = 0;
+        if (a.F.pst_en) {
             typename R4t<R>::type sh = b.shift[i];
             duix = sh.x * inv_dt; duiy = sh.y * inv_dt; duiz = sh.z * inv_dt;
         }
-        const bool two_d = (F.flags & SPH_TWO_D) != 0;
-        R e00 = 0, e11 = 0, e22 = 0, e01 = 0, e02 = 0, e12 = 0;
-        R rate = 0, drate = 0;
-        const R dhc = (R)(F.delta * F.h * F.c0);
-        const R eta2 = (R)F.eta2;
-        auto task = [&](int j, int mir, double rxd, double ryd, double rzd) {
-            R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
-            R r2 = rx * rx + ry * ry + rz * rz;
-            R rn = fsqrt<R>(r2);
-            R c = gradw_c<R>(F, rn);
-            R wj = wq<R>(F, b, b.V, j);
-            // E_i += w_j (x_j - x_i) (x) grad W = -w_j c r (x) r
-            R sc = -wj * c;
-            e00 += sc * rx * rx; e11 += sc * ry * ry; e01 += sc * rx * ry;
-            if (!two_d) { e22 += sc * rz * rz; e02 += sc * rx * rz; e12 += sc * ry * rz; }
-            R gx = c * rx, gy = c * ry, gz = c * rz;
-            R rho_j = st.rho[j];
-            double4 vj4 = b.v[j];
-            if (mir & 1) vj4.y = -vj4.y;
-            if (mir & 2) vj4.z = -vj4.z;
-            // v_i - v_j in f64 before narrowing (the difference is what the sum needs)
-            R dvx = (R)(vi4.x - vj4.x), dvy = (R)(vi4.y - vj4.y), dvz = (R)(vi4.z - vj4.z);
-            if (F.pst_en) {
-                typename R4t<R>::type sh = b.shift[j];
-                R djx = sh.x * inv_dt, djy = sh.y * inv_dt, djz = sh.z * inv_dt;
-                if (mir & 1) djy = -djy;
-                if (mir & 2) djz = -djz;
-                dvx += duix - djx; dvy += duiy - djy; dvz += duiz - djz;
-                R fl = (rho_j * djx - rho_i * duix) * gx + (rho_j * djy - rho_i * duiy) * gy +
-                       (rho_j * djz - rho_i * duiz) * gz;
-                rate += wj * fl * inv_rho_i;
-            }
-            rate += wj * (dvx * gx + dvy * gy + dvz * gz);
-            if (F.delta > 0.0) {
-                // psi_ij . grad W with psi = 2 (rho_j - rho_i)(-r)/(r^2 + eta^2) and grad W = c r
-                R pg = (R)2 * (rho_j - rho_i) * (-c * r2) * frcp<R>(r2 + eta2);
-                R dr = dhc * wj * pg * inv_rho_i;
-                rate += dr;
-                drate += dr;
-            }
-        };
-        Traverse tr{g, s.start, s.cellid, b.x, st.scal[SPH_S_OUT_OF_BOX] > 0.5};
-        tr(i, xi.x, xi.y, xi.z, task);
+        e00 = e11 = e22 = e01 = e02 = e12 = 0;
+        rate = drate = 0;
+    }
 
+    __device__ void pair(int j, int mir, double rxd, double ryd, double rzd) {
+        const DFluid& F = a.F;
+        const DBuf<R>& b = a.st.b;
+        R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
+        R r2 = rx * rx + ry * ry + rz * rz;
+        R rn = fsqrt<R>(r2);
+        R c = gradw_c<R>(F, rn);
+        R wj = wq<R>(F, b, b.V, j);
+        // E_i += w_j (x_j - x_i) (x) grad W = -w_j c r (x) r
+        R sc = -wj * c;
+        e00 += sc * rx * rx; e11 += sc * ry * ry; e01 += sc * rx * ry;
+        if (!(F.flags & SPH_TWO_D)) { e22 += sc * rz * rz; e02 += sc * rx * rz; e12 += sc * ry * rz; }
+        R gx = c * rx, gy = c * ry, gz = c * rz;
+        R rho_j = a.st.rho[j];
+        double4 vj4 = b.v[j];
+        if (mir & 1) vj4.y = -vj4.y;
+        if (mir & 2) vj4.z = -vj4.z;
+        // v_i - v_j in f64 before narrowing (the difference is what the sum needs)
+        R dvx = (R)(vi4.x - vj4.x), dvy = (R)(vi4.y - vj4.y), dvz = (R)(vi4.z - vj4.z);
+        if (F.pst_en) {
+            typename R4t<R>::type sh = b.shift[j];
+            R djx = sh.x * inv_dt, djy = sh.y * inv_dt, djz = sh.z * inv_dt;
+            if (mir & 1) djy = -djy;
+            if (mir & 2) djz = -djz;
+            dvx += duix - djx; dvy += duiy - djy; dvz += duiz - djz;
+            R fl = (rho_j * djx - rho_i * duix) * gx + (rho_j * djy - rho_i * duiy) * gy +
+                   (rho_j * djz - rho_i * duiz) * gz;
+            rate += wj * fl * inv_rho_i;
+        }
+        rate += wj * (dvx * gx + dvy * gy + dvz * gz);
+        if (F.delta > 0.0) {
+            // psi_ij . grad W with psi = 2 (rho_j - rho_i)(-r)/(r^2 + eta^2), grad W = c r
+            R pg = (R)2 * (rho_j - rho_i) * (-c * r2) * frcp<R>(r2 + (R)F.eta2);
+            R dr = (R)(F.delta * F.h * F.c0) * wj * pg * inv_rho_i;
+            rate += dr;
+            drate += dr;
+        }
+    }
+
+    __device__ void end(int i) {
+        const DFluid& F = a.F;
+        const DBuf<R>& b = a.st.b;
+        const DState<R>& st = a.st;
         // lambda and L (DSPH.py:639-689)
         double L6[6], lam;
-        if (two_d) {
-            double a = e00, bb = e01, d = e11;
-            double det = sqrt((a - d) * (a - d) + 4.0 * bb * bb);
-            double ev2 = 0.5 * (a + d - det);
-            lam = fmin(1.0, fmax(0.0, ev2));
+        if (F.flags & SPH_TWO_D) {
+            double aa = e00, bb = e01, d = e11;
+            double det = sqrt((aa - d) * (aa - d) + 4.0 * bb * bb);
+            lam = fmin(1.0, fmax(0.0, 0.5 * (aa + d - det)));
             spd_inverse2(e00, e01, e01, e11, F.tol_lam, L6);
         } else {
             double A6[6] = {(double)e00, (double)e11, (double)e22, (double)e01, (double)e02, (double)e12};
@@ -388,14 +627,13 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_pass_volume(DGrid g, DFluid F, DS
         R* Lo = st.L6 + 8 * (size_t)i;
 #pragma unroll
         for (int k = 0; k < 6; ++k) Lo[k] = (R)L6[k];
-
         // -div v, with the hoop rate in axisymmetry (DSPH.py:1454-1476)
         R dv = rate;
-        if (F.flags & SPH_AXISYMMETRIC) dv -= (R)(vi4.y / fmax(xi.y, F.axi_rmin));
+        if (F.flags & SPH_AXISYMMETRIC) dv -= (R)(vi4.y / fmax(xiy, F.axi_rmin));
         st.div[i] = dv;
-        wdiff = -(double)b.V[i] * (double)b.p[i] * (double)drate;
-
+        wdiff_acc += -(double)b.V[i] * (double)b.p[i] * (double)drate;
         // volume update (DSPH.py:1479-1494)
+        const double dt = st.scal[SPH_S_DT];
         R Vi = b.V[i];
         R dVdt = -Vi * dv;
         R Vn;
@@ -416,80 +654,88 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_pass_volume(DGrid g, DFluid F, DS
         // particle in this pass, only by its own diffusion work above
         b.p[i] = (R)tait_d(F, (double)b.m[i] / Vfull);
     }
-    double bs = block_sum(wdiff, red);
-    if (threadIdx.x == 0) s.part[PART_WDIFF * gridDim.x + blockIdx.x] = bs;
-}
+
+    __device__ void reduce(double* red, int slot) {
+        double bs = block_sum(wdiff_acc, red);
+        if (threadIdx.x == 0) a.s.part[PART_WDIFF * a.pstride + slot] = bs;
+    }
+};
 
 // ------------------------------------------------------------------------------------
 // pass B: <grad lambda> (compute_lam_grad_task) + compute_pressure_accel_task +
 //         compute_dt_task per pair; acceleration and the dt maxima per particle
 // ------------------------------------------------------------------------------------
-template <typename R>
-__global__ __launch_bounds__(SPH_BLOCK) void k_pass_force(DGrid g, DFluid F, DState<R> st, DScratch s) {
-    __shared__ double red[SPH_BLOCK / SPH_WAVE];
-    int i = xcd_block() * SPH_BLOCK + threadIdx.x;
-    double c2loc = F.csig2, amag = 0.0;
-    if (i < st.n) {
-        const DBuf<R>& b = st.b;
-        const R* Vn = st.V_new;
-        const double dt = st.scal[SPH_S_DT];
-        const R inv_dt = (R)(1.0 / dt);
-        double4 xi = b.x[i];
-        double4 vi4 = b.v[i];
-        R vix = (R)vi4.x, viy = (R)vi4.y, viz = (R)vi4.z;
-        const R rho_i = st.rho_new[i];
-        R p_i = b.p[i];
-        R lam_i = st.lam[i];
-        R glx = 0, gly = 0, glz = 0;    // Sum w_old (lam_j - lam_i) grad W
-        R apx = 0, apy = 0, apz = 0;    // Sum w_j (p_i + p_j) grad W
-        R avx = 0, avy = 0, avz = 0;    // Sum w_j (v_ij . r)/(r^2 + eta^2) grad W
-        R aax = 0, aay = 0, aaz = 0;    // Sum w_j rho_j (du_j . grad W)(v_j - v_i)
-        R c2max = (R)F.csig2;
-        const R eta2 = (R)F.eta2;
-        const R drho_eps = (R)(1e-3 * F.rho0);
-        const bool ale = F.pst_en && F.ale_mom;
-        auto task = [&](int j, int mir, double rxd, double ryd, double rzd) {
-            R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
-            R r2 = rx * rx + ry * ry + rz * rz;
-            R rn = fsqrt<R>(r2);
-            R c = gradw_c<R>(F, rn);
-            R gx = c * rx, gy = c * ry, gz = c * rz;
-            R wo = wq<R>(F, b, b.V, j);
-            R wj = wq<R>(F, b, Vn, j);
-            R dl = st.lam[j] - lam_i;
-            glx += wo * dl * gx; gly += wo * dl * gy; glz += wo * dl * gz;
-            R p_j = b.p[j];
-            R pp = wj * (p_i + p_j);
-            apx += pp * gx; apy += pp * gy; apz += pp * gz;
-            double4 vj4 = b.v[j];
-            if (mir & 1) vj4.y = -vj4.y;
-            if (mir & 2) vj4.z = -vj4.z;
-            R dvx = (R)(vi4.x - vj4.x), dvy = (R)(vi4.y - vj4.y), dvz = (R)(vi4.z - vj4.z);
-            if (F.visc > 0.0) {
-                R vr = dvx * rx + dvy * ry + dvz * rz;
-                R fv = wj * vr * frcp<R>(r2 + eta2);
-                avx += fv * gx; avy += fv * gy; avz += fv * gz;
-            }
-            R rho_j = st.rho_new[j];
-            if (ale) {
-                typename R4t<R>::type sh = b.shift[j];
-                R djx = sh.x * inv_dt, djy = sh.y * inv_dt, djz = sh.z * inv_dt;
-                if (mir & 1) djy = -djy;
-                if (mir & 2) djz = -djz;
-                R t = wj * rho_j * (djx * gx + djy * gy + djz * gz);
-                aax -= t * dvx; aay -= t * dvy; aaz -= t * dvz;
-            }
-            if (F.dt_pair) {
-                R drho = fabs(rho_i - rho_j);
-                if (drho > drho_eps) {
-                    R dp = fabs(p_i - p_j);
-                    if (dp > c2max * drho) c2max = dp / drho;   // divides only when the max grows
-                }
-            }
-        };
-        Traverse tr{g, s.start, s.cellid, b.x, st.scal[SPH_S_OUT_OF_BOX] > 0.5};
-        tr(i, xi.x, xi.y, xi.z, task);
+template <typename R> struct ForceBody {
+    typedef PassArgs<R> Args;
+    const Args& a;
+    double c2loc, amag;
+    double4 vi4;
+    double4 xi4;
+    R rho_i, p_i, lam_i, inv_dt, c2max;
+    R glx, gly, glz, apx, apy, apz, avx, avy, avz, aax, aay, aaz;
 
+    __device__ ForceBody(const Args& a_) : a(a_), c2loc(a_.F.csig2), amag(0.0) {}
+
+    __device__ void begin(int i, double& x, double& y, double& z) {
+        const DBuf<R>& b = a.st.b;
+        xi4 = b.x[i];
+        x = xi4.x; y = xi4.y; z = xi4.z;
+        vi4 = b.v[i];
+        rho_i = a.st.rho_new[i];
+        p_i = b.p[i];
+        lam_i = a.st.lam[i];
+        inv_dt = (R)(1.0 / a.st.scal[SPH_S_DT]);
+        c2max = (R)a.F.csig2;
+        glx = gly = glz = apx = apy = apz = avx = avy = avz = aax = aay = aaz = 0;
+    }
+
+    __device__ void pair(int j, int mir, double rxd, double ryd, double rzd) {
+        const DFluid& F = a.F;
+        const DBuf<R>& b = a.st.b;
+        const DState<R>& st = a.st;
+        R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
+        R r2 = rx * rx + ry * ry + rz * rz;
+        R rn = fsqrt<R>(r2);
+        R c = gradw_c<R>(F, rn);
+        R gx = c * rx, gy = c * ry, gz = c * rz;
+        R wo = wq<R>(F, b, b.V, j);
+        R wj = wq<R>(F, b, st.V_new, j);
+        R dl = st.lam[j] - lam_i;
+        glx += wo * dl * gx; gly += wo * dl * gy; glz += wo * dl * gz;
+        R p_j = b.p[j];
+        R pp = wj * (p_i + p_j);
+        apx += pp * gx; apy += pp * gy; apz += pp * gz;
+        double4 vj4 = b.v[j];
+        if (mir & 1) vj4.y = -vj4.y;
+        if (mir & 2) vj4.z = -vj4.z;
+        R dvx = (R)(vi4.x - vj4.x), dvy = (R)(vi4.y - vj4.y), dvz = (R)(vi4.z - vj4.z);
+        if (F.visc > 0.0) {
+            R vr = dvx * rx + dvy * ry + dvz * rz;
+            R fv = wj * vr * frcp<R>(r2 + (R)F.eta2);
+            avx += fv * gx; avy += fv * gy; avz += fv * gz;
+        }
+        R rho_j = st.rho_new[j];
+        if (F.pst_en && F.ale_mom) {
+            typename R4t<R>::type sh = b.shift[j];
+            R djx = sh.x * inv_dt, djy = sh.y * inv_dt, djz = sh.z * inv_dt;
+            if (mir & 1) djy = -djy;
+            if (mir & 2) djz = -djz;
+            R t = wj * rho_j * (djx * gx + djy * gy + djz * gz);
+            aax -= t * dvx; aay -= t * dvy; aaz -= t * dvz;
+        }
+        if (F.dt_pair) {
+            R drho = fabs(rho_i - rho_j);
+            if (drho > (R)(1e-3 * F.rho0)) {
+                R dp = fabs(p_i - p_j);
+                if (dp > c2max * drho) c2max = dp / drho;   // divides only when the max grows
+            }
+        }
+    }
+
+    __device__ void end(int i) {
+        const DFluid& F = a.F;
+        const DBuf<R>& b = a.st.b;
+        const DState<R>& st = a.st;
         // <grad lambda> = L_i Sum(...) (DSPH.py:690-702)
         const R* Li = st.L6 + 8 * (size_t)i;
         R lgx = Li[0] * glx + Li[3] * gly + Li[4] * glz;
@@ -498,34 +744,38 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_pass_force(DGrid g, DFluid F, DSt
         if (F.flags & SPH_TWO_D) lgz = 0;
         typename R4t<R>::type lg; lg.x = lgx; lg.y = lgy; lg.z = lgz; lg.w = 0;
         st.lam_grad[i] = lg;
-
         // acceleration: g + pressure + Morris (d = 2(3 + 2)) + ALE
-        R wi = wq<R>(F, b, Vn, i), mwi = mwq<R>(F, b, i);
+        R wi = wq<R>(F, b, st.V_new, i), mwi = mwq<R>(F, b, i);
         R kp = -wi / mwi;
         R kv = (R)(10.0 * F.visc);
         R ax = (R)F.g[0] + kp * apx + kv * avx;
         R ay = (R)F.g[1] + kp * apy + kv * avy;
         R az = (R)F.g[2] + kp * apz + kv * avz;
-        if (ale) { R ir = (R)1 / rho_i; ax += aax * ir; ay += aay * ir; az += aaz * ir; }
+        if (F.pst_en && F.ale_mom) { R ir = (R)1 / rho_i; ax += aax * ir; ay += aay * ir; az += aaz * ir; }
         typename R4t<R>::type a4; a4.x = ax; a4.y = ay; a4.z = az; a4.w = 0;
         st.acc[i] = a4;
-        if (F.track_visc) st.de_visc[i] = vix * kv * avx + viy * kv * avy + viz * kv * avz;
-        c2loc = (double)c2max;
-        if (finite3(xi.x, xi.y, xi.z)) amag = sqrt((double)ax * ax + (double)ay * ay + (double)az * az);
+        if (F.track_visc)
+            st.de_visc[i] = (R)vi4.x * kv * avx + (R)vi4.y * kv * avy + (R)vi4.z * kv * avz;
+        c2loc = fmax(c2loc, (double)c2max);
+        if (finite3(xi4.x, xi4.y, xi4.z))
+            amag = fmax(amag, sqrt((double)ax * ax + (double)ay * ay + (double)az * az));
     }
-    double bc = block_max(c2loc, red);
-    double ba = block_max(amag, red);
-    if (threadIdx.x == 0) {
-        atomic_max_pos(&s.maxv[MAX_C2], bc);
-        atomic_max_pos(&s.maxv[MAX_ACC], ba);
+
+    __device__ void reduce(double* red, int) {
+        double bc = block_max(c2loc, red);
+        double ba = block_max(amag, red);
+        if (threadIdx.x == 0) {
+            atomic_max_pos(&a.s.maxv[MAX_C2], bc);
+            atomic_max_pos(&a.s.maxv[MAX_ACC], ba);
+        }
     }
-}
+};
 
 // adaptive dt (DSPH.py:1418-1430) and banking of the diffusion work (1495)
-__global__ __launch_bounds__(1024) void k_dt(DFluid F, double* scal, DScratch s, int nblk) {
+__global__ __launch_bounds__(1024) void k_dt(DFluid F, double* scal, DScratch s, int nwdiff, int pstride) {
     __shared__ double red[1024 / SPH_WAVE];
     double acc = 0.0;
-    for (int k = threadIdx.x; k < nblk; k += blockDim.x) acc += s.part[PART_WDIFF * nblk + k];
+    for (int k = threadIdx.x; k < nwdiff; k += blockDim.x) acc += s.part[PART_WDIFF * pstride + k];
     double w = block_sum(acc, red);
     if (threadIdx.x == 0) {
         double dt_prev = scal[SPH_S_DT];
@@ -549,7 +799,7 @@ __device__ __forceinline__ double wrap_pos(const DGrid& g, int k, double x) {
 
 // accumulate_step_work (DSPH.py:1714-1763) + advect (1652-1665) + max |v| (712-723)
 template <typename R>
-__global__ __launch_bounds__(SPH_BLOCK) void k_advect(DGrid g, DFluid F, DState<R> st, DScratch s) {
+__global__ __launch_bounds__(SPH_BLOCK) void k_advect(DGrid g, DFluid F, DState<R> st, DScratch s, int pstride) {
     __shared__ double red[SPH_BLOCK / SPH_WAVE];
     int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
     double wl = 0.0, wv = 0.0, vmax = 0.0;
@@ -583,8 +833,8 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_advect(DGrid g, DFluid F, DState<
     double s1 = block_sum(wl, red);
     double s2 = block_sum(wv, red);
     if (threadIdx.x == 0) {
-        s.part[PART_LEFT * gridDim.x + blockIdx.x] = s1;
-        s.part[PART_VISC * gridDim.x + blockIdx.x] = s2;
+        s.part[PART_LEFT * pstride + blockIdx.x] = s1;
+        s.part[PART_VISC * pstride + blockIdx.x] = s2;
     }
     if (F.ma_mode == 1) {
         double bm = block_max(vmax, red);
@@ -595,27 +845,34 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_advect(DGrid g, DFluid F, DState<
 // ------------------------------------------------------------------------------------
 // pass C: apply_pst first pass (DSPH.py:975-1043) over the advected positions
 // ------------------------------------------------------------------------------------
-template <typename R>
-__global__ __launch_bounds__(SPH_BLOCK) void k_pass_shift(DGrid g, DFluid F, DState<R> st, DScratch s) {
-    int i = xcd_block() * SPH_BLOCK + threadIdx.x;
-    if (i >= st.n) return;
-    const DBuf<R>& b = st.b;
-    double4 xi = b.x[i];
-    R rho_i = st.rho_new[i];
-    R sx = 0, sy = 0, sz = 0;
-    const R inv_wdx0 = F.Wdx0 > 1e-10 ? (R)(1.0 / F.Wdx0) : (R)0;
-    const R alpha = (R)F.alpha;
-    auto task = [&](int j, int mir, double rxd, double ryd, double rzd) {
+template <typename R> struct ShiftBody {
+    typedef PassArgs<R> Args;
+    const Args& a;
+    R rho_i, sx, sy, sz;
+
+    __device__ ShiftBody(const Args& a_) : a(a_) {}
+
+    __device__ void begin(int i, double& x, double& y, double& z) {
+        double4 xi = a.st.b.x[i];
+        x = xi.x; y = xi.y; z = xi.z;
+        rho_i = a.st.rho_new[i];
+        sx = sy = sz = 0;
+    }
+
+    __device__ void pair(int j, int mir, double rxd, double ryd, double rzd) {
+        const DFluid& F = a.F;
+        const DBuf<R>& b = a.st.b;
+        (void)mir;
         R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
         R r2 = rx * rx + ry * ry + rz * rz;
         R rn = fsqrt<R>(r2);
         R c = gradw_c<R>(F, rn);
         R q = rn * (R)F.inv_h;
         R W = 0;
-        if (q < (R)1) { R om = (R)1 - q; W = alpha * om * om * om * om * ((R)4 * q + (R)1); }
+        if (q < (R)1) { R om = (R)1 - q; W = (R)F.alpha * om * om * om * om * ((R)4 * q + (R)1); }
         R f = 0;
         if (F.Wdx0 > 1e-10) {
-            R t = W * inv_wdx0, tp = 1;
+            R t = W * (R)(1.0 / F.Wdx0), tp = 1;
             for (int k = 0; k < F.pst_n; ++k) tp *= t;
             f = (R)F.pst_R * tp;
         }
@@ -626,71 +883,121 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_pass_shift(DGrid g, DFluid F, DSt
         } else {
             R mj = mwq<R>(F, b, j);
             if (mj > (R)1e-10) {
-                R rho_j = st.rho_new[j];
-                R ra = (R)0.5 * (rho_i + rho_j);
+                R ra = (R)0.5 * (rho_i + a.st.rho_new[j]);
                 if (ra > (R)1e-10) wgt = mj / ra;
             }
         }
         R k = ((R)1 + f) * c * wgt;
         sx += k * rx; sy += k * ry; sz += k * rz;
-    };
-    Traverse tr{g, s.start, s.cellid, b.x, st.scal[SPH_S_OUT_OF_BOX] > 0.5};
-    tr(i, xi.x, xi.y, xi.z, task);
+    }
 
-    double4 v = b.v[i];
-    double vn = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
-    double Ma;
-    if (F.ma_mode == 2) Ma = F.pst_ma_fixed;
-    else if (F.ma_mode == 1) Ma = st.scal[SPH_S_MAX_V] / F.c0;
-    else Ma = vn / F.c0;
-    double drx, dry, drz;
-    if (F.colle) {
-        double kk = -2.0 * F.h * F.c0 * Ma;
-        double dvx = kk * sx, dvy = kk * sy, dvz = kk * sz;
-        if (F.clamp_vel) {
-            double cap = F.m_delta * vn;
-            double dn = sqrt(dvx * dvx + dvy * dvy + dvz * dvz);
-            if (dn > cap) { double r = cap / fmax(dn, 1e-30); dvx *= r; dvy *= r; dvz *= r; }
-        }
-        double dt = st.scal[SPH_S_DT];
-        drx = dvx * dt; dry = dvy * dt; drz = dvz * dt;
-    } else {
-        double kk = -F.cfl * Ma * F.h * F.h;
-        drx = kk * sx; dry = kk * sy; drz = kk * sz;
-    }
-    if (F.clamp_disp) {
-        double mx = F.pst_max_shift * F.dx0;
-        double dn = sqrt(drx * drx + dry * dry + drz * drz);
-        if (dn > mx) { double r = mx / dn; drx *= r; dry *= r; drz *= r; }
-    }
-    // pst_constrain (DSPH.py:888-941)
-    double lam = st.lam[i];
-    if (F.surf_none) {
-        if (lam < F.lam_hi) drx = dry = drz = 0.0;
-    } else if (lam < F.lam_hi) {
-        if (lam < F.lam_lo) {
-            drx = dry = drz = 0.0;
+    __device__ void end(int i) {
+        const DFluid& F = a.F;
+        const DBuf<R>& b = a.st.b;
+        const DState<R>& st = a.st;
+        double4 v = b.v[i];
+        double vn = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+        double Ma;
+        if (F.ma_mode == 2) Ma = F.pst_ma_fixed;
+        else if (F.ma_mode == 1) Ma = st.scal[SPH_S_MAX_V] / F.c0;
+        else Ma = vn / F.c0;
+        double drx, dry, drz;
+        if (F.colle) {
+            double kk = -2.0 * F.h * F.c0 * Ma;
+            double dvx = kk * sx, dvy = kk * sy, dvz = kk * sz;
+            if (F.clamp_vel) {
+                double cap = F.m_delta * vn;
+                double dn = sqrt(dvx * dvx + dvy * dvy + dvz * dvz);
+                if (dn > cap) { double r = cap / fmax(dn, 1e-30); dvx *= r; dvy *= r; dvz *= r; }
+            }
+            double dt = st.scal[SPH_S_DT];
+            drx = dvx * dt; dry = dvy * dt; drz = dvz * dt;
         } else {
-            typename R4t<R>::type ng = st.lam_grad[i];
-            double nx = ng.x, ny = ng.y, nz = (F.flags & SPH_TWO_D) ? 0.0 : (double)ng.z;
-            double nn = sqrt(nx * nx + ny * ny + nz * nz);
-            if (nn > 1e-10) {
-                nx /= nn; ny /= nn; nz /= nn;
-                double d = drx * nx + dry * ny + drz * nz;
-                drx -= d * nx; dry -= d * ny; drz -= d * nz;
+            double kk = -F.cfl * Ma * F.h * F.h;
+            drx = kk * sx; dry = kk * sy; drz = kk * sz;
+        }
+        if (F.clamp_disp) {
+            double mx = F.pst_max_shift * F.dx0;
+            double dn = sqrt(drx * drx + dry * dry + drz * drz);
+            if (dn > mx) { double r = mx / dn; drx *= r; dry *= r; drz *= r; }
+        }
+        // pst_constrain (DSPH.py:888-941)
+        double lam = st.lam[i];
+        if (F.surf_none) {
+            if (lam < F.lam_hi) drx = dry = drz = 0.0;
+        } else if (lam < F.lam_hi) {
+            if (lam < F.lam_lo) {
+                drx = dry = drz = 0.0;
+            } else {
+                typename R4t<R>::type ng = st.lam_grad[i];
+                double nx = ng.x, ny = ng.y, nz = (F.flags & SPH_TWO_D) ? 0.0 : (double)ng.z;
+                double nn = sqrt(nx * nx + ny * ny + nz * nz);
+                if (nn > 1e-10) {
+                    nx /= nn; ny /= nn; nz /= nn;
+                    double d = drx * nx + dry * ny + drz * nz;
+                    drx -= d * nx; dry -= d * ny; drz -= d * nz;
+                }
             }
         }
+        if (F.flags & SPH_TWO_D) drz = 0.0;
+        typename R4t<R>::type o; o.x = (R)drx; o.y = (R)dry; o.z = (R)drz; o.w = 0;
+        b.shift[i] = o;
     }
-    if (F.flags & SPH_TWO_D) drz = 0.0;
-    typename R4t<R>::type o; o.x = (R)drx; o.y = (R)dry; o.z = (R)drz; o.w = 0;
-    b.shift[i] = o;
-}
+
+    __device__ void reduce(double*, int) {}
+};
+
+// ------------------------------------------------------------------------------------
+// neighbour pairs (for the bit-exact test) through the same tiles as the passes
+// ------------------------------------------------------------------------------------
+struct PairArgs {
+    DScratch s;
+    const double4* X;
+    const int32_t* pid;
+    const double* scal;
+    int32_t* out;
+    long long cap;
+    unsigned long long* cnt;
+    __device__ bool any_out() const { return scal[SPH_S_OUT_OF_BOX] > 0.5; }
+};
+
+struct PairsBody {
+    typedef PairArgs Args;
+    const Args& a;
+    int pi;
+    double mine;
+    __device__ PairsBody(const Args& a_) : a(a_), pi(0), mine(0.0) {}
+    __device__ void begin(int i, double& x, double& y, double& z) {
+        double4 xi = a.X[i];
+        x = xi.x; y = xi.y; z = xi.z;
+        pi = a.pid[i];
+    }
+    __device__ void pair(int j, int mir, double, double, double) {
+        if (a.out) {
+            unsigned long long k = atomicAdd(a.cnt, 1ull);
+            if ((long long)k < a.cap) {
+                a.out[3 * k + 0] = pi;
+                a.out[3 * k + 1] = a.pid[j];
+                a.out[3 * k + 2] = mir;
+            }
+        } else {
+            mine += 1.0;
+        }
+    }
+    __device__ void end(int) {}
+    __device__ void reduce(double* red, int) {
+        if (!a.out) {
+            double b = block_sum(mine, red);
+            if (threadIdx.x == 0 && b > 0.0) atomicAdd(a.cnt, (unsigned long long)b);
+        }
+    }
+};
 
 // ------------------------------------------------------------------------------------
 // apply_pst second pass (DSPH.py:1045-1049) and the boundary kernels (sph_base.py:622-655)
 // ------------------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(SPH_BLOCK) void k_finish(DGrid g, DFluid F, DState<R> st, DScratch s) {
+__global__ __launch_bounds__(SPH_BLOCK) void k_finish(DGrid g, DFluid F, DState<R> st, DScratch s, int pstride) {
     __shared__ double red[SPH_BLOCK / SPH_WAVE];
     int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
     double ww = 0.0;
@@ -761,16 +1068,16 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_finish(DGrid g, DFluid F, DState<
         }
     }
     double bs = block_sum(ww, red);
-    if (threadIdx.x == 0) s.part[PART_WALL * gridDim.x + blockIdx.x] = bs;
+    if (threadIdx.x == 0) s.part[PART_WALL * pstride + blockIdx.x] = bs;
 }
 
-__global__ __launch_bounds__(1024) void k_bank(double* scal, DScratch s, int nblk, int ma_global) {
+__global__ __launch_bounds__(1024) void k_bank(double* scal, DScratch s, int nblk, int pstride, int ma_global) {
     __shared__ double red[1024 / SPH_WAVE];
     double a = 0.0, b = 0.0, c = 0.0;
     for (int k = threadIdx.x; k < nblk; k += blockDim.x) {
-        a += s.part[PART_LEFT * nblk + k];
-        b += s.part[PART_VISC * nblk + k];
-        c += s.part[PART_WALL * nblk + k];
+        a += s.part[PART_LEFT * pstride + k];
+        b += s.part[PART_VISC * pstride + k];
+        c += s.part[PART_WALL * pstride + k];
     }
     double sa = block_sum(a, red), sb = block_sum(b, red), sc = block_sum(c, red);
     if (threadIdx.x == 0) {
@@ -788,33 +1095,11 @@ __global__ void k_publish_vmax(double* scal, DScratch s) {
 }
 
 // ------------------------------------------------------------------------------------
-// neighbour pairs for the bit-exact test
-// ------------------------------------------------------------------------------------
-__global__ __launch_bounds__(SPH_BLOCK) void k_pairs(DGrid g, int n, const double4* __restrict__ X,
-                                                     const int32_t* __restrict__ pid, DScratch s, const double* scal,
-                                                     int32_t* out, long long cap, unsigned long long* cnt) {
-    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
-    if (i >= n) return;
-    double4 xi = X[i];
-    int pi = pid[i];
-    auto task = [&](int j, int mir, double, double, double) {
-        unsigned long long k = atomicAdd(cnt, 1ull);
-        if ((long long)k < cap) {
-            out[3 * k + 0] = pi;
-            out[3 * k + 1] = pid[j];
-            out[3 * k + 2] = mir;
-        }
-    };
-    Traverse tr{g, s.start, s.cellid, X, scal[SPH_S_OUT_OF_BOX] > 0.5};
-    tr(i, xi.x, xi.y, xi.z, task);
-}
-
-// ------------------------------------------------------------------------------------
 // f64 diagnostics
 // ------------------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(SPH_BLOCK) void k_reduce(int n, DBuf<R> b, int which, double g0, double g1, double g2,
-                                                      DFluid F, DScratch s) {
+                                                      DFluid F, DScratch s, int pstride) {
     __shared__ double red[SPH_BLOCK / SPH_WAVE];
     int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
     double r0 = 0, r1 = 0, r2 = 0;
@@ -842,19 +1127,19 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_reduce(int n, DBuf<R> b, int whic
     }
     double a = block_sum(r0, red), bb = block_sum(r1, red), c = block_sum(r2, red);
     if (threadIdx.x == 0) {
-        s.part[PART_R0 * gridDim.x + blockIdx.x] = a;
-        s.part[PART_R1 * gridDim.x + blockIdx.x] = bb;
-        s.part[PART_R2 * gridDim.x + blockIdx.x] = c;
+        s.part[PART_R0 * pstride + blockIdx.x] = a;
+        s.part[PART_R1 * pstride + blockIdx.x] = bb;
+        s.part[PART_R2 * pstride + blockIdx.x] = c;
     }
 }
 
-__global__ __launch_bounds__(1024) void k_reduce_final(DScratch s, int nblk, double* out) {
+__global__ __launch_bounds__(1024) void k_reduce_final(DScratch s, int nblk, int pstride, double* out) {
     __shared__ double red[1024 / SPH_WAVE];
     double a = 0, b = 0, c = 0;
     for (int k = threadIdx.x; k < nblk; k += blockDim.x) {
-        a += s.part[PART_R0 * nblk + k];
-        b += s.part[PART_R1 * nblk + k];
-        c += s.part[PART_R2 * nblk + k];
+        a += s.part[PART_R0 * pstride + k];
+        b += s.part[PART_R1 * pstride + k];
+        c += s.part[PART_R2 * pstride + k];
     }
     double sa = block_sum(a, red), sb = block_sum(b, red), sc = block_sum(c, red);
     if (threadIdx.x == 0) { out[0] = sa; out[1] = sb; out[2] = sc; }
@@ -875,6 +1160,10 @@ struct sph_ctx {
     int64_t nbuck;        // dense + hashed + off-grid
     int nblk;             // particle blocks
     int scan_blocks;
+    TileGeo tiles;        // cell tiles of the pair passes
+    int pstride;          // stride of a partial-sum slot: tiles + particle blocks
+    int* slow_idx;        // [n] particles the tiles could not serve
+    int* slow_cnt;        // [1]
     std::string err;
     bool timing = false;
     TimerKind tk[4];
@@ -967,6 +1256,21 @@ extern "C" int sph_ctx_create(sph_ctx** out, const sph_dims* d) {
     c->nbuck = nbox + T + 1;
     c->nblk = (int)((d->n + SPH_BLOCK - 1) / SPH_BLOCK);
     c->scan_blocks = (int)((c->nbuck + 1 + SCAN_TILE - 1) / SCAN_TILE);
+    {
+        // a tile holds about 200 particles: B cells of the lattice's particles per cell
+        const bool two_d = (d->flags & SPH_TWO_D) != 0;
+        double dx = d->dx0 > 0.0 ? d->dx0 : d->h / 2.6;
+        double ppc = two_d ? (d->cell[0] * d->cell[1]) / (dx * dx)
+                           : (d->cell[0] * d->cell[1] * d->cell[2]) / (dx * dx * dx);
+        int B = (int)llround(200.0 / fmax(ppc, 1e-3));
+        B = std::max(1, std::min(B, std::min((int)d->box_n[0], TILE_MAXB)));
+        c->tiles.B = B;
+        c->tiles.ntx = (d->box_n[0] + B - 1) / B;
+        int64_t nt = (int64_t)c->tiles.ntx * d->box_n[1] * d->box_n[2];
+        if (nt > (int64_t)1 << 30) { delete c; return SPH_EINVAL; }
+        c->tiles.ntiles = (int)nt;
+    }
+    c->pstride = c->tiles.ntiles + c->nblk;
 
     size_t n = (size_t)d->n;
     size_t nb1 = (size_t)c->nbuck + 1;
@@ -982,12 +1286,15 @@ extern "C" int sph_ctx_create(sph_ctx** out, const sph_dims* d) {
     ALLOC(s.bsum, (size_t)c->scan_blocks * sizeof(int));
     ALLOC(s.tmp, n * sizeof(int));
     ALLOC(s.perm, n * sizeof(int));
-    ALLOC(s.part, (size_t)SPH_NPART * c->nblk * sizeof(double));
+    ALLOC(s.part, (size_t)SPH_NPART * c->pstride * sizeof(double));
     ALLOC(s.maxv, 4 * sizeof(double));
+    ALLOC(c->slow_idx, n * sizeof(int));
+    ALLOC(c->slow_cnt, sizeof(int));
 #undef ALLOC
     if (e == hipSuccess) e = hipMemset(s.count, 0, nb1 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(s.maxv, 0, 4 * sizeof(double));
     if (e == hipSuccess) e = hipMemset(s.start, 0, nb1 * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(c->slow_cnt, 0, sizeof(int));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         sph_ctx_destroy(c);
@@ -1000,7 +1307,8 @@ extern "C" int sph_ctx_create(sph_ctx** out, const sph_dims* d) {
 extern "C" int sph_ctx_destroy(sph_ctx* c) {
     if (!c) return SPH_OK;
     DScratch& s = c->s;
-    void* ptrs[] = {s.key, s.cellw, s.cellid, s.count, s.start, s.cursor, s.bsum, s.tmp, s.perm, s.part, s.maxv};
+    void* ptrs[] = {s.key, s.cellw, s.cellid, s.count, s.start, s.cursor, s.bsum, s.tmp, s.perm, s.part, s.maxv,
+                    c->slow_idx, c->slow_cnt};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& t : c->tk) for (auto ev : t.ev) (void)hipEventDestroy(ev);
     delete c;
@@ -1153,25 +1461,43 @@ static DFluid make_fluid(const sph_ctx* c, const sph_fluid_params* P) {
     return F;
 }
 
+template <class Body>
+static void launch_pair_pass(sph_ctx* c, const typename Body::Args& a, hipStream_t hs) {
+    SlowList sl{c->slow_idx, c->slow_cnt};
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, hs, c->slow_cnt);
+    hipLaunchKernelGGL((k_tiled<Body>), dim3(c->tiles.ntiles), dim3(SPH_BLOCK), 0, hs, c->grid, c->tiles, a, sl);
+    hipLaunchKernelGGL((k_slow<Body>), dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, (int)c->dims.n,
+                       c->tiles.ntiles, a, sl);
+}
+
+template <typename R> static PassArgs<R> pass_args(sph_ctx* c, sph_state* st, const DFluid& F) {
+    PassArgs<R> a;
+    a.F = F;
+    a.st = dstate<R>(c, st);
+    a.s = c->s;
+    a.X = (const double4*)st->buf[st->cur].x;
+    a.pstride = c->pstride;
+    return a;
+}
+
 template <typename R> static int pass_volume(sph_ctx* c, sph_state* st, const DFluid& F, hipStream_t hs) {
     timer_begin(c, 0, hs);
-    hipLaunchKernelGGL((k_pass_volume<R>), dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st),
-                       c->s);
+    launch_pair_pass<VolumeBody<R>>(c, pass_args<R>(c, st, F), hs);
     timer_end(c, 0, hs);
     return check_launch(c, "pass_volume");
 }
 
 template <typename R> static int pass_force(sph_ctx* c, sph_state* st, const DFluid& F, hipStream_t hs) {
     timer_begin(c, 1, hs);
-    hipLaunchKernelGGL((k_pass_force<R>), dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st),
-                       c->s);
+    launch_pair_pass<ForceBody<R>>(c, pass_args<R>(c, st, F), hs);
     timer_end(c, 1, hs);
-    hipLaunchKernelGGL(k_dt, dim3(1), dim3(1024), 0, hs, F, st->scal, c->s, c->nblk);
+    hipLaunchKernelGGL(k_dt, dim3(1), dim3(1024), 0, hs, F, st->scal, c->s, c->tiles.ntiles + c->nblk, c->pstride);
     return check_launch(c, "pass_force");
 }
 
 template <typename R> static int advect(sph_ctx* c, sph_state* st, const DFluid& F, hipStream_t hs) {
-    hipLaunchKernelGGL((k_advect<R>), dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st), c->s);
+    hipLaunchKernelGGL((k_advect<R>), dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st), c->s,
+                       c->pstride);
     if (F.ma_mode == 1) hipLaunchKernelGGL(k_publish_vmax, dim3(1), dim3(64), 0, hs, st->scal, c->s);
     return check_launch(c, "advect");
 }
@@ -1179,15 +1505,16 @@ template <typename R> static int advect(sph_ctx* c, sph_state* st, const DFluid&
 template <typename R> static int pass_shift(sph_ctx* c, sph_state* st, const DFluid& F, hipStream_t hs) {
     if (!F.pst_en) return SPH_OK;
     timer_begin(c, 2, hs);
-    hipLaunchKernelGGL((k_pass_shift<R>), dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st),
-                       c->s);
+    launch_pair_pass<ShiftBody<R>>(c, pass_args<R>(c, st, F), hs);
     timer_end(c, 2, hs);
     return check_launch(c, "pass_shift");
 }
 
 template <typename R> static int finish(sph_ctx* c, sph_state* st, const DFluid& F, hipStream_t hs) {
-    hipLaunchKernelGGL((k_finish<R>), dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st), c->s);
-    hipLaunchKernelGGL(k_bank, dim3(1), dim3(1024), 0, hs, st->scal, c->s, c->nblk, F.ma_mode == 1 ? 1 : 0);
+    hipLaunchKernelGGL((k_finish<R>), dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st), c->s,
+                       c->pstride);
+    hipLaunchKernelGGL(k_bank, dim3(1), dim3(1024), 0, hs, st->scal, c->s, c->nblk, c->pstride,
+                       F.ma_mode == 1 ? 1 : 0);
     return check_launch(c, "finish");
 }
 
@@ -1242,19 +1569,25 @@ extern "C" int sph_neighbor_pairs(sph_ctx* c, const sph_state* st, int32_t* pair
                                   void* stream) {
     int rc = check_state(c, st, false);
     if (rc) return rc;
-    if (!pairs || !count || cap < 0) return set_err(c, SPH_EINVAL, "null output%s", "", 0);
+    if (!count || cap < 0 || (!pairs && cap > 0)) return set_err(c, SPH_EINVAL, "null output%s", "", 0);
     hipStream_t hs = (hipStream_t)stream;
     const sph_sorted& A = st->buf[st->cur];
     HIPCHECK(c, hipMemsetAsync(count, 0, sizeof(int64_t), hs));
-    hipLaunchKernelGGL(k_pairs, dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, (int)c->dims.n,
-                       (const double4*)A.x, (const int32_t*)A.pid, c->s, (const double*)st->scal, pairs, (long long)cap,
-                       (unsigned long long*)count);
+    PairArgs a;
+    a.s = c->s;
+    a.X = (const double4*)A.x;
+    a.pid = (const int32_t*)A.pid;
+    a.scal = st->scal;
+    a.out = pairs;
+    a.cap = (long long)cap;
+    a.cnt = (unsigned long long*)count;
+    launch_pair_pass<PairsBody>(c, a, hs);
     rc = check_launch(c, "neighbor_pairs");
     if (rc) return rc;
     int64_t h = 0;
     HIPCHECK(c, hipMemcpyAsync(&h, count, sizeof h, hipMemcpyDeviceToHost, hs));
     HIPCHECK(c, hipStreamSynchronize(hs));
-    if (h > cap) return set_err(c, SPH_ECAPACITY, "pair buffer too small%s: %lld pairs", "", (long long)h);
+    if (pairs && h > cap) return set_err(c, SPH_ECAPACITY, "pair buffer too small%s: %lld pairs", "", (long long)h);
     return SPH_OK;
 }
 
@@ -1273,10 +1606,10 @@ extern "C" int sph_reduce(sph_ctx* c, const sph_state* st, int32_t which, const 
     int n = (int)c->dims.n;
     if (c->dims.precision == SPH_PREC_MIXED)
         hipLaunchKernelGGL(k_reduce<float>, dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, n, dbuf<float>(st->buf[st->cur]),
-                           which, g0, g1, g2, F, c->s);
+                           which, g0, g1, g2, F, c->s, c->pstride);
     else
         hipLaunchKernelGGL(k_reduce<double>, dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, n,
-                           dbuf<double>(st->buf[st->cur]), which, g0, g1, g2, F, c->s);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, hs, c->s, c->nblk, out);
+                           dbuf<double>(st->buf[st->cur]), which, g0, g1, g2, F, c->s, c->pstride);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, hs, c->s, c->nblk, c->pstride, out);
     return check_launch(c, "reduce");
 }

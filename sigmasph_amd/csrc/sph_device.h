@@ -162,15 +162,15 @@ struct Traverse {
         if (my && yi < g.h) {
             for (int oz = -1; oz <= 1; ++oz)
                 for (int ox = -1; ox <= 1; ++ox)
-                    cell(cx + ox, 0, cz + oz, i, 1, xi, -yi, zi, f);
+                    cell(wrap_cell(g, 0, cx + ox), 0, wrap_cell(g, 2, cz + oz), i, 1, xi, -yi, zi, f);
         }
         if (mz && zi < g.h) {
             for (int oy = -1; oy <= 1; ++oy)
                 for (int ox = -1; ox <= 1; ++ox)
-                    cell(cx + ox, cy + oy, 0, i, 2, xi, yi, -zi, f);
+                    cell(wrap_cell(g, 0, cx + ox), wrap_cell(g, 1, cy + oy), 0, i, 2, xi, yi, -zi, f);
         }
         if (my && mz && yi < g.h && zi < g.h) {
-            for (int ox = -1; ox <= 1; ++ox) cell(cx + ox, 0, 0, i, 3, xi, -yi, -zi, f);
+            for (int ox = -1; ox <= 1; ++ox) cell(wrap_cell(g, 0, cx + ox), 0, 0, i, 3, xi, -yi, -zi, f);
         }
     }
 };

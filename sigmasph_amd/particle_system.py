@@ -266,6 +266,7 @@ class SceneLayout:
         for k in range(3):
             d.box_lo[k] = los[k]
             d.box_n[k] = nbs[k]
+        d.dx0 = self.particle_diameter
         d.hash_bits = int(min(24, max(12, math.ceil(math.log2(max(2 * n if nbs == [1, 1, 1] else n // 8, 4096))))))
         return d
 
@@ -324,6 +325,10 @@ class ParticleSystem(SceneLayout):
         b0["x0"][:, :3].copy_(torch.from_numpy(x))
         b0["v"][:, :3].copy_(torch.from_numpy(v))
         b0["V"].copy_(torch.from_numpy(V))
+        if R == torch.float32:
+            # the float volume's rounding residual starts in V_lo, so V + V_lo is the
+            # laid volume to f64 precision
+            b0["V_lo"].copy_(torch.from_numpy(V - V.astype(np.float32).astype(np.float64)))
         b0["m"].copy_(torch.from_numpy(m))
 
         self.state = N.SphState()
@@ -396,6 +401,13 @@ class ParticleSystem(SceneLayout):
         N.check(rc, self.ctx, "sph_neighbor_pairs")
         k = int(cnt.item())
         return out[:k].cpu().numpy()
+
+    def neighbor_count(self):
+        """Number of directed neighbour pairs of the current state (no list kept)."""
+        cnt = torch.zeros(1, dtype=torch.int64, device=self.device)
+        N.check(self._lib.sph_neighbor_pairs(self.ctx, ctypes.byref(self.state), None, 0, cnt.data_ptr(),
+                                             self.stream()), self.ctx, "sph_neighbor_pairs")
+        return int(cnt.item())
 
     def sorted_by_pid(self, field):
         """`field.to_numpy()` reordered by stable particle id."""

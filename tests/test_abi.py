@@ -41,7 +41,7 @@ def test_null_arguments_rejected():
 
 STRUCTS = {
     "sph_dims": (N.SphDims, ["n", "precision", "flags", "h", "cell", "origin", "period", "box_lo",
-                             "box_n", "hash_bits"]),
+                             "box_n", "hash_bits", "dx0"]),
     "sph_sorted": (N.SphSorted, ["pid", "obj", "x", "x0", "u", "v", "shift", "V", "V_lo", "m", "p"]),
     "sph_state": (N.SphState, ["buf", "cur", "acc", "lam", "L6", "lam_grad", "div", "de_visc", "V_new",
                                "rho", "rho_new", "scal"]),

@@ -22,8 +22,21 @@ FIELDS = (("x", "x"), ("v", "v"), ("u", "u"), ("V", "V"), ("pressure", "p"),
           ("acceleration", "acc"), ("lam", "lam"), ("divergence", "div"), ("pst_shift", "shift"))
 
 
-def run_pair(scene, steps=1, precision="fp64"):
+def perturb_volume(ps, rel=0.01, seed=3):
+    """Give the initial state a real pressure field: V -> V (1 + rel u), u ~ U(-1, 1),
+    on the host layout (which the oracle starts from) and on the device."""
+    rng = np.random.default_rng(seed)
+    ps.V_init *= 1.0 + rel * rng.uniform(-1.0, 1.0, size=len(ps.V_init))
+    b = ps._bufs[ps.state.cur]
+    b["V"].copy_(torch.from_numpy(ps.V_init))
+    if b["V"].dtype == torch.float32:
+        b["V_lo"].copy_(torch.from_numpy(ps.V_init - ps.V_init.astype(np.float32).astype(np.float64)))
+
+
+def run_pair(scene, steps=1, precision="fp64", perturb=False):
     ps = ParticleSystem(SimConfig(scene), precision=precision)
+    if perturb:
+        perturb_volume(ps)
     solver = ps.build_solver()
     fs = FluidSettings(ps)
     s = O.OracleState.from_snapshot(ps.snapshot())
@@ -46,21 +59,23 @@ def scaled_err(gpu, ora, floor=0.0):
     return np.abs(gpu - ora).max() / scale
 
 
-def floors(ps, s):
+def floors(ps, s, eps=0.0):
     """Physical scales below which a field is round-off around zero: -div v against the
-    transport speed over h, the pressure against 1e-6 of the EOS stiffness, the
-    acceleration against that pressure over rho0 h."""
+    transport speed over h; the pressure against 1e-6 of the EOS stiffness or, with
+    float storage (eps = 2^-24), the pressure quantum 64 gamma S eps of a float density;
+    the acceleration against that pressure over rho0 h."""
     c = ps.cfg.get_cfg
-    S = float(c("c0")) ** 2 * float(c("density0")) / float(c("exponent"))
+    gam = float(c("exponent"))
+    S = float(c("c0")) ** 2 * float(c("density0")) / gam
     h = ps.support_radius
     U = np.abs(s.v).max() + np.abs(s.shift).max() / s.scal[0]
-    return {"divergence": U / h, "pressure": 1e-6 * S,
-            "acceleration": 1e-6 * S / (float(c("density0")) * h)}
+    pf = max(1e-6 * S, 64.0 * gam * S * eps)
+    return {"divergence": U / h, "pressure": pf, "acceleration": pf / (float(c("density0")) * h)}
 
 
-def compare(ps, s, tol, fields=FIELDS):
+def compare(ps, s, tol, fields=FIELDS, eps=0.0):
     errs = {}
-    fl = floors(ps, s)
+    fl = floors(ps, s, eps)
     for fname, oname in fields:
         g = ps.sorted_by_pid(getattr(ps, fname))
         o = s.by_pid(oname)
@@ -119,11 +134,21 @@ def test_five_steps_fp64(gpu, name):
     ps.close()
 
 
-@pytest.mark.parametrize("name", ["tgv", "block", "plane_strain"])
+@pytest.mark.parametrize("name", ["tgv", "block", "plane_strain", "mirror_yz"])
 def test_one_step_mixed(gpu, name):
-    ps, solver, s = run_pair(SCENES[name](), steps=1, precision="mixed")
-    errs = compare(ps, s, 2e-4)
+    """Float pair sums against the f64 oracle on a state with a +-1% volume (pressure)
+    field: every field within 2e-4 of its scale, positions within 1e-9."""
+    ps, solver, s = run_pair(SCENES[name](), steps=1, precision="mixed", perturb=True)
+    errs = compare(ps, s, 2e-4, eps=2.0 ** -24)
     assert errs["x"] < 1e-9
+    assert solver.dt[None] == pytest.approx(s.scal[0], rel=1e-5)
+    ps.close()
+
+
+@pytest.mark.parametrize("name", ["tgv", "block"])
+def test_one_step_fp64_pressure_field(gpu, name):
+    ps, solver, s = run_pair(SCENES[name](), steps=1, perturb=True)
+    compare(ps, s, 1e-9)
     ps.close()
 
 

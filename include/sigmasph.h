@@ -61,6 +61,17 @@ extern "C" {
 #define SPH_S_OFF_GRID_NUM  8   /* particles quarantined (non-finite position)         */
 #define SPH_S_OFF_GRID_MASS 9
 #define SPH_S_OUT_OF_BOX   10   /* particles binned in the hashed overflow table        */
+#define SPH_S_SIM_TIME     11   /* solid: simulation time, advanced by every step       */
+#define SPH_S_LOAD_S       12   /* solid: load ramp factor s(t) of the grips            */
+#define SPH_S_LOAD_SDOT    13   /* solid: ds/dt                                         */
+#define SPH_S_PLASTIC_WORK 14   /* solid: Sum V sigma_vM d_eps_p                        */
+#define SPH_S_HG_WORK      15   /* solid: kinetic energy the hourglass damper removed   */
+#define SPH_S_IE_PAIR      16   /* solid: Sum dt m de_int (pair work conjugate)         */
+#define SPH_S_IE_ALE       17   /* solid: Sum dt m (ALE energy transport)               */
+#define SPH_S_STS_STAGES   18   /* solid: hourglass sub-steps of the last step          */
+#define SPH_S_KE_PRE       19   /* solid: kinetic energy before the damper (scratch)    */
+#define SPH_S_IE_ALE_ADV   20   /* solid: the advection share of SPH_S_IE_ALE           */
+#define SPH_S_IE_ALE_DIFF  21   /* solid: the diffusion share of SPH_S_IE_ALE           */
 #define SPH_NSCAL          32
 
 /* ---- reductions (sph_reduce `which`) ---------------------------------------------- */
@@ -71,6 +82,15 @@ extern "C" {
 #define SPH_R_GROSS_MOMENTUM 4  /* Sum m |v|                                -> out[0]   */
 #define SPH_R_TAIT_ENERGY    5  /* Sum m e_tait(rho), clamped below rho0    -> out[0]   */
 #define SPH_R_KINETIC_OFFGRID 6 /* kinetic energy of non-finite particles   -> out[0]   */
+#define SPH_R_INTERNAL       7  /* solid: Sum m e_int                       -> out[0]   */
+#define SPH_R_ELASTIC        8  /* solid: Sum V (K theta^2/2 + s:s/(4G))    -> out[0]   */
+
+/* ---- materials of the solid step (materials.py:96-97, 506-548) -------------------- */
+#define SPH_EOS_LINEAR       0  /* p = K (rho/rho0 - 1): Tait at gamma = 1, S = K      */
+#define SPH_EOS_TAIT         1  /* p = S ((rho/rho0)^gamma - 1)                         */
+#define SPH_PLASTIC_NONE     0
+#define SPH_PLASTIC_LINEAR   1  /* J2 with linear isotropic hardening (MAT001)          */
+#define SPH_PLASTIC_JC       2  /* Johnson-Cook (MAT002)                                */
 
 /* Cell lattice.  A particle at x sits in cell c = floor((x - origin) / cell) per axis.
  * Cells inside the dense box [box_lo, box_lo + box_n) are addressed directly; every
@@ -154,6 +174,69 @@ typedef struct {
     int32_t compensated_volume;
 } sph_fluid_params;
 
+/* One material: its row of the reference's mat_tab, eos_tab and plastic_tab. */
+typedef struct {
+    double rho0, K, G, c_p, c0, sigma_y0, hardening, cv, T0, e0;
+    double eos_S, eos_gamma, delta;      /* EOS stiffness and exponent; delta-SPH coefficient */
+    double jc_A, jc_B, jc_n, jc_C, jc_eps0_dot, jc_T0, jc_Tm, jc_m;
+    int32_t eos_kind;                    /* SPH_EOS_*     */
+    int32_t plastic_kind;                /* SPH_PLASTIC_* */
+} sph_material;
+
+/* Parameters of the hypoelastic solid step (the HypoElasticSolver constructor,
+ * SOLID.py:138-1400).  Scene-wide numerics shared with the fluid step (CFL, viscosity,
+ * gravity, shifting, walls) stay in sph_fluid_params. */
+typedef struct {
+    double hourglass;        /* HOURGLASS_ALPHA, 0 = off (SOLID.py:642)                    */
+    double hg_rate;          /* alpha c_p / h S (SOLID.py:774-778): the damper's rate       */
+    double w_dx0;            /* W(dx0), the damper's weight normaliser (SOLID.py:755)       */
+    double V0;               /* reference cell: dx0^2 single layer, dx0^3 in 3-D            */
+    double ale_diffusion;    /* PST_ALE_DIFFUSION xi (SOLID.py:379)                         */
+    double ramp_time;        /* Load.rampTime (SOLID.py:5001-5017)                          */
+    double damping;          /* dampingCoefficient (SOLID.py:729)                           */
+    double l_eig_tol;        /* L_EIG_TOL of the corrected grad v (SOLID.py:361)            */
+    int32_t kernel_correction;
+    int32_t jaumann;         /* OBJECTIVE_RATE "jaumann" (1) or "none" (0)                  */
+    int32_t ale_history;     /* PST_ALE_HISTORY: transport s, eps_p and e with the shift    */
+    int32_t sts;             /* HOURGLASS_STS in subcycle mode                              */
+    int32_t sts_stages;      /* fixed sub-step count, 0 = sized from dt                     */
+    int32_t sts_max_stages;
+    int32_t sts_refresh;     /* recompute grad v after the first half kick                  */
+    int32_t dt_pair_mode;    /* 0 all pairs, 1 same material, 2 off (SOLID.py:1031)        */
+    int32_t visc_work;       /* solid_visc_work_on (SOLID.py:1263)                          */
+    int32_t thermal;         /* some material has a specific heat (SOLID.py:494)            */
+    int32_t plastic;         /* some material can yield (SOLID.py:407)                      */
+    int32_t nmat;            /* rows of the material table                                  */
+    int32_t ncons;           /* displacement constraints (grips)                            */
+    int32_t pad_;
+} sph_solid_params;
+
+/* The solid step's per-particle fields (device pointers, Real = double or float as
+ * sph_dims.precision says).
+ *  - history fields sig, eps_p, e_int: two buffers with their own live index `hcur`.
+ *    sph_sort gathers buf[hcur] into buf[1-hcur]; the stress pass reads buf[hcur] and
+ *    writes the updated values to buf[1-hcur]; each flips `hcur`.
+ *  - sorted fields T, de_int, eos_id, bc_*: two buffers that follow sph_state.cur.
+ *  - grad_v, hg_y, v_sync: derived within a step. */
+typedef struct {
+    void*    sig[2];         /* [n][6] deviatoric stress xx,yy,zz,xy,xz,yz (SOLID.py:421)   */
+    void*    eps_p[2];       /* [n] equivalent plastic strain                               */
+    void*    e_int[2];       /* [n] specific internal energy (SOLID.py:488)                 */
+    int32_t  hcur;
+    int32_t  pad_;
+    void*    T[2];           /* [n] temperature (SOLID.py:490)                              */
+    void*    de_int[2];      /* [n] pair-work rate, consumed by the next step (SOLID.py:1240)*/
+    int32_t* eos_id[2];      /* [n] material row (SOLID.py:443)                             */
+    int32_t* bc_flag[2];     /* [n] constrained components, bit k = axis k                  */
+    int32_t* bc_id[2];       /* [n] constraint index, -1 for none                           */
+    void*    bc_disp[2];     /* [n][4] prescribed displacement at load factor 1             */
+    void*    grad_v;         /* [n][9] velocity gradient G[a][b] = dv_a/dx_b, row-major     */
+    double*  hg_y[2];        /* [n][4] hourglass sub-step velocities                        */
+    double*  v_sync;         /* [n][4] velocity after the first half kick                   */
+    const sph_material* mat; /* device [nmat]                                               */
+    double*  cons;           /* device [ncons][6]: grip force xyz, grip displacement xyz    */
+} sph_solid;
+
 /* ---- context ---------------------------------------------------------------------- */
 typedef struct sph_ctx sph_ctx;
 
@@ -204,6 +287,26 @@ int  sph_fluid_pass_shift(sph_ctx* ctx, sph_state* st, const sph_fluid_params* p
  * (sph_base.py:622-655) + banking of the step's f64 work partials. */
 int  sph_fluid_finish(sph_ctx* ctx, sph_state* st, const sph_fluid_params* prm, void* stream);
 
+/* ---- hypoelastic solid step (A19-A28) ---------------------------------------------- */
+/* sph_sort plus the solid's own sorted fields (the registry entries of SOLID.py:443,
+ * 488-499, 836, 1240, 1271, 1307); flips st->cur and sd->hcur. */
+int  sph_solid_sort(sph_ctx* ctx, sph_state* st, sph_solid* sd, void* stream);
+
+/* One HypoElasticSolver.step (SOLID.py:5227-5231): the load factor from the simulation
+ * clock, the sort, the substep's phases in the order of SOLID.py:5060-5196 (lambda,
+ * grad v, stress update and return map, internal energy and temperature, volume, EOS,
+ * stress divergence, adaptive dt and its hourglass cap, the super-time-stepped
+ * hourglass damper or the plain kick, constraints, shifting), the boundary kernels of
+ * SPHBase.step, and the clock.  dt, the load factor and the clock live in st->scal;
+ * the grips' force and displacement in sd->cons. */
+int  sph_solid_step(sph_ctx* ctx, sph_state* st, sph_solid* sd, const sph_fluid_params* prm,
+                    const sph_solid_params* sprm, void* stream);
+
+/* f64 solid reductions (SPH_R_INTERNAL: compute_internal_energy; SPH_R_ELASTIC:
+ * compute_elastic_energy, SOLID.py:2295-2322) into device double out[0]. */
+int  sph_solid_reduce(sph_ctx* ctx, const sph_state* st, const sph_solid* sd, int32_t which,
+                      double* out, void* stream);
+
 /* ---- diagnostics (A29) -------------------------------------------------------------- */
 /* Deterministic f64 reduction over the live buffer into device double out[0..2]
  * (sph_base.py:425-620, DSPH.py:1670-1712).  `g` is the gravitation for POTENTIAL;
@@ -214,7 +317,8 @@ int  sph_reduce(sph_ctx* ctx, const sph_state* st, int32_t which, const double* 
 /* ---- timing helper for the benchmark ------------------------------------------------ */
 /* Average device duration in milliseconds of the pair kernels launched since the last
  * reset, measured with HIP events on the stream they run on (which = 0: volume pass,
- * 1: force pass, 2: shift pass, 3: sort).  Enabled by sph_timing(ctx, 1). */
+ * 1: force pass, 2: shift pass, 3: sort, 4: solid stress pass, 5: solid force pass,
+ * 6: hourglass sub-step pass).  Enabled by sph_timing(ctx, 1). */
 int  sph_timing(sph_ctx* ctx, int32_t enable);
 int  sph_timing_read(sph_ctx* ctx, int32_t which, double* avg_ms, int64_t* launches);
 

@@ -28,7 +28,12 @@ class OrState(ctypes.Structure):
                 ("acc", _dp), ("lam_grad", _dp),
                 ("lam", _dp), ("div", _dp), ("de_visc", _dp),
                 ("L6", _dp),
-                ("scal", _dp)]
+                ("scal", _dp),
+                ("sig", _dp),
+                ("eps_p", _dp), ("e_int", _dp), ("T", _dp), ("de_int", _dp), ("de_hg", _dp),
+                ("eos_id", _ip), ("bc_flag", _ip), ("bc_id", _ip),
+                ("bc_disp", _dp), ("grad_v", _dp), ("cons", _dp),
+                ("mat", ctypes.c_void_p)]
 
 
 def build():
@@ -67,6 +72,23 @@ def lib():
         L.oracle_spd_weight.restype = ctypes.c_double
         L.oracle_tait_energy.argtypes = [ctypes.c_double] * 4
         L.oracle_tait_energy.restype = ctypes.c_double
+        L.oracle_solid_step.argtypes = [vp, vp, vp, ctypes.POINTER(OrState)]
+        L.oracle_solid_step.restype = None
+        for name in ("oracle_solid_velocity_gradient", "oracle_solid_stress_accelerations",
+                     "oracle_solid_update_stress"):
+            f = getattr(L, name)
+            f.argtypes = [vp, vp, vp, ctypes.POINTER(OrState)]
+            f.restype = None
+        L.oracle_solid_hg_accel.argtypes = [vp, vp, vp, ctypes.POINTER(OrState), _dp, _dp]
+        L.oracle_solid_hg_accel.restype = None
+        L.oracle_sort_only.argtypes = [vp, ctypes.POINTER(OrState)]
+        L.oracle_sort_only.restype = None
+        L.oracle_solid_reduce.argtypes = [ctypes.POINTER(OrState), ctypes.c_int]
+        L.oracle_solid_reduce.restype = ctypes.c_double
+        L.oracle_j2_return.argtypes = [_dp] + [ctypes.c_double] * 4 + [_dp]
+        L.oracle_j2_return.restype = None
+        L.oracle_jc_return.argtypes = [_dp] + [ctypes.c_double] * 12 + [_dp, _ip]
+        L.oracle_jc_return.restype = None
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_max_threads.restype = ctypes.c_int
         _lib = L
@@ -82,6 +104,9 @@ class OracleState:
 
     FIELDS3 = ("x", "x0", "u", "v", "shift", "acc", "lam_grad")
     FIELDS1 = ("V", "m", "p", "lam", "div", "de_visc")
+    SOLID_D = {"sig": 6, "eps_p": 0, "e_int": 0, "T": 0, "de_int": 0, "de_hg": 0,
+               "bc_disp": 3, "grad_v": 9}
+    SOLID_I = ("eos_id", "bc_flag", "bc_id")
 
     def __init__(self, n):
         self.n = n
@@ -93,6 +118,21 @@ class OracleState:
         self.pid = np.arange(n, dtype=np.int32)
         self.obj = np.zeros(n, dtype=np.int32)
         self.scal = np.zeros(32)
+        self.solid = False
+
+    def make_solid(self, materials, ncons=0):
+        """Attach the solid fields (zero stress, e = e0, T = T0 of each row's material)
+        and the material table (a ctypes array of SphMaterial)."""
+        n = self.n
+        for f, w in self.SOLID_D.items():
+            setattr(self, f, np.zeros((n, w)) if w else np.zeros(n))
+        for f in self.SOLID_I:
+            setattr(self, f, np.zeros(n, dtype=np.int32))
+        self.bc_id[:] = -1
+        self.cons = np.zeros((max(ncons, 1), 6))
+        self.mat = materials
+        self.solid = True
+        return self
 
     @classmethod
     def from_snapshot(cls, snap):
@@ -109,6 +149,12 @@ class OracleState:
         st.pid, st.obj = _ptr(self.pid, _ip), _ptr(self.obj, _ip)
         for f in self.FIELDS3 + self.FIELDS1 + ("L6", "scal"):
             setattr(st, f, _ptr(getattr(self, f)))
+        if self.solid:
+            for f in list(self.SOLID_D) + ["cons"]:
+                setattr(st, f, _ptr(getattr(self, f)))
+            for f in self.SOLID_I:
+                setattr(st, f, _ptr(getattr(self, f), _ip))
+            st.mat = ctypes.cast(self.mat, ctypes.c_void_p)
         self._c = st
         return st
 
@@ -120,6 +166,55 @@ class OracleState:
 def fluid_step(dims, params, s):
     st = s.c()
     lib().oracle_fluid_step(ctypes.byref(dims), ctypes.byref(params), ctypes.byref(st))
+
+
+def solid_step(dims, params, sparams, s):
+    st = s.c()
+    lib().oracle_solid_step(ctypes.byref(dims), ctypes.byref(params), ctypes.byref(sparams),
+                            ctypes.byref(st))
+
+
+def sort_only(dims, s):
+    st = s.c()
+    lib().oracle_sort_only(ctypes.byref(dims), ctypes.byref(st))
+
+
+def solid_phase(name, dims, params, sparams, s):
+    """Run one solid phase (velocity_gradient, stress_accelerations, update_stress) on
+    the state as it stands, with the grid of the last sort."""
+    st = s.c()
+    getattr(lib(), "oracle_solid_" + name)(ctypes.byref(dims), ctypes.byref(params),
+                                           ctypes.byref(sparams), ctypes.byref(st))
+
+
+def solid_hg_accel(dims, params, sparams, s, y):
+    y = np.ascontiguousarray(y, dtype=np.float64).reshape(s.n, 3)
+    a = np.zeros((s.n, 3))
+    st = s.c()
+    lib().oracle_solid_hg_accel(ctypes.byref(dims), ctypes.byref(params), ctypes.byref(sparams),
+                                ctypes.byref(st), _ptr(y), _ptr(a))
+    return a
+
+
+def solid_reduce(s, which):
+    st = s.c()
+    return lib().oracle_solid_reduce(ctypes.byref(st), int(which))
+
+
+def j2_return(s, G, sy, H, eps_p):
+    q = np.ascontiguousarray(s, dtype=np.float64).reshape(9).copy()
+    d = np.zeros(1)
+    lib().oracle_j2_return(_ptr(q), float(G), float(sy), float(H), float(eps_p), _ptr(d))
+    return q.reshape(3, 3), float(d[0])
+
+
+def jc_return(s, G, A, B, n, C, eps0_dot, T0, Tm, m, eps_p, temp, dt):
+    q = np.ascontiguousarray(s, dtype=np.float64).reshape(9).copy()
+    d = np.zeros(1)
+    it = np.zeros(1, dtype=np.int32)
+    lib().oracle_jc_return(_ptr(q), *[float(v) for v in (G, A, B, n, C, eps0_dot, T0, Tm, m,
+                                                          eps_p, temp, dt)], _ptr(d), _ptr(it, _ip))
+    return q.reshape(3, 3), float(d[0]), int(it[0])
 
 
 def sort_and_pairs(dims, s, cap=None):

@@ -42,6 +42,14 @@ typedef struct {
     double *lam, *div, *de_visc;         /* [n]    */
     double *L6;                          /* [n][6] (00,11,22,01,02,12) */
     double *scal;                        /* [SPH_NSCAL] */
+    /* the solid step's fields; all NULL on a fluid state */
+    double *sig;                         /* [n][6] deviatoric stress xx,yy,zz,xy,xz,yz */
+    double *eps_p, *e_int, *T, *de_int, *de_hg;   /* [n] */
+    int32_t *eos_id, *bc_flag, *bc_id;   /* [n] */
+    double *bc_disp;                     /* [n][3] */
+    double *grad_v;                      /* [n][9] row-major, G[a][b] = dv_a/dx_b */
+    double *cons;                        /* [ncons][6] grip force (3), grip displacement (3) */
+    const sph_material* mat;             /* [nmat] */
 } or_state;
 
 typedef struct {
@@ -141,7 +149,7 @@ int64_t oracle_sort(const sph_dims* d, or_state* s, or_grid* g) {
         r[i].key = key; r[i].pid = s->pid[i]; r[i].idx = i;
     }
     qsort(r, n, sizeof(or_sortrec), or_cmp);
-    int64_t* perm = (int64_t*)malloc(sizeof(int64_t) * n);
+    int64_t* perm = (int64_t*)calloc(n > 0 ? n : 1, sizeof(int64_t));
     for (int64_t k = 0; k < n; ++k) perm[k] = r[k].idx;
     or_permute_i(s->pid, perm, n);
     or_permute_i(s->obj, perm, n);
@@ -153,6 +161,20 @@ int64_t oracle_sort(const sph_dims* d, or_state* s, or_grid* g) {
     or_permute_d(s->V, perm, n, 1);
     or_permute_d(s->m, perm, n, 1);
     or_permute_d(s->p, perm, n, 1);
+    if (s->sig) {          /* the solid's sorted fields (SOLID.py:443,488-499,836,1240,1271,1307) */
+        or_permute_d(s->sig, perm, n, 6);
+        or_permute_d(s->eps_p, perm, n, 1);
+        or_permute_d(s->e_int, perm, n, 1);
+        or_permute_d(s->T, perm, n, 1);
+        or_permute_d(s->de_int, perm, n, 1);
+        or_permute_d(s->de_hg, perm, n, 1);
+        or_permute_d(s->de_visc, perm, n, 1);
+        or_permute_d(s->acc, perm, n, 3);
+        or_permute_i(s->eos_id, perm, n);
+        or_permute_i(s->bc_flag, perm, n);
+        or_permute_i(s->bc_id, perm, n);
+        or_permute_d(s->bc_disp, perm, n, 3);
+    }
     free(g->key); free(g->cell); free(g->start);
     g->n = n;
     g->nbuck = offgrid + 1;
@@ -455,9 +477,22 @@ static void or_mirror(int mir, double* a) {
 /* ------------------------------------------------------------------------------------ */
 typedef struct {
     const sph_dims* d; const sph_fluid_params* P; or_state* s;
-    double acc[9];     /* generic accumulator */
+    double acc[16];    /* generic accumulator */
     double dt;
+    const sph_solid_params* S;    /* solid step only */
 } or_tctx;
+
+/* same_material and pair_delta of the solid (SOLID.py:4037-4064, 4102-4114); the fluid
+ * branch has one material and the scene-wide delta (DSPH.py:1182-1206) */
+static int or_same_material(const or_state* s, int64_t i, int64_t j) {
+    return s->eos_id == NULL || s->eos_id[i] == s->eos_id[j];
+}
+static double or_pair_delta(const sph_fluid_params* P, const or_state* s, int64_t i, int64_t j) {
+    if (s->eos_id == NULL) return P->delta_sph;
+    double di = s->mat[s->eos_id[i]].delta, dj = s->mat[s->eos_id[j]].delta;
+    double den = di + dj;
+    return den > 1e-12 ? 2.0 * di * dj / den : 0.0;
+}
 
 /* compute_L_task (DSPH.py:563-589) */
 static void or_L_task(void* c, int64_t i, int64_t j, int mir, const double* r) {
@@ -544,12 +579,13 @@ static void or_depsvol_task(void* c, int64_t i, int64_t j, int mir, const double
     double cont = 0.0;
     for (int k = 0; k < 3; ++k) cont += (vi[k] - vj[k]) * gW[k];
     t->acc[0] += wj * cont;
-    if (P->delta_sph > 0.0) {
+    double delta = or_same_material(s, i, j) ? or_pair_delta(P, s, i, j) : 0.0;
+    if (delta > 0.0) {
         double h = t->d->h, eta2 = 0.01 * h * h;
         double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
         double pg = 0.0;
         for (int k = 0; k < 3; ++k) pg += (2.0 * (rho_j - rho_i) * (-r[k]) / (r2 + eta2)) * gW[k];
-        double diffusion = P->delta_sph * h * P->c0 * wj * pg;
+        double diffusion = delta * h * P->c0 * wj * pg;
         double diff_rate = diffusion / rho_i;
         t->acc[0] += diff_rate;
         t->acc[1] += diff_rate;
@@ -620,6 +656,7 @@ static void or_dt_task(void* c, int64_t i, int64_t j, int mir, const double* r) 
     (void)mir; (void)r;
     or_tctx* t = (or_tctx*)c;
     const or_state* s = t->s;
+    if (t->S && t->S->dt_pair_mode == 1 && !or_same_material(s, i, j)) return;   /* SOLID.py:4145 */
     double dp = fabs(s->p[i] - s->p[j]);
     double drho = fabs(s->m[i] / s->V[i] - s->m[j] / s->V[j]);
     if (drho > 1e-3 * t->P->rho0) {
@@ -647,13 +684,14 @@ static void or_pressure_accelerations(const sph_dims* d, const sph_fluid_params*
 }
 
 /* compute_adaptive_dt (DSPH.py:1365-1430) */
-static void or_adaptive_dt(const sph_dims* d, const sph_fluid_params* P, or_state* s, const or_grid* g) {
+static void or_adaptive_dt(const sph_dims* d, const sph_fluid_params* P, or_state* s, const or_grid* g,
+                           const sph_solid_params* S) {
     int64_t n = s->n;
     double* c2 = (double*)malloc(sizeof(double) * n);
     double max_c2 = P->c_signal * P->c_signal, max_a = 1e-10;
     #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < n; ++i) {
-        or_tctx t = { d, P, s, { 0 }, 0.0 };
+        or_tctx t = { d, P, s, { 0 }, 0.0, S };
         t.acc[0] = P->c_signal * P->c_signal;
         if (P->dt_pair) or_for_all_neighbors(d, s, g, i, or_dt_task, &t);
         c2[i] = t.acc[0];
@@ -823,6 +861,7 @@ static void or_apply_pst(const sph_dims* d, const sph_fluid_params* P, or_state*
             }
         }
         if (d->flags & SPH_TWO_D) dr[2] = 0.0;
+        if (s->bc_flag && s->bc_flag[i] != 0) dr[0] = dr[1] = dr[2] = 0.0;   /* DSPH.py:927-933 */
         for (int k = 0; k < 3; ++k) s->shift[3 * i + k] = dr[k];
     }
     for (int64_t i = 0; i < n; ++i) or_displace(d, P, s, i, s->shift + 3 * i);
@@ -836,6 +875,11 @@ static void or_place(const sph_fluid_params* P, or_state* s, int64_t i, int k, d
 static void or_reflect(const sph_fluid_params* P, or_state* s, int64_t i, int k) {
     s->x[3 * i + k] = -s->x[3 * i + k];
     if (P->displacement_positions) s->u[3 * i + k] = -s->u[3 * i + k] - 2.0 * s->x0[3 * i + k];
+    if (s->sig) {          /* the deviator picks up M s M (sph_base.py:329, 358, 367) */
+        double* q = s->sig + 6 * i;
+        if (k == 1) { q[3] = -q[3]; q[5] = -q[5]; }
+        else        { q[4] = -q[4]; q[5] = -q[5]; }
+    }
 }
 
 /* simulate_collisions (sph_base.py:160-221) */
@@ -945,7 +989,7 @@ void oracle_fluid_step(const sph_dims* d, const sph_fluid_params* P, or_state* s
     or_compute_lam(d, P, s, &g_grid);
     or_volume(d, P, s, &g_grid);
     or_pressure_accelerations(d, P, s, &g_grid);
-    or_adaptive_dt(d, P, s, &g_grid);
+    or_adaptive_dt(d, P, s, &g_grid, NULL);
     or_step_work(P, s);
     or_advect(d, P, s);
     if (P->pst_enabled) or_apply_pst(d, P, s, &g_grid);
@@ -976,4 +1020,683 @@ void oracle_reduce(const or_state* s, int which, const double* g, const sph_flui
         default: break;
         }
     }
+}
+
+/* ==================================================================================== */
+/* the hypoelastic solid step (SOLID.py), literal phase order of substep (5060-5196)     */
+/* ==================================================================================== */
+static void or_sym6(const double* q, double S[3][3]) {
+    S[0][0] = q[0]; S[1][1] = q[1]; S[2][2] = q[2];
+    S[0][1] = S[1][0] = q[3]; S[0][2] = S[2][0] = q[4]; S[1][2] = S[2][1] = q[5];
+}
+/* mirror_mat (particle_system.py:1141-1146): M T M, M = diag(1, sy, sz) */
+static void or_mirror_mat(int mir, double T[3][3]) {
+    double m[3] = { 1.0, (mir & 1) ? -1.0 : 1.0, (mir & 2) ? -1.0 : 1.0 };
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) T[a][b] *= m[a] * m[b];
+}
+static double or_r_axi(const sph_fluid_params* P, const or_state* s, int64_t p) {
+    return fmax(s->x[3 * p + 1], P->axi_r_min);          /* particle_system.py:1076-1078 */
+}
+static const sph_material* or_mat(const or_state* s, int64_t p) { return s->mat + s->eos_id[p]; }
+
+/* j2_radial_return (MAT001_linear_plasticity.py:65-81) */
+void oracle_j2_return(double s[9], double G, double sy, double H, double eps_p, double* d_eps_p) {
+    *d_eps_p = 0.0;
+    if (sy > 0.0) {
+        double ss = 0.0;
+        for (int k = 0; k < 9; ++k) ss += s[k] * s[k];
+        double vm = sqrt(1.5 * ss);
+        double sig_y = sy + H * eps_p;
+        if (vm > sig_y && (3.0 * G + H) > 0.0) {
+            double d = (vm - sig_y) / (3.0 * G + H);
+            double sc = (sig_y + H * d) / vm;
+            for (int k = 0; k < 9; ++k) s[k] *= sc;
+            *d_eps_p = d;
+        }
+    }
+}
+
+/* _jc_residual (MAT002_johnson_cook.py:251-277) */
+static double or_jc_residual(double d, double vm, double G, double A, double B, double n,
+                             double C, double e0d, double m_temp, double eps_p, double dt,
+                             double* dg) {
+    double et = eps_p + d;
+    double ss = A, dss = 0.0;
+    if (et > 0.0 && B > 0.0 && n > 0.0) {
+        ss += B * pow(et, n);
+        if (et > 1e-12) dss = n * B * pow(et, n - 1.0);
+    }
+    double rate = 1.0;
+    if (dt > 0.0 && e0d > 0.0) rate = (d / dt) / e0d;
+    double mr = 1.0, dmr = 0.0;
+    if (rate > 1.0 && C > 0.0) { mr = 1.0 + C * log(rate); dmr = C / fmax(1e-12, d); }
+    double sy = ss * mr * m_temp;
+    *dg = -3.0 * G - (dss * mr + ss * dmr) * m_temp;
+    return vm - 3.0 * G * d - sy;
+}
+
+/* jc_radial_return (MAT002_johnson_cook.py:280-354); the discarded d_temp is not formed */
+void oracle_jc_return(double s[9], double G, double A, double B, double n, double C, double e0d,
+                      double T0, double Tm, double mexp, double eps_p, double temp, double dt,
+                      double* d_eps_p, int* iters) {
+    *d_eps_p = 0.0;
+    if (iters) *iters = 0;
+    if (!(G > 0.0 && A > 0.0)) return;
+    double ss = 0.0;
+    for (int k = 0; k < 9; ++k) ss += s[k] * s[k];
+    double vm = sqrt(1.5 * ss);
+    double m_temp = 1.0;
+    if (Tm > T0 && temp > T0) {
+        double ts = fmin(1.0, fmax(0.0, (temp - T0) / (Tm - T0)));
+        m_temp = fmax(0.0, 1.0 - pow(ts, mexp));
+    }
+    double s0 = A;
+    if (eps_p > 0.0 && B > 0.0 && n > 0.0) s0 += B * pow(eps_p, n);
+    double sy0 = s0 * m_temp;
+    if (!(vm > sy0 && sy0 > 0.0)) return;
+    double lo = 0.0, hi = (vm - sy0) / (3.0 * G), dg;
+    double d_ref = 0.0;
+    if (dt > 0.0 && e0d > 0.0 && C > 0.0) d_ref = dt * e0d;
+    if (d_ref > 0.0 && d_ref < hi) {
+        double g = or_jc_residual(d_ref, vm, G, A, B, n, C, e0d, m_temp, eps_p, dt, &dg);
+        if (g > 0.0) lo = d_ref; else hi = d_ref;
+    }
+    double d = lo > 0.0 ? lo : hi;
+    int it;
+    for (it = 0; it < 40; ++it) {
+        double g = or_jc_residual(d, vm, G, A, B, n, C, e0d, m_temp, eps_p, dt, &dg);
+        if (g > 0.0) lo = d; else hi = d;
+        if (fabs(g) <= 1e-6 * vm || hi - lo <= 1e-6 * hi) break;
+        double dn = d - g / dg;
+        if (!(dn > lo && dn < hi)) dn = lo > 0.0 ? sqrt(lo * hi) : 0.5 * hi;
+        if (fabs(dn - d) <= 1e-6 * d) { d = dn; break; }
+        d = dn;
+    }
+    if (iters) *iters = it + 1;
+    double sy_final = fmax(0.0, vm - 3.0 * G * d);
+    double sc = sy_final / vm;
+    for (int k = 0; k < 9; ++k) s[k] *= sc;
+    *d_eps_p = d;
+}
+
+/* compute_grad_v_task (SOLID.py:1721-1731) */
+static void or_gradv_task(void* c, int64_t i, int64_t j, int mir, const double* r) {
+    or_tctx* t = (or_tctx*)c;
+    const or_state* s = t->s;
+    double gW[3];
+    oracle_wendland_grad(t->d, r, gW);
+    double vj[3] = { s->v[3 * j], s->v[3 * j + 1], s->v[3 * j + 2] };
+    or_mirror(mir, vj);
+    double wj = or_w(t->d, t->P, s, j);
+    for (int a = 0; a < 3; ++a) {
+        double dv = vj[a] - s->v[3 * i + a];
+        for (int b = 0; b < 3; ++b) t->acc[3 * a + b] += wj * (dv * gW[b]);
+    }
+}
+
+/* compute_velocity_gradient (SOLID.py:1743-1794) */
+static void or_velocity_gradient(const sph_dims* d, const sph_fluid_params* P,
+                                 const sph_solid_params* S, or_state* s, const or_grid* g) {
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < s->n; ++i) {
+        or_tctx t = { d, P, s, { 0 }, 0.0, S };
+        or_for_all_neighbors(d, s, g, i, or_gradv_task, &t);
+        double G[3][3];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) G[a][b] = t.acc[3 * a + b];
+        if (S->kernel_correction) {
+            or_tctx e = { d, P, s, { 0 }, 0.0, S };
+            or_for_all_neighbors(d, s, g, i, or_L_task, &e);       /* compute_E_task 1733-1741 */
+            double L6[6], me, L[3][3];
+            if (d->flags & SPH_TWO_D) oracle_spd_inverse2(e.acc, S->l_eig_tol, L6);
+            else oracle_spd_inverse3(e.acc, S->l_eig_tol, L6, &me);
+            or_sym6(L6, L);
+            if (d->flags & SPH_TWO_D) { L[2][2] = 1.0; L[0][2] = L[2][0] = L[1][2] = L[2][1] = 0.0; }
+            double H[3][3];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    double acc = 0.0;
+                    for (int k = 0; k < 3; ++k) acc += G[a][k] * L[k][b];
+                    H[a][b] = acc;
+                }
+            memcpy(G, H, sizeof G);
+        }
+        if (d->flags & SPH_TWO_D)
+            for (int k = 0; k < 3; ++k) G[2][k] = G[k][2] = 0.0;
+        if (d->flags & SPH_AXISYMMETRIC) G[2][2] = s->v[3 * i + 1] / or_r_axi(P, s, i);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) s->grad_v[9 * i + 3 * a + b] = G[a][b];
+    }
+}
+
+/* compute_ale_transport_task and _eps_p_task (SOLID.py:1799-1862): the conservative
+ * difference form of (du . grad) phi for the deviator (acc 0-8) and eps_p (acc 9) */
+static void or_ale_hist_task(void* c, int64_t i, int64_t j, int mir, const double* r) {
+    or_tctx* t = (or_tctx*)c;
+    const or_state* s = t->s;
+    double gW[3];
+    oracle_wendland_grad(t->d, r, gW);
+    double dui[3], duj[3];
+    for (int k = 0; k < 3; ++k) { dui[k] = s->shift[3 * i + k] / t->dt; duj[k] = s->shift[3 * j + k] / t->dt; }
+    or_mirror(mir, duj);
+    double Sj[3][3], Si[3][3];
+    or_sym6(s->sig + 6 * j, Sj);
+    or_mirror_mat(mir, Sj);
+    or_sym6(s->sig + 6 * i, Si);
+    double rho_i = s->m[i] / s->V[i], rho_j = s->m[j] / s->V[j];
+    double djg = duj[0] * gW[0] + duj[1] * gW[1] + duj[2] * gW[2];
+    double dig = dui[0] * gW[0] + dui[1] * gW[1] + dui[2] * gW[2];
+    double drd = 0.0;
+    for (int k = 0; k < 3; ++k) drd += (rho_j * duj[k] - rho_i * dui[k]) * gW[k];
+    double wj = or_w(t->d, t->P, s, j);
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            double flux = (rho_j * djg) * Sj[a][b] - (rho_i * dig) * Si[a][b];
+            t->acc[3 * a + b] += wj * (flux - Si[a][b] * drd) / rho_i;
+        }
+    if (t->S->plastic) {
+        double ei = s->eps_p[i], ej = s->eps_p[j];
+        double flux = (rho_j * djg) * ej - (rho_i * dig) * ei;
+        t->acc[9] += wj * (flux - ei * drd) / rho_i;
+    }
+}
+
+/* update_stress (SOLID.py:2691-3004), with the J2 and Johnson-Cook return maps */
+static void or_update_stress(const sph_dims* d, const sph_fluid_params* P,
+                             const sph_solid_params* S, or_state* s, const or_grid* g) {
+    int64_t n = s->n;
+    double dt = s->scal[SPH_S_DT];
+    double* dsig = (double*)malloc(sizeof(double) * 9 * n);
+    double* dep = (double*)calloc(n, sizeof(double));
+    double* pw = (double*)calloc(n, sizeof(double));
+    int ale = P->pst_enabled && S->ale_history;
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+        const double* Lv = s->grad_v + 9 * i;
+        double ed[3][3], sp[3][3], Si[3][3], sd[3][3];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                ed[a][b] = 0.5 * (Lv[3 * a + b] + Lv[3 * b + a]);
+                sp[a][b] = 0.5 * (Lv[3 * a + b] - Lv[3 * b + a]);
+            }
+        double tr = ed[0][0] + ed[1][1] + ed[2][2];
+        or_sym6(s->sig + 6 * i, Si);
+        double G = or_mat(s, i)->G;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) sd[a][b] = 2.0 * G * (ed[a][b] - (a == b ? tr / 3.0 : 0.0));
+        if (S->jaumann)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    double ws = 0.0, sw = 0.0;
+                    for (int k = 0; k < 3; ++k) { ws += sp[a][k] * Si[k][b]; sw += Si[a][k] * sp[k][b]; }
+                    sd[a][b] += ws - sw;
+                }
+        if (ale) {
+            or_tctx t = { d, P, s, { 0 }, dt, S };
+            or_for_all_neighbors(d, s, g, i, or_ale_hist_task, &t);
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) sd[a][b] += t.acc[3 * a + b];
+            dep[i] = t.acc[9];
+        }
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) dsig[9 * i + 3 * a + b] = sd[a][b];
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        double Si[3][3], sn[9], sy[9];
+        or_sym6(s->sig + 6 * i, Si);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) sn[3 * a + b] = Si[a][b] + dt * dsig[9 * i + 3 * a + b];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) sy[3 * a + b] = 0.5 * (sn[3 * a + b] + sn[3 * b + a]);
+        double tr = sy[0] + sy[4] + sy[8];
+        for (int a = 0; a < 3; ++a) sy[4 * a] -= tr / 3.0;
+        const sph_material* M = or_mat(s, i);
+        if (M->G <= 0.0) for (int k = 0; k < 9; ++k) sy[k] = 0.0;
+        if (ale && S->plastic) s->eps_p[i] = fmax(0.0, s->eps_p[i] + dt * dep[i]);
+        double dp = 0.0;
+        if (S->plastic) {
+            if (M->plastic_kind == SPH_PLASTIC_LINEAR)
+                oracle_j2_return(sy, M->G, M->sigma_y0, M->hardening, s->eps_p[i], &dp);
+            else if (M->plastic_kind == SPH_PLASTIC_JC)
+                oracle_jc_return(sy, M->G, M->jc_A, M->jc_B, M->jc_n, M->jc_C, M->jc_eps0_dot,
+                                 M->jc_T0, M->jc_Tm, M->jc_m, s->eps_p[i], s->T[i], dt, &dp, NULL);
+            s->eps_p[i] += dp;
+            if (dp > 0.0) {
+                double ss = 0.0;
+                for (int k = 0; k < 9; ++k) ss += sy[k] * sy[k];
+                pw[i] = s->V[i] * sqrt(1.5 * ss) * dp;
+            }
+        }
+        double* q = s->sig + 6 * i;
+        q[0] = sy[0]; q[1] = sy[4]; q[2] = sy[8]; q[3] = sy[1]; q[4] = sy[2]; q[5] = sy[5];
+    }
+    double w = 0.0;
+    for (int64_t i = 0; i < n; ++i) w += pw[i];
+    s->scal[SPH_S_PLASTIC_WORK] += w;
+    free(dsig); free(dep); free(pw);
+}
+
+/* compute_ale_transport_e_task (SOLID.py:1949-2045): acc[0] advection, acc[1] diffusion */
+static void or_ale_e_task(void* c, int64_t i, int64_t j, int mir, const double* r) {
+    or_tctx* t = (or_tctx*)c;
+    const or_state* s = t->s;
+    double gW[3];
+    oracle_wendland_grad(t->d, r, gW);
+    double dui[3], duj[3];
+    for (int k = 0; k < 3; ++k) { dui[k] = s->shift[3 * i + k] / t->dt; duj[k] = s->shift[3 * j + k] / t->dt; }
+    or_mirror(mir, duj);
+    double rho_i = s->m[i] / s->V[i], rho_j = s->m[j] / s->V[j];
+    double ei = s->e_int[i], ej = s->e_int[j];
+    double wj = or_w(t->d, t->P, s, j);
+    double djg = duj[0] * gW[0] + duj[1] * gW[1] + duj[2] * gW[2];
+    t->acc[0] += wj * rho_j * (ej - ei) * djg / rho_i;
+    if (t->S->ale_diffusion > 0.0 && or_same_material(s, i, j)) {
+        double h = t->d->h, eta2 = 0.01 * h * h;
+        double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        double pg = 0.0;
+        for (int k = 0; k < 3; ++k) pg += (2.0 * (ej - ei) * (-r[k]) / (r2 + eta2)) * gW[k];
+        double su[3] = { dui[0] + duj[0], dui[1] + duj[1], dui[2] + duj[2] };
+        double speed = 0.5 * sqrt(su[0] * su[0] + su[1] * su[1] + su[2] * su[2]);
+        double rho_bar = 0.5 * (rho_i + rho_j);
+        t->acc[1] += t->S->ale_diffusion * 0.5 * t->P->dx0 * speed * wj * pg * rho_bar / rho_i;
+    }
+}
+
+/* compute_ale_e_rates + update_internal_energy (SOLID.py:2047-2237) */
+static void or_internal_energy(const sph_dims* d, const sph_fluid_params* P,
+                               const sph_solid_params* S, or_state* s, const or_grid* g) {
+    int64_t n = s->n;
+    double dt = s->scal[SPH_S_DT];
+    int transport_e = P->pst_enabled && S->ale_history;
+    double* rate = (double*)calloc(2 * n, sizeof(double));
+    if (transport_e) {
+        #pragma omp parallel for schedule(dynamic, 256)
+        for (int64_t i = 0; i < n; ++i) {
+            or_tctx t = { d, P, s, { 0 }, dt, S };
+            or_for_all_neighbors(d, s, g, i, or_ale_e_task, &t);
+            rate[2 * i] = t.acc[0];
+            rate[2 * i + 1] = t.acc[1];
+        }
+    }
+    double ie_pair = 0.0, ie_ale = 0.0, ie_adv = 0.0, ie_diff = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        double de = s->de_int[i];
+        ie_pair += dt * s->m[i] * s->de_int[i];
+        if (transport_e) {
+            double adv = rate[2 * i], ale = adv + rate[2 * i + 1];
+            double mdt = dt * s->m[i];
+            de += ale;
+            ie_adv += mdt * adv;
+            ie_diff += mdt * rate[2 * i + 1];
+            ie_ale += dt * s->m[i] * ale;
+        }
+        s->e_int[i] += dt * de;
+    }
+    s->scal[SPH_S_IE_PAIR] += ie_pair;
+    s->scal[SPH_S_IE_ALE] += ie_ale;
+    s->scal[SPH_S_IE_ALE_ADV] += ie_adv;
+    s->scal[SPH_S_IE_ALE_DIFF] += ie_diff;
+    if (S->hourglass > 0.0 && !S->sts) {
+        double w = 0.0;
+        for (int64_t i = 0; i < n; ++i) w += dt * or_mw(d, P, s, i) * s->de_hg[i];
+        s->scal[SPH_S_HG_WORK] += w;
+    }
+    if (S->visc_work) {
+        double w = 0.0;
+        for (int64_t i = 0; i < n; ++i) w += s->m[i] * s->de_visc[i];
+        s->scal[SPH_S_VISC_WORK] += dt * w;
+    }
+    double g2 = P->g[0] * P->g[0] + P->g[1] * P->g[1] + P->g[2] * P->g[2], wl = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        double f[3];
+        for (int k = 0; k < 3; ++k) f[k] = s->acc[3 * i + k] - P->g[k];
+        wl += 0.5 * s->m[i] * dt * dt * (f[0] * f[0] + f[1] * f[1] + f[2] * f[2] - g2);
+    }
+    s->scal[SPH_S_LEFTOVER] += wl;
+    free(rate);
+}
+
+/* cold_specific_energy (SOLID.py:2511-2567) for the Tait/linear rows */
+static double or_cold_energy(const sph_fluid_params* P, const or_state* s, int64_t i) {
+    const sph_material* M = or_mat(s, i);
+    double rho = s->m[i] / s->V[i];
+    double e = oracle_tait_energy(rho, M->rho0, M->eos_S, M->eos_gamma);
+    if (!P->allow_negative_pressure && rho < M->rho0) e = 0.0;
+    return e;
+}
+
+/* update_temperature (SOLID.py:2239-2293) */
+static void or_temperature(const sph_fluid_params* P, or_state* s) {
+    for (int64_t i = 0; i < s->n; ++i) {
+        const sph_material* M = or_mat(s, i);
+        if (M->cv > 0.0) {
+            double e_th = s->e_int[i] - or_cold_energy(P, s, i) - M->e0;
+            s->T[i] = M->T0 + e_th / M->cv;
+        }
+    }
+}
+
+/* eos_pressure_i for the linear/Tait branch (SOLID.py:3009-3038, 3142-3181) */
+static double or_solid_pressure(const sph_fluid_params* P, const sph_material* M, double rho) {
+    double p = M->eos_kind == SPH_EOS_LINEAR ? M->eos_S * (rho / M->rho0 - 1.0)
+                                             : M->eos_S * (pow(rho / M->rho0, M->eos_gamma) - 1.0);
+    if (!P->allow_negative_pressure) p = fmax(p, 0.0);
+    return p;
+}
+
+/* compute_stress_accel_task (SOLID.py:3544-3852): acc 0-2 dv/dt, 3 de_int, 4 de_visc,
+ * 5 de_hg */
+static void or_stress_task(void* c, int64_t i, int64_t j, int mir, const double* r) {
+    or_tctx* t = (or_tctx*)c;
+    const or_state* s = t->s;
+    const sph_fluid_params* P = t->P;
+    const sph_solid_params* S = t->S;
+    const sph_dims* d = t->d;
+    double gW[3];
+    oracle_wendland_grad(d, r, gW);
+    double vj[3] = { s->v[3 * j], s->v[3 * j + 1], s->v[3 * j + 2] };
+    or_mirror(mir, vj);
+    const double* vi = s->v + 3 * i;
+    double Si[3][3], Sj[3][3];
+    or_sym6(s->sig + 6 * i, Si);
+    or_sym6(s->sig + 6 * j, Sj);
+    or_mirror_mat(mir, Sj);
+    double sg[3][3];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            sg[a][b] = (-s->p[i] * (a == b) + Si[a][b]) + (-s->p[j] * (a == b) + Sj[a][b]);
+    double wi = or_w(d, P, s, i), wj = or_w(d, P, s, j), mwi = or_mw(d, P, s, i);
+    double f[3];
+    for (int a = 0; a < 3; ++a) f[a] = wi * wj * (sg[a][0] * gW[0] + sg[a][1] * gW[1] + sg[a][2] * gW[2]);
+    double vij[3] = { vi[0] - vj[0], vi[1] - vj[1], vi[2] - vj[2] };
+    for (int k = 0; k < 3; ++k) t->acc[k] += f[k] / mwi;
+    t->acc[3] += -0.5 * (f[0] * vij[0] + f[1] * vij[1] + f[2] * vij[2]) / mwi;
+    double dd = 2.0 * (3 + 2);
+    double vxy = vij[0] * r[0] + vij[1] * r[1] + vij[2] * r[2];
+    double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double eta2 = 0.01 * d->h * d->h;
+    double am[3];
+    for (int k = 0; k < 3; ++k) am[k] = dd * P->viscosity * wj * vxy / (r2 + eta2) * gW[k];
+    for (int k = 0; k < 3; ++k) t->acc[k] += am[k];
+    if (P->viscosity > 0.0) {
+        double wm = -0.5 * (am[0] * vij[0] + am[1] * vij[1] + am[2] * vij[2]);
+        t->acc[3] += wm;
+        if (S->visc_work) t->acc[4] += -wm;
+    }
+    if (S->hourglass > 0.0 && !S->sts) {
+        double rn = sqrt(r2);
+        if (rn > 1e-9) {
+            double rh[3] = { -r[0] / rn, -r[1] / rn, -r[2] / rn };
+            double Gj[3][3], dv[3];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) Gj[a][b] = s->grad_v[9 * j + 3 * a + b];
+            or_mirror_mat(mir, Gj);
+            for (int a = 0; a < 3; ++a) {
+                double ga = 0.0;
+                for (int b = 0; b < 3; ++b) ga += 0.5 * (s->grad_v[9 * i + 3 * a + b] + Gj[a][b]) * (-r[b]);
+                dv[a] = (vj[a] - vi[a]) - ga;
+            }
+            double hg = dv[0] * rh[0] + dv[1] * rh[1] + dv[2] * rh[2];
+            double Wij = oracle_wendland(d, rn);
+            const sph_material* M = or_mat(s, i);
+            double rho_i = s->m[i] / s->V[i];
+            double damp = S->hourglass * (M->rho0 / rho_i) * (M->c_p / d->h) * (wj / S->V0)
+                          * (Wij / S->w_dx0) * hg;
+            double ah[3] = { damp * rh[0], damp * rh[1], damp * rh[2] };
+            for (int k = 0; k < 3; ++k) t->acc[k] += ah[k];
+            t->acc[5] += -0.5 * (ah[0] * vij[0] + ah[1] * vij[1] + ah[2] * vij[2]);
+        }
+    }
+    if (P->pst_enabled && P->pst_ale_momentum) {
+        double dui[3], duj[3];
+        for (int k = 0; k < 3; ++k) { dui[k] = s->shift[3 * i + k] / t->dt; duj[k] = s->shift[3 * j + k] / t->dt; }
+        or_mirror(mir, duj);
+        double rho_i = s->m[i] / s->V[i], rho_j = s->m[j] / s->V[j];
+        double djg = duj[0] * gW[0] + duj[1] * gW[1] + duj[2] * gW[2];
+        double dig = dui[0] * gW[0] + dui[1] * gW[1] + dui[2] * gW[2];
+        double drd = 0.0;
+        for (int k = 0; k < 3; ++k) drd += (rho_j * duj[k] - rho_i * dui[k]) * gW[k];
+        for (int k = 0; k < 3; ++k) {
+            double dv = (rho_j * djg) * vj[k] - (rho_i * dig) * vi[k];
+            t->acc[k] += wj * (dv - vi[k] * drd) / rho_i;
+        }
+    }
+}
+
+/* compute_non_pressure_forces + compute_stress_accelerations (DSPH.py:1642-1650,
+ * SOLID.py:3204-3521) */
+static void or_stress_accelerations(const sph_dims* d, const sph_fluid_params* P,
+                                    const sph_solid_params* S, or_state* s, const or_grid* g) {
+    int64_t n = s->n;
+    double dt = s->scal[SPH_S_DT];
+    for (int64_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) s->acc[3 * i + k] = P->g[k];
+        s->p[i] = or_solid_pressure(P, or_mat(s, i), s->m[i] / s->V[i]);
+        s->de_int[i] = 0.0; s->de_hg[i] = 0.0; s->de_visc[i] = 0.0;
+    }
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+        or_tctx t = { d, P, s, { 0 }, dt, S };
+        or_for_all_neighbors(d, s, g, i, or_stress_task, &t);
+        double dv[3] = { t.acc[0], t.acc[1], t.acc[2] };
+        if (d->flags & SPH_AXISYMMETRIC) {
+            double rho_i = s->m[i] / s->V[i];
+            const double* q = s->sig + 6 * i;
+            double inv = 1.0 / (rho_i * or_r_axi(P, s, i));
+            dv[0] += q[3] * inv;
+            dv[1] += (q[1] - q[2]) * inv;
+        }
+        for (int k = 0; k < 3; ++k) s->acc[3 * i + k] += dv[k];
+        if (S->damping > 0.0)
+            for (int k = 0; k < 3; ++k) s->acc[3 * i + k] -= S->damping * s->v[3 * i + k];
+        s->de_int[i] = t.acc[3];
+        s->de_visc[i] = t.acc[4];
+        s->de_hg[i] = t.acc[5];
+    }
+}
+
+/* _hg_rate_accel (SOLID.py:4260-4300) on the stage velocities y (acc 0-2) */
+typedef struct { or_tctx t; const double* y; } or_hgctx;
+static void or_hg_task(void* c, int64_t i, int64_t j, int mir, const double* r) {
+    or_hgctx* h = (or_hgctx*)c;
+    or_tctx* t = &h->t;
+    const or_state* s = t->s;
+    const sph_dims* d = t->d;
+    const sph_solid_params* S = t->S;
+    double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    if (!(rn > 1e-9)) return;
+    double yj[3] = { h->y[3 * j], h->y[3 * j + 1], h->y[3 * j + 2] };
+    or_mirror(mir, yj);
+    double Gj[3][3];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) Gj[a][b] = s->grad_v[9 * j + 3 * a + b];
+    or_mirror_mat(mir, Gj);
+    double rh[3] = { -r[0] / rn, -r[1] / rn, -r[2] / rn }, dv[3];
+    for (int a = 0; a < 3; ++a) {
+        double ga = 0.0;
+        for (int b = 0; b < 3; ++b) ga += 0.5 * (s->grad_v[9 * i + 3 * a + b] + Gj[a][b]) * (-r[b]);
+        dv[a] = (yj[a] - h->y[3 * i + a]) - ga;
+    }
+    double hg = dv[0] * rh[0] + dv[1] * rh[1] + dv[2] * rh[2];
+    double Wij = oracle_wendland(d, rn);
+    const sph_material* M = or_mat(s, i);
+    double rho_i = s->m[i] / s->V[i];
+    double wj = or_w(d, t->P, s, j);
+    double damp = S->hourglass * (M->rho0 / rho_i) * (M->c_p / d->h) * (wj / S->V0)
+                  * (Wij / S->w_dx0) * hg;
+    for (int k = 0; k < 3; ++k) t->acc[k] += damp * rh[k];
+}
+
+/* M y for every particle (compute_hg_accel, SOLID.py:4302-4309) */
+static void or_hg_accel(const sph_dims* d, const sph_fluid_params* P, const sph_solid_params* S,
+                        or_state* s, const or_grid* g, const double* y, double* a) {
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < s->n; ++i) {
+        or_hgctx h = { { d, P, s, { 0 }, 0.0, S }, y };
+        or_for_all_neighbors(d, s, g, i, or_hg_task, &h);
+        for (int k = 0; k < 3; ++k) a[3 * i + k] = h.t.acc[k];
+    }
+}
+
+static double or_kinetic(const or_state* s) {
+    double ke = 0.0;
+    for (int64_t i = 0; i < s->n; ++i) {
+        const double* v = s->v + 3 * i;
+        ke += 0.5 * s->m[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    return ke;
+}
+
+/* sts_stage_count (SOLID.py:4510-4539), subcycle mode */
+static int or_sts_stage_count(const sph_solid_params* S, or_state* s) {
+    if (S->hg_rate <= 0.0) return 1;
+    double want = S->hg_rate * s->scal[SPH_S_DT];
+    int st = S->sts_stages > 0 ? S->sts_stages : (int)fmax(1.0, ceil(want / 1.6));
+    if (st > S->sts_max_stages) st = S->sts_max_stages;
+    double cap = st * 1.6 / S->hg_rate;
+    if (s->scal[SPH_S_DT] > cap) s->scal[SPH_S_DT] = cap;
+    return st;
+}
+
+/* advect_velocity, hourglass_sts, advect_velocity_finish, advect_position
+ * (SOLID.py:4541-4615) with hg_ke_snapshot/bank (2324-2349) */
+static void or_sts_advect(const sph_dims* d, const sph_fluid_params* P, const sph_solid_params* S,
+                          or_state* s, const or_grid* g, int st) {
+    int64_t n = s->n;
+    double dt = s->scal[SPH_S_DT];
+    double* vs = (double*)malloc(sizeof(double) * 3 * n);
+    double* y = (double*)malloc(sizeof(double) * 3 * n);
+    double* a = (double*)malloc(sizeof(double) * 3 * n);
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            s->v[3 * i + k] += 0.5 * dt * s->acc[3 * i + k];
+            vs[3 * i + k] = s->v[3 * i + k];
+        }
+    if (S->sts_refresh) or_velocity_gradient(d, P, S, s, g);
+    double ke0 = or_kinetic(s);
+    memcpy(y, s->v, sizeof(double) * 3 * n);
+    double sub = dt / st;
+    for (int k = 0; k < st; ++k) {
+        or_hg_accel(d, P, S, s, g, y, a);
+        for (int64_t i = 0; i < 3 * n; ++i) y[i] = 1.0 * y[i] + 0.0 * y[i] + sub * a[i];
+    }
+    memcpy(s->v, y, sizeof(double) * 3 * n);
+    s->scal[SPH_S_HG_WORK] += ke0 - or_kinetic(s);
+    double h = 0.5 * dt, w = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double* ai = s->acc + 3 * i;
+        double* v = s->v + 3 * i;
+        double dv[3] = { v[0] - vs[3 * i], v[1] - vs[3 * i + 1], v[2] - vs[3 * i + 2] };
+        w += s->m[i] * (dv[0] * h * ai[0] + dv[1] * h * ai[1] + dv[2] * h * ai[2]);
+        for (int k = 0; k < 3; ++k) v[k] += h * ai[k];
+    }
+    s->scal[SPH_S_LEFTOVER] += w;
+    for (int64_t i = 0; i < n; ++i) {
+        double dr[3] = { dt * s->v[3 * i], dt * s->v[3 * i + 1], dt * s->v[3 * i + 2] };
+        or_displace(d, P, s, i, dr);
+    }
+    free(vs); free(y); free(a);
+}
+
+/* clear_constraint_forces + apply_constraints (SOLID.py:5019-5055) */
+static void or_constraints(const sph_dims* d, const sph_fluid_params* P, const sph_solid_params* S,
+                           or_state* s) {
+    (void)P;
+    double ls = s->scal[SPH_S_LOAD_S], lsd = s->scal[SPH_S_LOAD_SDOT];
+    double scale = (d->flags & SPH_AXISYMMETRIC) ? 2.0 * M_PI : 1.0;
+    for (int k = 0; k < 6 * S->ncons; ++k) s->cons[k] = 0.0;
+    for (int64_t i = 0; i < s->n; ++i) {
+        int f = s->bc_flag[i];
+        if (!f) continue;
+        int cid = s->bc_id[i];
+        for (int k = 0; k < 3; ++k) {
+            if (!(f & (1 << k))) continue;
+            if (S->ncons > 0 && cid >= 0) {
+                s->cons[6 * cid + k] += scale * s->m[i] * s->acc[3 * i + k];
+                s->cons[6 * cid + 3 + k] = s->bc_disp[3 * i + k] * ls;
+            }
+            s->u[3 * i + k] = s->bc_disp[3 * i + k] * ls;
+            s->x[3 * i + k] = s->x0[3 * i + k] + s->u[3 * i + k];
+            s->v[3 * i + k] = s->bc_disp[3 * i + k] * lsd;
+            s->acc[3 * i + k] = 0.0;
+        }
+    }
+}
+
+/* update_load_factor (SOLID.py:5001-5017) */
+static void or_load_factor(const sph_solid_params* S, or_state* s) {
+    double t = s->scal[SPH_S_SIM_TIME];
+    if (S->ramp_time <= 0.0) { s->scal[SPH_S_LOAD_S] = 1.0; s->scal[SPH_S_LOAD_SDOT] = 0.0; return; }
+    double u = fmin(1.0, fmax(0.0, t / S->ramp_time));
+    s->scal[SPH_S_LOAD_S] = u * u * (3.0 - 2.0 * u);
+    s->scal[SPH_S_LOAD_SDOT] = 6.0 * u * (1.0 - u) / S->ramp_time;
+}
+
+/* HypoElasticSolver.step (SOLID.py:5227-5231): load factor, SPHBase.step (sort, substep,
+ * boundaries), then the clock. */
+void oracle_solid_step(const sph_dims* d, const sph_fluid_params* P, const sph_solid_params* S,
+                       or_state* s) {
+    or_load_factor(S, s);
+    oracle_sort(d, s, &g_grid);
+    or_compute_lam(d, P, s, &g_grid);
+    or_velocity_gradient(d, P, S, s, &g_grid);
+    or_update_stress(d, P, S, s, &g_grid);
+    or_internal_energy(d, P, S, s, &g_grid);
+    if (S->thermal) or_temperature(P, s);
+    or_volume(d, P, s, &g_grid);
+    or_stress_accelerations(d, P, S, s, &g_grid);
+    or_adaptive_dt(d, P, s, &g_grid, S);
+    int st = 0;
+    if (!S->sts && S->hourglass > 0.0) {           /* clamp_dt_hourglass (4158-4161) */
+        double cap = 1.6 / S->hg_rate;
+        if (s->scal[SPH_S_DT] > cap) s->scal[SPH_S_DT] = cap;
+    }
+    if (S->sts && S->hourglass > 0.0) {
+        st = or_sts_stage_count(S, s);
+        or_sts_advect(d, P, S, s, &g_grid, st);
+    } else {
+        or_advect(d, P, s);
+    }
+    s->scal[SPH_S_STS_STAGES] = st;
+    or_constraints(d, P, S, s);
+    if (P->pst_enabled) or_apply_pst(d, P, s, &g_grid);
+    or_boundaries(d, P, s);
+    s->scal[SPH_S_SIM_TIME] += s->scal[SPH_S_DT];
+}
+
+/* Single phases on the state as it stands (grid of the last sort), for unit tests. */
+void oracle_solid_velocity_gradient(const sph_dims* d, const sph_fluid_params* P,
+                                    const sph_solid_params* S, or_state* s) {
+    or_velocity_gradient(d, P, S, s, &g_grid);
+}
+void oracle_solid_stress_accelerations(const sph_dims* d, const sph_fluid_params* P,
+                                       const sph_solid_params* S, or_state* s) {
+    or_stress_accelerations(d, P, S, s, &g_grid);
+}
+void oracle_solid_hg_accel(const sph_dims* d, const sph_fluid_params* P, const sph_solid_params* S,
+                           or_state* s, const double* y, double* a) {
+    or_hg_accel(d, P, S, s, &g_grid, y, a);
+}
+void oracle_solid_update_stress(const sph_dims* d, const sph_fluid_params* P,
+                                const sph_solid_params* S, or_state* s) {
+    or_update_stress(d, P, S, s, &g_grid);
+}
+void oracle_sort_only(const sph_dims* d, or_state* s) { oracle_sort(d, s, &g_grid); }
+
+/* f64 solid diagnostics: compute_internal_energy, compute_elastic_energy (SOLID.py:2295-2322) */
+double oracle_solid_reduce(const or_state* s, int which) {
+    double out = 0.0;
+    for (int64_t i = 0; i < s->n; ++i) {
+        const sph_material* M = or_mat(s, i);
+        if (which == SPH_R_INTERNAL) out += s->m[i] * s->e_int[i];
+        else if (which == SPH_R_ELASTIC && M->G > 0.0) {
+            double V = s->V[i], rho = s->m[i] / V, th = (rho - M->rho0) / M->rho0;
+            const double* q = s->sig + 6 * i;
+            double ss = q[0] * q[0] + q[1] * q[1] + q[2] * q[2]
+                        + 2.0 * (q[3] * q[3] + q[4] * q[4] + q[5] * q[5]);
+            out += 0.5 * M->K * th * th * V + 0.25 * ss / M->G * V;
+        }
+    }
+    return out;
 }

@@ -1,0 +1,262 @@
+"""HypoElasticSolver: the hypoelastic solid step on the GPU.
+
+Same construction, configuration reading and public methods as the reference's
+`HypoElasticSolver` (SOLID.py:136-5231): `initialize()`, `step()`, `dt[None]`,
+`sim_time`, `get_constraint_data()`, `energy_budget()` and the energy accessors.
+`step()` is one call into libsigmasph.so (`sph_solid_step`).
+
+The feature set is the one the tensile and plane-strain decks use: linear and Tait
+equations of state, J2 (linear hardening) and Johnson-Cook return maps, the Jaumann
+rate, ALE transport of the deviator, the plastic strain and the internal energy under
+the shift (with the upwind energy diffusion), Morris viscosity, the rate hourglass
+damper (explicit with its dt cap, or super-time-stepped in subcycle mode), the load
+ramp and displacement grips, axisymmetry and plane strain.  Everything else the
+reference offers on this class raises NotImplementedError when a scene asks for it:
+JWL / Mie-Grueneisen / HJC, damage and erosion, the VNR and Monaghan viscosities,
+the conservative axisymmetric forms, the viscous / adaptive / saturating hourglass
+forms, RKL super-time-stepping, the velocity limiters, and a table whose materials do
+not share one EOS sound speed and one acoustic impedance.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .DSPH import FluidSettings
+from .fields import Field, Scalar
+from .materials import table as material_table
+from .sph_base import SPHBase
+
+
+def reference_kernel_sum(two_d, h, dx0):
+    """S = Sum_j W(r_j)/W(dx0) over the undeformed lattice (_reference_kernel_sum,
+    SOLID.py:1632-1657, with _wendland_scalar 1705-1719)."""
+    alpha = 7.0 / (math.pi * h * h) if two_d else 21.0 / (2.0 * math.pi * h ** 3)
+
+    def W(r):
+        q = r / h
+        return alpha * (1.0 - q) ** 4 * (4.0 * q + 1.0) if q < 1.0 else 0.0
+
+    reach = int(math.ceil(h / dx0))
+    w_ref = W(dx0)
+    if w_ref <= 0.0:
+        return 1.0, w_ref
+    total = 0.0
+    span = range(-reach, reach + 1)
+    for i in span:
+        for j in span:
+            for k in ((0,) if two_d else span):
+                r = dx0 * math.sqrt(i * i + j * j + k * k)
+                if r > 0.0:
+                    total += W(r)
+    return total / w_ref, w_ref
+
+
+class SolidSettings:
+    """The constructor half of HypoElasticSolver that reads the scene (SOLID.py:138-1400),
+    host only: the fluid-shared numerics (`fluid`, an SphFluidParams with the solid's
+    c_signal = c_p), the solid's own parameters (`solid_params()`), the material table
+    and the initial dt."""
+
+    def __init__(self, layout):
+        self.ps = ps = layout
+        c = ps.cfg.get_cfg
+        self.materials = ps.materials
+        mats = list(self.materials.values())
+        if len({round(m.c_eos, 12) for m in mats}) > 1:
+            raise NotImplementedError("materials with different EOS sound speeds (the c0_mixed "
+                                      "branch, SOLID.py:244-245) are not part of this build")
+        if len({round(m.rho0 * m.c_p, 12) for m in mats}) > 1:
+            raise NotImplementedError("materials with different acoustic impedances (the "
+                                      "hourglass impedance ratio, SOLID.py:229) are not built")
+        for name, what in (("AXI_CONSERVATIVE_X", "Domain.conservativeAxialMomentum"),
+                           ("AXI_CONSERVATIVE_R", "Domain.conservativeRadialForce"),
+                           ("PST_ALE_E_CONSERVATIVE", "Shifting.aleEnergyConservative")):
+            if c(name):
+                raise NotImplementedError("%s is not part of this build" % what)
+        if str(c("HOURGLASS_FORM") or "rate").lower() != "rate":
+            raise NotImplementedError("only the rate hourglass form is built")
+        if c("HOURGLASS_STS") and str(c("HOURGLASS_STS_MODE") or "subcycle").lower() != "subcycle":
+            raise NotImplementedError("RKL super-time-stepping is not built (subcycle is)")
+        if str(c("DELTA_PAIRS") or "samematerial").lower() != "samematerial":
+            raise NotImplementedError("Diffusion.pairs other than sameMaterial is not built")
+        self.fs = FluidSettings(ps)
+        self.default_material = ps.default_material
+        self.c_p = float(self.default_material.c_p)
+        self.fs.c_signal = self.c_p
+        self.CFL = self.fs.CFL
+        self.dt_initial = self.CFL * ps.h_min / self.c_p                 # SOLID.py:317
+        if c("timeStepSize") is not None:
+            self.dt_initial = float(c("timeStepSize"))
+        self.dx0 = 2.0 * ps.particle_radius
+        self.hourglass = float(c("HOURGLASS_ALPHA") or 0.0)
+        self.hg_sts = bool(c("HOURGLASS_STS"))
+        self.hg_sts_stages = int(c("HOURGLASS_STS_STAGES") or 0)
+        self.hg_sts_max_stages = 32
+        rf = c("HOURGLASS_STS_REFRESH")
+        self.hg_sts_refresh = True if rf is None else bool(rf)
+        self.hg_neighbour_sum, self.w_dx0 = reference_kernel_sum(ps.two_d, ps.support_radius,
+                                                                 self.dx0)
+        self.hg_rate = 0.0
+        self.dt_hg_cap = 1.0e30
+        if self.hourglass > 0.0:
+            self.hg_rate = self.hourglass * self.c_p / ps.h_min * self.hg_neighbour_sum
+            self.dt_hg_cap = 1.6 / self.hg_rate
+        self.ramp_time = float(c("rampTime") or 0.0)
+        self.damping = float(c("dampingCoefficient") or 0.0)
+        self.kernel_correction = bool(c("KERNEL_CORRECTION") or False)
+        tol = c("L_EIG_TOL")
+        self.l_eig_tol = 0.1 if tol is None else float(tol)
+        self.jaumann = str(c("OBJECTIVE_RATE") or "jaumann").lower() != "none"
+        ah = c("PST_ALE_HISTORY")
+        self.pst_ale_history = True if ah is None else bool(ah)
+        ad = c("PST_ALE_DIFFUSION")
+        self.pst_ale_diffusion = 1.0 if ad is None else float(ad)
+        mode = str(c("DT_PAIR_SOUND_SPEED") or "samematerial").lower()
+        self.dt_pair_mode = {"all": 0, "samematerial": 1, "off": 2}[mode]
+        track = c("TRACK_VISCOUS_WORK")
+        self.solid_visc_work_on = (True if track is None else bool(track)) and self.fs.viscosity > 0.0
+        self.thermal_on = any(m.cv > 0.0 for m in mats)
+        self.plastic = any(m.plastic_kind != N.PLASTIC_NONE for m in mats)
+        self.ncons = len(getattr(ps, "constraints", []) or [])
+        self.mat_table = material_table(self.materials)
+
+    def params(self):
+        P = self.fs.params()
+        P.c_signal = self.c_p
+        P.dt_pair = int(self.dt_pair_mode != 2)
+        P.track_visc_work = int(self.solid_visc_work_on)
+        return P
+
+    def solid_params(self):
+        S = N.SphSolidParams()
+        S.hourglass, S.hg_rate, S.w_dx0, S.V0 = self.hourglass, self.hg_rate, self.w_dx0, self.ps.V0
+        S.ale_diffusion, S.ramp_time, S.damping = self.pst_ale_diffusion, self.ramp_time, self.damping
+        S.l_eig_tol = self.l_eig_tol
+        S.kernel_correction = int(self.kernel_correction)
+        S.jaumann = int(self.jaumann)
+        S.ale_history = int(self.pst_ale_history)
+        S.sts = int(self.hg_sts)
+        S.sts_stages, S.sts_max_stages = self.hg_sts_stages, self.hg_sts_max_stages
+        S.sts_refresh = int(self.hg_sts_refresh)
+        S.dt_pair_mode = self.dt_pair_mode
+        S.visc_work = int(self.solid_visc_work_on)
+        S.thermal = int(self.thermal_on)
+        S.plastic = int(self.plastic)
+        S.nmat = len(self.materials)
+        S.ncons = self.ncons
+        return S
+
+    def load_factor(self, t):
+        """update_load_factor (SOLID.py:5001-5017): (s, ds/dt) at time t."""
+        if self.ramp_time <= 0.0:
+            return 1.0, 0.0
+        u = min(1.0, max(0.0, t / self.ramp_time))
+        return u * u * (3.0 - 2.0 * u), 6.0 * u * (1.0 - u) / self.ramp_time
+
+
+class HypoElasticSolver(SPHBase):
+    def __init__(self, particle_system):
+        super().__init__(particle_system)
+        ps = self.ps
+        self.settings = st = SolidSettings(ps)
+        for k, v in vars(st).items():
+            if k not in ("ps", "g", "viscosity", "fs"):
+                setattr(self, k, v)
+        self.dt[None] = st.dt_initial
+        self.dt_max = st.dt_initial
+        self.params = st.params()
+        self.sparams = st.solid_params()
+        self.sim_time_s = Scalar(ps.scal, N.SPH_S_SIM_TIME)
+        self.load_s = Scalar(ps.scal, N.SPH_S_LOAD_S)
+        self.load_sdot = Scalar(ps.scal, N.SPH_S_LOAD_SDOT)
+        self.plastic_work = Scalar(ps.scal, N.SPH_S_PLASTIC_WORK)
+        self.hg_work = Scalar(ps.scal, N.SPH_S_HG_WORK)
+        self.ie_pair = Scalar(ps.scal, N.SPH_S_IE_PAIR)
+        self.ie_ale = Scalar(ps.scal, N.SPH_S_IE_ALE)
+        self.ie_ale_adv = Scalar(ps.scal, N.SPH_S_IE_ALE_ADV)
+        self.ie_ale_diff = Scalar(ps.scal, N.SPH_S_IE_ALE_DIFF)
+        self.visc_work = Scalar(ps.scal, N.SPH_S_VISC_WORK)
+        self.diff_work = Scalar(ps.scal, N.SPH_S_DIFF_WORK)
+        self.leftover_work = Scalar(ps.scal, N.SPH_S_LEFTOVER)
+        self.hg_sts_last_s = Scalar(ps.scal, N.SPH_S_STS_STAGES, dtype=int)
+        self.constraints_list = list(getattr(ps, "constraints", []) or [])
+        self.num_constraints = len(self.constraints_list)
+        self._mat_dev = torch.frombuffer(bytearray(bytes(st.mat_table)), dtype=torch.uint8).to(ps.device)
+        ps.solid.mat = self._mat_dev.data_ptr()
+        self.sigma_dev = ps.sigma_dev
+        self.eps_plastic = ps.eps_plastic
+        self.e_int = ps.e_int
+        self.temperature = ps.temperature
+
+    # ----------------------------------------------------------------------------------
+    @property
+    def sim_time(self):
+        return float(self.sim_time_s[None])
+
+    def initialize(self):
+        self.ps.initialize_particle_system()
+
+    def step(self):
+        """update_load_factor, SPHBase.step, sim_time += dt (SOLID.py:5227-5231)."""
+        ps = self.ps
+        N.check(ps._lib.sph_solid_step(ps.ctx, ctypes.byref(ps.state), ctypes.byref(ps.solid),
+                                       ctypes.byref(self.params), ctypes.byref(self.sparams),
+                                       ps.stream()), ps.ctx, "sph_solid_step")
+
+    def get_constraint_data(self):
+        """The grips' latest force and displacement (SOLID.py:5198-5225)."""
+        if self.num_constraints == 0:
+            return []
+        d = self.ps.cons.cpu().numpy()
+        return [{"index": k, "name": c.get("name") or "constraint_%d" % k,
+                 "displacement": d[k, 3:6].copy(), "force": d[k, 0:3].copy()}
+                for k, c in enumerate(self.constraints_list)]
+
+    # ---- energy budget (SOLID.py:2295-2645) -----------------------------------------
+    def _solid_reduce(self, which):
+        ps = self.ps
+        rc = ps._lib.sph_solid_reduce(ps.ctx, ctypes.byref(ps.state), ctypes.byref(ps.solid),
+                                      which, self._out.data_ptr(), ps.stream())
+        N.check(rc, ps.ctx, "sph_solid_reduce")
+        return float(self._out[0].item())
+
+    def compute_internal_energy(self):
+        return self._solid_reduce(N.SPH_R_INTERNAL)
+
+    def compute_elastic_energy(self):
+        return self._solid_reduce(N.SPH_R_ELASTIC)
+
+    def compute_hourglass_energy(self):
+        return float(self.hg_work[None])
+
+    def compute_plastic_work(self):
+        return float(self.plastic_work[None])
+
+    def compute_leftover_energy(self):
+        return float(self.leftover_work[None])
+
+    def compute_leftover_raw_energy(self):
+        return float(self.leftover_work[None])
+
+    def compute_viscous_energy(self):
+        return float(self.visc_work[None])
+
+    def compute_diffusion_work(self):
+        return float(self.diff_work[None])
+
+    def energy_budget(self):
+        ke = self.compute_kinetic_energy()
+        ie = self.compute_internal_energy()
+        hg = self.compute_hourglass_energy()
+        pe = self.compute_potential_energy()
+        lw = self.compute_leftover_energy()
+        return {"ke": ke, "ie": ie, "hg": hg, "pe": pe, "se": self.compute_elastic_energy(),
+                "pw": self.compute_plastic_work(), "ie_pair": float(self.ie_pair[None]),
+                "ie_ale": float(self.ie_ale[None]), "ie_ale_adv": float(self.ie_ale_adv[None]),
+                "ie_ale_diff": float(self.ie_ale_diff[None]),
+                "wall": self.compute_wall_energy(), "visc": self.compute_viscous_energy(),
+                "leftover": lw, "leftover_raw": lw, "wd": self.compute_diffusion_work(),
+                "total": ke + ie + hg + pe - lw}

@@ -34,6 +34,17 @@ SPH_S_MAX_V = 7
 SPH_S_OFF_GRID_NUM = 8
 SPH_S_OFF_GRID_MASS = 9
 SPH_S_OUT_OF_BOX = 10
+SPH_S_SIM_TIME = 11
+SPH_S_LOAD_S = 12
+SPH_S_LOAD_SDOT = 13
+SPH_S_PLASTIC_WORK = 14
+SPH_S_HG_WORK = 15
+SPH_S_IE_PAIR = 16
+SPH_S_IE_ALE = 17
+SPH_S_STS_STAGES = 18
+SPH_S_KE_PRE = 19
+SPH_S_IE_ALE_ADV = 20
+SPH_S_IE_ALE_DIFF = 21
 SPH_NSCAL = 32
 # reductions
 SPH_R_KINETIC = 0
@@ -43,6 +54,11 @@ SPH_R_MOMENTUM = 3
 SPH_R_GROSS_MOMENTUM = 4
 SPH_R_TAIT_ENERGY = 5
 SPH_R_KINETIC_OFFGRID = 6
+SPH_R_INTERNAL = 7
+SPH_R_ELASTIC = 8
+# material kinds
+EOS_LINEAR, EOS_TAIT = 0, 1
+PLASTIC_NONE, PLASTIC_LINEAR, PLASTIC_JC = 0, 1, 2
 
 _d3 = ctypes.c_double * 3
 _i3 = ctypes.c_int32 * 3
@@ -121,12 +137,47 @@ class SphFluidParams(ctypes.Structure):
     ]
 
 
+class SphMaterial(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "rho0", "K", "G", "c_p", "c0", "sigma_y0", "hardening", "cv", "T0", "e0",
+        "eos_S", "eos_gamma", "delta",
+        "jc_A", "jc_B", "jc_n", "jc_C", "jc_eps0_dot", "jc_T0", "jc_Tm", "jc_m")] + [
+        ("eos_kind", ctypes.c_int32), ("plastic_kind", ctypes.c_int32)]
+
+
+class SphSolidParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "hourglass", "hg_rate", "w_dx0", "V0", "ale_diffusion", "ramp_time", "damping",
+        "l_eig_tol")] + [(n, ctypes.c_int32) for n in (
+        "kernel_correction", "jaumann", "ale_history", "sts", "sts_stages", "sts_max_stages",
+        "sts_refresh", "dt_pair_mode", "visc_work", "thermal", "plastic", "nmat", "ncons",
+        "pad_")]
+
+
+_vp2 = ctypes.c_void_p * 2
+
+
+class SphSolid(ctypes.Structure):
+    _fields_ = [
+        ("sig", _vp2), ("eps_p", _vp2), ("e_int", _vp2),
+        ("hcur", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("T", _vp2), ("de_int", _vp2), ("eos_id", _vp2), ("bc_flag", _vp2),
+        ("bc_id", _vp2), ("bc_disp", _vp2),
+        ("grad_v", ctypes.c_void_p),
+        ("hg_y", _vp2),
+        ("v_sync", ctypes.c_void_p),
+        ("mat", ctypes.c_void_p),
+        ("cons", ctypes.c_void_p),
+    ]
+
+
 #: every symbol include/sigmasph.h declares
 EXPORTS = (
     "sph_ctx_create", "sph_ctx_destroy", "sph_last_error", "sph_version",
     "sph_sort", "sph_neighbor_pairs",
     "sph_fluid_step", "sph_fluid_pass_volume", "sph_fluid_pass_force",
     "sph_fluid_advect", "sph_fluid_pass_shift", "sph_fluid_finish",
+    "sph_solid_sort", "sph_solid_step", "sph_solid_reduce",
     "sph_reduce", "sph_timing", "sph_timing_read",
 )
 
@@ -162,6 +213,13 @@ def lib():
         f = getattr(L, name)
         f.restype = ctypes.c_int
         f.argtypes = [vp, P(SphState), P(SphFluidParams), vp]
+    L.sph_solid_sort.restype = ctypes.c_int
+    L.sph_solid_sort.argtypes = [vp, P(SphState), P(SphSolid), vp]
+    L.sph_solid_step.restype = ctypes.c_int
+    L.sph_solid_step.argtypes = [vp, P(SphState), P(SphSolid), P(SphFluidParams),
+                                 P(SphSolidParams), vp]
+    L.sph_solid_reduce.restype = ctypes.c_int
+    L.sph_solid_reduce.argtypes = [vp, P(SphState), P(SphSolid), ctypes.c_int32, vp, vp]
     L.sph_reduce.restype = ctypes.c_int
     L.sph_reduce.argtypes = [vp, P(SphState), ctypes.c_int32, vp, P(SphFluidParams), vp, vp]
     L.sph_timing.restype = ctypes.c_int

@@ -7,9 +7,11 @@ applied on top, `Materials[]`, and `FluidBlocks` / `SolidBlocks`.  Everything do
 reads flat names through `get_cfg(name)`.  Unknown keys are errors with a suggestion, as
 in the reference (params.py:1479-1488).
 
-Only the parameters of the delta-plus-SPH particle step are registered here (the subset
-of params.py:253-1316 that the step reads); `Domain.periodic` is an addition of this
-build (periodic boxes do not exist in the reference).
+Only the parameters of the particle step are registered here (the subset of
+params.py:253-1316 that the fluid and solid steps read); `Domain.periodic` is an
+addition of this build (periodic boxes do not exist in the reference).  A sub-card whose
+mere presence switches a feature on (`Hourglass.superTimeStepping`) sets its flag as in
+params.py:1326-1332.
 """
 import difflib
 import json
@@ -61,12 +63,16 @@ PARAMS = [
     P("DOMAIN_BOUNDARY", "Domain", "boundary", "str", "open", choices=("open", "reflect")),
     P("DOMAIN_RESTITUTION", "Domain", "restitution", "float", 0.5),
     P("PERIODIC", "Domain", "periodic", "str", ""),
+    P("AXI_CONSERVATIVE_X", "Domain", "conservativeAxialMomentum", "bool", False),
+    P("AXI_CONSERVATIVE_R", "Domain", "conservativeRadialForce", "bool", False),
     # Time (params.py:394-417)
     P("CFL", "Time", "CFL", "float", required=True),
     P("timeStepSize", "Time", "stepSize", "float", None),
     P("sim_duration", "Time", "duration", "float", None),
     P("maxSteps", "Time", "maxSteps", "int", None),
     P("numberOfStepsPerRenderUpdate", "Time", "stepsPerRenderUpdate", "int", None),
+    P("DT_PAIR_SOUND_SPEED", "Time", "pairSoundSpeed", "str", "samematerial",
+      choices=("all", "samematerial", "off")),
     # Solver (params.py:418-495)
     P("simulationMethod", "Solver", "type", "str", required=True,
       choices=("deltaPlusSPH", "hypoElastic")),
@@ -76,8 +82,13 @@ PARAMS = [
     P("COMPENSATED_VOLUME", "Solver", "compensatedVolume", "bool", True),
     P("stiffness", "Solver", "stiffness", "float", 50000.0),
     P("TRACK_VISCOUS_WORK", "Solver", "trackViscousWork", "bool", True),
+    P("dampingCoefficient", "Solver", "dampingCoefficient", "float", 0.0),
+    P("OBJECTIVE_RATE", "Solver", "objectiveRate", "str", "jaumann", choices=("jaumann", "none")),
+    P("KERNEL_CORRECTION", "Solver", "kernelCorrection", "bool", False),
+    P("L_EIG_TOL", "Solver", "lEigTol", "float", 0.1),
     # Load (params.py:977)
     P("gravitation", "Load", "gravitation", "vec3", required=True),
+    P("rampTime", "Load", "rampTime", "float", 0.0),
     # Diffusion (params.py:986-1013)
     P("DELTA_SPH", "Diffusion", "delta", "float", 0.0),
     P("DELTA_PAIRS", "Diffusion", "pairs", "str", "samematerial"),
@@ -100,13 +111,28 @@ PARAMS = [
     P("PST_M_DELTA", "Shifting", "mDelta", "float", 0.4),
     P("PST_ALE_MOMENTUM", "Shifting", "aleMomentum", "bool", True),
     P("L_EIG_TOL_LAM", "Shifting", "lEigTolLam", "float", 0.02),
+    P("PST_ALE_HISTORY", "Shifting", "aleHistory", "bool", True),
+    P("PST_ALE_DIFFUSION", "Shifting", "aleDiffusion", "float", 1.0),
+    P("PST_ALE_E_CONSERVATIVE", "Shifting", "aleEnergyConservative", "bool", False),
+    # Hourglass (params.py:1133-1192)
+    P("HOURGLASS_ALPHA", "Hourglass", "alpha", "float", 0.0),
+    P("HOURGLASS_FORM", "Hourglass", "form", "str", "rate",
+      choices=("rate", "epj_viscous", "mm_viscous")),
+    P("HOURGLASS_STS_MODE", "Hourglass.superTimeStepping", "mode", "str", "subcycle",
+      choices=("subcycle", "rkl")),
+    P("HOURGLASS_STS_STAGES", "Hourglass.superTimeStepping", "stages", "int", 0),
+    P("HOURGLASS_STS_REFRESH", "Hourglass.superTimeStepping", "refreshGradV", "bool", True),
+    P("HOURGLASS_STS_SAFETY", "Hourglass.superTimeStepping", "safety", "float", 0.8),
     # legacy flat material constants (derived from Materials[] when it is given)
     P("density0", "Materials", "density0", "float", 1000.0),
     P("c0", "Materials", "c0", "float", None),
     P("exponent", "Materials", "exponent", "float", 7.0),
 ]
 BY_NAME = {p.name: p for p in PARAMS}
-CARDS = ("Domain", "Time", "Solver", "Load", "Diffusion", "Shifting")
+CARDS = ("Domain", "Time", "Solver", "Load", "Diffusion", "Shifting", "Hourglass")
+#: sub-cards whose presence switches a feature on (params.py:1326-1332)
+PRESENCE = {"Hourglass.superTimeStepping": "HOURGLASS_STS"}
+CONSTRAINT_KEYS = {"name", "region", "components", "displacement"}
 BLOCK_KEYS = {"objectId", "start", "end", "translation", "scale", "velocity", "density",
               "color", "rotVelocity", "divVelocity", "shape", "material",
               # build-side initial conditions (no reference counterpart)
@@ -209,10 +235,28 @@ class SimConfig:
                     errors.append("card %s: expected an object" % section)
                     continue
                 fields = {p.field: p for p in PARAMS if p.card == section}
+                subs = {c.split(".", 1)[1]: c for c in PRESENCE if c.split(".")[0] == section}
                 for key, value in body.items():
+                    if key in subs:
+                        sub = subs[key]
+                        if not isinstance(value, dict):
+                            errors.append("card %s: %s must be an object" % (sub, key))
+                            continue
+                        given[PRESENCE[sub]] = True
+                        sfields = {p.field: p for p in PARAMS if p.card == sub}
+                        for sk, sv in value.items():
+                            sp = sfields.get(sk)
+                            if sp is None:
+                                errors.append("card %s: unknown key %r.%s" % (sub, sk, suggest(sk, sfields)))
+                                continue
+                            v = _coerce(sp, sv, "%s.%s" % (sub, sk), errors)
+                            if v is not None:
+                                given[sp.name] = v
+                        continue
                     p = fields.get(key)
                     if p is None:
-                        errors.append("card %s: unknown key %r.%s" % (section, key, suggest(key, fields)))
+                        errors.append("card %s: unknown key %r.%s" % (section, key,
+                                                                      suggest(key, list(fields) + list(subs))))
                         continue
                     v = _coerce(p, value, "%s.%s" % (section, key), errors)
                     if v is not None:
@@ -229,7 +273,12 @@ class SimConfig:
             v = _coerce(p, value, "Configuration.%s" % key, errors)
             if v is not None:
                 given[key] = v
-        derived = self._materials(raw.get("Materials"), errors)
+        self.materials = {}
+        method = given.get("simulationMethod")
+        if method == "hypoElastic":
+            derived = self._solid_materials(raw.get("Materials"), given.get("DELTA_SPH", 0.0), errors)
+        else:
+            derived = self._materials(raw.get("Materials"), errors)
         resolved = {}
         shifting_present = "Shifting" in raw or "PST_ENABLED" in flat
         deferred = []
@@ -244,6 +293,8 @@ class SimConfig:
                 resolved[p.name] = p.default
         for p in deferred:
             resolved[p.name] = p.default(resolved)
+        for flag in PRESENCE.values():
+            resolved[flag] = bool(given.get(flag, False))
         resolved.update(derived)
         for p in PARAMS:
             if p.required and resolved.get(p.name) is None:
@@ -256,6 +307,18 @@ class SimConfig:
                 for req in ("objectId", "start", "end"):
                     if req not in b:
                         errors.append("%s[%d]: %s is required" % (key, k, req))
+        for k, cst in enumerate(raw.get("Constraints") or []):
+            if not isinstance(cst, dict):
+                errors.append("Constraints[%d]: expected an object" % k)
+                continue
+            for ck in cst:
+                if ck not in CONSTRAINT_KEYS:
+                    errors.append("Constraints[%d]: unknown key %r.%s" % (k, ck, suggest(ck, CONSTRAINT_KEYS)))
+            reg = cst.get("region")
+            if not (isinstance(reg, list) and len(reg) == 2 and all(len(r) == 3 for r in reg)):
+                errors.append("Constraints[%d]: region must be [[x0, y0, z0], [x1, y1, z1]]" % k)
+            if set(str(cst.get("components", "xyz")).lower()) - set("xyz"):
+                errors.append("Constraints[%d]: components takes the letters x, y, z" % k)
         if resolved.get("PERIODIC"):
             if set(resolved["PERIODIC"].lower()) - set("xyz"):
                 errors.append("Domain.periodic: only the letters x, y, z are allowed")
@@ -304,6 +367,28 @@ class SimConfig:
         out = {"c0": max(c0s), "density0": max(rhos)}
         out["exponent"] = gam if len(mats) == 1 else 1.0
         return out
+
+    def _solid_materials(self, mats, default_delta, errors):
+        """The solid deck's materials (materials.py:148-449) and the two scene-wide
+        scalars derive_globals takes from them (materials.py:551-604): c0 is the fastest
+        EOS sound speed, density0 the largest reference density."""
+        from .materials import resolve, MaterialError
+        try:
+            found = resolve(mats or [], default_delta)
+        except MaterialError as e:
+            errors.append(str(e))
+            return {}
+        if not found:
+            errors.append("Materials: a hypoElastic deck declares its materials")
+            return {}
+        self.materials = found
+        return {"c0": max(m.c_eos for m in found.values()),
+                "density0": max(m.rho0 for m in found.values()),
+                "exponent": 1.0}
+
+    def get_materials(self):
+        """name -> materials.Material, for a hypoElastic deck (config_builder.py:1340)."""
+        return self.materials
 
     def get_cfg(self, name, enforce_exist=False):
         c = self.config["Configuration"]

@@ -1613,3 +1613,22 @@ extern "C" int sph_reduce(sph_ctx* c, const sph_state* st, int32_t which, const 
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, hs, c->s, c->nblk, c->pstride, out);
     return check_launch(c, "reduce");
 }
+
+// ---- hypoelastic solid step: entry points (kernels in progress) ---------------------
+extern "C" int sph_solid_sort(sph_ctx* c, sph_state* st, sph_solid* sd, void* stream) {
+    (void)st; (void)sd; (void)stream;
+    if (!c) return SPH_EINVAL;
+    return set_err(c, SPH_EINVAL, "the solid step is not built in this library%s", "", 0);
+}
+extern "C" int sph_solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const sph_fluid_params* P,
+                              const sph_solid_params* S, void* stream) {
+    (void)st; (void)sd; (void)P; (void)S; (void)stream;
+    if (!c) return SPH_EINVAL;
+    return set_err(c, SPH_EINVAL, "the solid step is not built in this library%s", "", 0);
+}
+extern "C" int sph_solid_reduce(sph_ctx* c, const sph_state* st, const sph_solid* sd, int32_t which,
+                                double* out, void* stream) {
+    (void)st; (void)sd; (void)which; (void)out; (void)stream;
+    if (!c) return SPH_EINVAL;
+    return set_err(c, SPH_EINVAL, "the solid step is not built in this library%s", "", 0);
+}

@@ -34,6 +34,36 @@ def lattice_points(lower, size, diameter, dim=3):
     return grid.reshape(dim, -1).T
 
 
+def carve(x, shape):
+    """Keep the lattice points inside a block shape.  One shape is built:
+    `gaugeDogbone`, a round bar with a parallel gauge in the (x, r) half-plane, centred
+    on x = 0: gauge of diameter `gaugeDiameter` and length `gaugeLength`, circular
+    fillets of radius `filletRadius` tangent to the gauge, heads of diameter
+    `headDiameter` and length `headLength`.  (The reference's own dogbone shapes,
+    geometry.py:17-371, have no parallel gauge.)"""
+    kind = str(shape.get("type", "")).lower()
+    if kind != "gaugedogbone":
+        raise NotImplementedError("block shape %r is not part of this build (gaugeDogbone is)"
+                                  % shape.get("type"))
+    Rh = 0.5 * float(shape["headDiameter"])
+    Rg = 0.5 * float(shape["gaugeDiameter"])
+    Lg = float(shape["gaugeLength"])
+    Lh = float(shape["headLength"])
+    Rf = float(shape["filletRadius"])
+    if not (0.0 < Rg < Rh and Rf >= Rh - Rg):
+        raise ValueError("gaugeDogbone: need 0 < gauge < head and filletRadius >= the step")
+    xf = math.sqrt(Rf * Rf - (Rf - (Rh - Rg)) ** 2)
+    ax = np.abs(x[:, 0])
+    r = np.full(len(x), -1.0)
+    r[ax <= 0.5 * Lg] = Rg
+    fil = (ax > 0.5 * Lg) & (ax <= 0.5 * Lg + xf)
+    r[fil] = Rg + Rf - np.sqrt(Rf * Rf - (ax[fil] - 0.5 * Lg) ** 2)
+    head = (ax > 0.5 * Lg + xf) & (ax <= 0.5 * Lg + xf + Lh)
+    r[head] = Rh
+    keep = (x[:, 1] >= 0.0) & (x[:, 1] <= r)
+    return x[keep]
+
+
 class SceneLayout:
     """Host-side part of ParticleSystem.__init__ (particle_system.py:25-592): the mode
     flags, the particle lattice and the cell lattice, in float64, without a GPU."""
@@ -106,7 +136,12 @@ class SceneLayout:
                   [(b, self.material_solid) for b in self.cfg.get_solid_blocks()])
         if c("ParticlesFile"):
             raise NotImplementedError("ParticlesFile input is not part of this build")
-        xs, vs, rhos, objs = [], [], [], []
+        self.materials = self.cfg.get_materials() if self.is_solid else {}
+        if self.is_solid and not self.materials:
+            raise SystemExit("a hypoElastic deck needs Materials[]")
+        self.default_material = (max(self.materials.values(), key=lambda m: m.c_p)
+                                 if self.materials else None)
+        xs, vs, rhos, objs, mids = [], [], [], [], []
         for blk, _mat in blocks:
             x, v, rho = self._lay_block(blk)
             if len(x) == 0:
@@ -115,6 +150,12 @@ class SceneLayout:
             self.object_collection[blk["objectId"]] = blk
             xs.append(x); vs.append(v); rhos.append(rho)
             objs.append(np.full(len(x), int(blk["objectId"]), dtype=np.int32))
+            if self.is_solid:
+                name = blk.get("material")
+                if name is not None and name not in self.materials:
+                    raise SystemExit("block %s names unknown material %r" % (blk["objectId"], name))
+                mat = self.materials[name] if name is not None else self.default_material
+                mids.append(np.full(len(x), mat.index, dtype=np.int32))
         if not xs:
             raise SystemExit("the scene lays no particles")
         x = np.concatenate(xs)
@@ -139,6 +180,36 @@ class SceneLayout:
                     x[:, k] = x[:, k] - L * np.floor((x[:, k] - self.domain_start[k]) / L)
 
         self._init_arrays(x, v, rho, obj)
+        if self.is_solid:
+            self._init_solid(x, np.concatenate(mids))
+
+    def _init_solid(self, x, eos_id):
+        """The solid's initial per-particle state: material row, e = e0 and T = T0 of
+        that material (SOLID.py:507-540), and the grips (_apply_constraint_regions,
+        particle_system.py:875-911; selected on the laid float64 positions)."""
+        n = len(x)
+        rows = sorted(self.materials.values(), key=lambda m: m.index)
+        self.eos_init = eos_id
+        self.e_init = np.array([rows[k].e0 for k in eos_id], dtype=np.float64)
+        self.T_init = np.array([rows[k].T0 for k in eos_id], dtype=np.float64)
+        flag = np.zeros(n, dtype=np.int32)
+        disp = np.zeros((n, 3))
+        cid = np.full(n, -1, dtype=np.int32)
+        self.constraints = self.cfg.get_constraints()
+        for k, c in enumerate(self.constraints):
+            lo = np.array(c["region"][0], dtype=np.float64)
+            hi = np.array(c["region"][1], dtype=np.float64)
+            sel = np.all((x >= lo) & (x <= hi), axis=1)
+            bits = 0
+            for i, axis in enumerate("xyz"):
+                if axis in str(c.get("components", "xyz")).lower():
+                    bits |= 1 << i
+            flag[sel] |= bits
+            disp[sel] = np.array(c.get("displacement", [0.0, 0.0, 0.0]), dtype=np.float64)
+            cid[sel] = k
+        if self.constraints and not flag.any():
+            raise SystemExit("Constraints were given but no particle falls inside any region")
+        self.bc_flag_init, self.bc_disp_init, self.bc_id_init = flag, disp, cid
 
     def _init_arrays(self, x, v, rho, obj):
         n = len(x)
@@ -163,7 +234,7 @@ class SceneLayout:
         dx = self.particle_diameter
         x = lattice_points(start, size, dx, 3)
         if blk.get("shape"):
-            raise NotImplementedError("block shapes are not part of this build")
+            x = carve(x, blk["shape"])
         n = len(x)
         rho = np.full(n, float(blk.get("density") or self.cfg.get_cfg("density0") or 1000.0))
         v = np.zeros((n, 3))
@@ -274,9 +345,14 @@ class SceneLayout:
         """The initial particles as float64 numpy arrays in pid order."""
         n = len(self.x_init)
         z3 = np.zeros((n, 3))
-        return {"pid": np.arange(n, dtype=np.int32), "obj": self.obj_init, "x": self.x_init.copy(),
+        snap = {"pid": np.arange(n, dtype=np.int32), "obj": self.obj_init, "x": self.x_init.copy(),
                 "x0": self.x_init.copy(), "u": z3.copy(), "v": self.v_init.copy(), "shift": z3.copy(),
                 "V": self.V_init.copy(), "m": self.m_init.copy(), "p": np.zeros(n)}
+        if self.is_solid:
+            snap.update({"eos_id": self.eos_init.copy(), "e_int": self.e_init.copy(),
+                         "T": self.T_init.copy(), "bc_flag": self.bc_flag_init.copy(),
+                         "bc_id": self.bc_id_init.copy(), "bc_disp": self.bc_disp_init.copy()})
+        return snap
 
 
 class ParticleSystem(SceneLayout):

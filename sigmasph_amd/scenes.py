@@ -95,3 +95,117 @@ def free_block(n=(10, 10, 10), dx=0.01, jitter=0.05, seed=0, velocity=(0.0, 0.0,
         for k, v in extra.items():
             sc.setdefault(k, {}).update(v) if isinstance(v, dict) else sc.__setitem__(k, v)
     return sc
+
+
+# ---- solids (units: mm, ms, GPa, kg; velocities in mm/ms = m/s) ----------------------------
+def _steel(jc=True, yield_stress=0.31, C=0.02, eps0_dot=0.001, E=193.0, nu=0.30):
+    if jc:
+        return {"name": "steel", "library": "stainless_steel",
+                "strength": {"youngsModulus": E, "poissonRatio": nu,
+                             "plasticity": {"C": C, "eps0_dot": eps0_dot}}}
+    return {"name": "steel", "density0": 7.9e-06, "specificHeat": 500.0, "eos": {"type": "linear"},
+            "strength": {"type": "hypoElastic", "youngsModulus": E, "poissonRatio": nu,
+                         "plasticity": {"model": "linear", "yieldStress": yield_stress,
+                                        "hardeningModulus": 1.0}}}
+
+
+def solid_block(n=(20, 8, 1), dx=0.1, two_d="planeStrain", jitter=0.0, seed=0,
+                velocity=(0.0, 0.0, 0.0), div_velocity=None, rot_velocity=None,
+                hourglass=4.0, sts=True, sts_stages=0, pst=True, ale_history=True,
+                viscosity=0.02, delta=0.1, cfl=0.3, jc=True, yield_stress=0.31, grips=None,
+                ramp_time=0.5, kernel_correction=False, damping=0.0, extra=None,
+                material=None, factor=6.0, y0=None):
+    """A solid bar for tests: a single plane-strain or axisymmetric layer (`two_d`) or a
+    3-D block (`two_d=None`), steel (Johnson-Cook or linear J2 hardening), Colle shifting
+    with ALE history transport, the rate hourglass damper (super-time-stepped when
+    `sts`), optional grips: `grips = D` pulls the outer `max(2, n0//6)` columns of each
+    end apart by +-D (components "xy") over `ramp_time`."""
+    lo = [-0.5 * n[0] * dx + 0.5 * dx, 0.5 * dx if y0 is None else y0, 0.5 * dx]
+    hi = [lo[0] + n[0] * dx - 0.5 * dx, lo[1] + n[1] * dx - 0.5 * dx, lo[2] + n[2] * dx - 0.5 * dx]
+    if two_d:
+        lo[2], hi[2] = 0.0, 0.25 * dx
+    dom = {"start": [-1.0, 0.0, 0.0], "end": [1.0, 1.0, 1.0], "particleRadius": 0.5 * dx,
+           "supportRadiusFactor": factor}
+    if two_d:
+        dom[two_d] = True
+    blk = {"objectId": 0, "start": lo, "end": [hi[k] + 0.25 * dx for k in range(3)],
+           "density": 7.9e-06, "material": "steel", "jitter": jitter, "seed": seed,
+           "velocity": list(velocity)}
+    if div_velocity is not None:
+        blk["divVelocity"] = list(div_velocity)
+    if rot_velocity is not None:
+        blk["rotVelocity"] = [0.0, 0.0, float(rot_velocity)]
+    sc = {
+        "Domain": dom,
+        "Time": {"CFL": cfl},
+        "Solver": {"type": "hypoElastic", "viscosity": viscosity, "allowNegativePressure": True,
+                   "kernelCorrection": bool(kernel_correction), "dampingCoefficient": damping},
+        "Materials": [material or _steel(jc, yield_stress)],
+        "Load": {"gravitation": [0.0, 0.0, 0.0], "rampTime": ramp_time},
+        "Diffusion": {"delta": delta},
+        "Shifting": {"enabled": bool(pst), "mode": "colle", "maMode": "fixed", "maFixed": 0.05,
+                     "R": 0.2, "n": 4, "maxShift": 0.05, "aleMomentum": True,
+                     "aleHistory": bool(ale_history)},
+        "Hourglass": {"alpha": hourglass},
+        "SolidBlocks": [blk],
+    }
+    if sts:
+        sc["Hourglass"]["superTimeStepping"] = {"mode": "subcycle", "stages": sts_stages}
+    if grips:
+        w = max(2, n[0] // 6) * dx
+        zlo, zhi = -1.0, 1.0
+        sc["Constraints"] = [
+            {"name": "left", "region": [[lo[0] - dx, -1.0, zlo], [lo[0] + w - 0.5 * dx, 1.0, zhi]],
+             "components": "xy", "displacement": [-grips, 0.0, 0.0]},
+            {"name": "right", "region": [[hi[0] - w + 0.5 * dx, -1.0, zlo], [hi[0] + dx, 1.0, zhi]],
+             "components": "xy", "displacement": [grips, 0.0, 0.0]},
+        ]
+    if extra:
+        for k, v in extra.items():
+            sc.setdefault(k, {}).update(v) if isinstance(v, dict) else sc.__setitem__(k, v)
+    return sc
+
+
+def dogbone_c2(dx0=0.1, elongation=3.0, rate=1.0, cfl=0.3):
+    """The axisymmetric round-bar tension test: (x, r) half-plane at dx0 (innermost ring
+    at r = dx0/2), 10 mm heads 15 mm long, 6 mm gauge 30 mm long, 5 mm fillets; steel
+    rho 7.9e-6, K = 160, G = 77 GPa, Johnson-Cook A 0.31, B 1.0, n 0.65, C 0.07,
+    eps0_dot = 1/ms; numerics of the reference's tensile_axi_dogbone deck (support 6
+    particle radii, CFL 0.3, viscosity 0.02, delta 0.1, Colle shifting with fixed
+    Ma 0.05, hourglass alpha 4 super-time-stepped).  The grips (the heads beyond the
+    fillets) move apart by `elongation` mm in total; the load ramp is sized so that the
+    mean elongation rate is `rate` mm/ms."""
+    K, G = 160.0, 77.0
+    E = 9.0 * K * G / (3.0 * K + G)
+    nu = (3.0 * K - 2.0 * G) / (2.0 * (3.0 * K + G))
+    Lg, Lh, Rf, Dh, Dg = 30.0, 15.0, 5.0, 10.0, 6.0
+    xf = math.sqrt(Rf * Rf - (Rf - 0.5 * (Dh - Dg)) ** 2)
+    half = 0.5 * Lg + xf + Lh
+    start = [-half + 0.5 * dx0, 0.5 * dx0, 0.0]
+    end = [half, 0.5 * Dh + 0.25 * dx0, 0.25 * dx0]
+    grip_in = 0.5 * Lg + xf + 2.0
+    mat = {"name": "specimen", "library": "stainless_steel",
+           "strength": {"youngsModulus": E, "poissonRatio": nu,
+                        "plasticity": {"C": 0.07, "eps0_dot": 1.0}}}
+    return {
+        "Domain": {"start": [-50.0, 0.0, 0.0], "end": [50.0, 20.0, 1.0], "particleRadius": 0.5 * dx0,
+                   "supportRadiusFactor": 6.0, "axisymmetric": True},
+        "Time": {"CFL": cfl},
+        "Solver": {"type": "hypoElastic", "viscosity": 0.02, "allowNegativePressure": True},
+        "Materials": [mat],
+        "Load": {"gravitation": [0.0, 0.0, 0.0], "rampTime": elongation / rate},
+        "Diffusion": {"delta": 0.1},
+        "Shifting": {"enabled": True, "mode": "colle", "maMode": "fixed", "maFixed": 0.05,
+                     "R": 0.2, "n": 4, "maxShift": 0.05, "aleMomentum": True, "aleHistory": True},
+        "Hourglass": {"alpha": 4.0, "superTimeStepping": {"mode": "subcycle"}},
+        "SolidBlocks": [{"objectId": 0, "start": start, "end": end, "density": 7.9e-06,
+                         "material": "specimen",
+                         "shape": {"type": "gaugeDogbone", "headDiameter": Dh, "headLength": Lh,
+                                   "gaugeDiameter": Dg, "gaugeLength": Lg, "filletRadius": Rf}}],
+        "Constraints": [
+            {"name": "left", "region": [[-half - 1.0, -1.0, -1.0], [-grip_in, 20.0, 1.0]],
+             "components": "xy", "displacement": [-0.5 * elongation, 0.0, 0.0]},
+            {"name": "right", "region": [[grip_in, -1.0, -1.0], [half + 1.0, 20.0, 1.0]],
+             "components": "xy", "displacement": [0.5 * elongation, 0.0, 0.0]},
+        ],
+    }

@@ -63,3 +63,22 @@ def brute_pairs(x, h, mirror_y=False, mirror_z=False, period=None):
 
 def pair_set(pairs):
     return set(map(tuple, np.asarray(pairs).tolist()))
+
+
+def solid_layout(scene, precision="fp64"):
+    from sigmasph_amd.SOLID import SolidSettings
+    lay = SceneLayout(SimConfig(scene), precision)
+    st = SolidSettings(lay)
+    return lay, st, st.params(), st.solid_params()
+
+
+def solid_oracle_state(lay, st):
+    """Oracle state of a solid layout: particles, materials, grips, e = e0, T = T0."""
+    snap = lay.snapshot()
+    s = O.OracleState(len(snap["x"]))
+    s.make_solid(st.mat_table, st.ncons)
+    for k, v in snap.items():
+        cur = getattr(s, k)
+        setattr(s, k, np.ascontiguousarray(np.asarray(v, dtype=cur.dtype).reshape(cur.shape)))
+    s.scal[0] = st.dt_initial
+    return s

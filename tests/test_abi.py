@@ -37,6 +37,8 @@ def test_null_arguments_rejected():
     assert L.sph_ctx_destroy(None) == N.SPH_OK
     assert L.sph_sort(None, None, None) == N.SPH_EINVAL
     assert L.sph_fluid_step(None, None, None, None) == N.SPH_EINVAL
+    assert L.sph_solid_step(None, None, None, None, None, None) == N.SPH_EINVAL
+    assert L.sph_solid_sort(None, None, None, None) == N.SPH_EINVAL
 
 
 STRUCTS = {
@@ -46,6 +48,9 @@ STRUCTS = {
     "sph_state": (N.SphState, ["buf", "cur", "acc", "lam", "L6", "lam_grad", "div", "de_visc", "V_new",
                                "rho", "rho_new", "scal"]),
     "sph_fluid_params": (N.SphFluidParams, [f[0] for f in N.SphFluidParams._fields_]),
+    "sph_material": (N.SphMaterial, [f[0] for f in N.SphMaterial._fields_]),
+    "sph_solid_params": (N.SphSolidParams, [f[0] for f in N.SphSolidParams._fields_]),
+    "sph_solid": (N.SphSolid, [f[0] for f in N.SphSolid._fields_]),
 }
 
 

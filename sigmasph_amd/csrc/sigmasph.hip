@@ -70,7 +70,11 @@ struct DScratch {
 };
 
 enum { PART_WDIFF = 0, PART_LEFT = 1, PART_VISC = 2, PART_WALL = 3, PART_R0 = 4, PART_R1 = 5,
-       PART_R2 = 6, SPH_NPART = 7 };
+       PART_R2 = 6,
+       // the solid step's partials
+       PART_PLASTIC = 7, PART_IEPAIR = 8, PART_IEADV = 9, PART_IEDIFF = 10, PART_IEALE = 11,
+       PART_HGW = 12, PART_KE = 13, PART_KE2 = 14, PART_CROSS = 15, PART_VISCS = 16,
+       SPH_NPART = 17 };
 enum { MAX_C2 = 0, MAX_ACC = 1, MAX_V = 2 };
 
 // ------------------------------------------------------------------------------------
@@ -224,7 +228,7 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_rank(int n, DScratch s, const int
 
 struct PermField { const void* src; void* dst; int bytes; };
 struct PermTable {
-    PermField f[16];
+    PermField f[24];
     int nf;
     int real_bytes;                        // 8: fp64, 4: mixed
     const void *m, *V, *V_lo;              // source mass / volume (V_lo: mixed only)
@@ -249,7 +253,17 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_permute(int n, PermTable T, DScra
         case 4: ((int*)F.dst)[k] = ((const int*)F.src)[src]; break;
         case 8: ((double*)F.dst)[k] = ((const double*)F.src)[src]; break;
         case 16: ((float4*)F.dst)[k] = ((const float4*)F.src)[src]; break;
+        case 24: {   // a float stress row
+            const float2* a = (const float2*)F.src + 3 * (size_t)src;
+            float2* d = (float2*)F.dst + 3 * (size_t)k;
+            d[0] = a[0]; d[1] = a[1]; d[2] = a[2];
+        } break;
         case 32: ((double4*)F.dst)[k] = ((const double4*)F.src)[src]; break;
+        case 48: {   // a double stress row
+            const double2* a = (const double2*)F.src + 3 * (size_t)src;
+            double2* d = (double2*)F.dst + 3 * (size_t)k;
+            d[0] = a[0]; d[1] = a[1]; d[2] = a[2];
+        } break;
         default: break;
         }
     }
@@ -531,6 +545,7 @@ template <typename R> struct PassArgs {
     DScratch s;
     const double4* X;
     int pstride;     // stride of one partial-sum slot
+    const int32_t* bcf;   // solid: constrained components per particle (null on a fluid)
     __device__ bool any_out() const { return st.scal[SPH_S_OUT_OF_BOX] > 0.5; }
 };
 
@@ -940,6 +955,7 @@ template <typename R> struct ShiftBody {
             }
         }
         if (F.flags & SPH_TWO_D) drz = 0.0;
+        if (a.bcf && a.bcf[i] != 0) drx = dry = drz = 0.0;   // grips are driven (DSPH.py:927-933)
         typename R4t<R>::type o; o.x = (R)drx; o.y = (R)dry; o.z = (R)drz; o.w = 0;
         b.shift[i] = o;
     }
@@ -996,11 +1012,16 @@ struct PairsBody {
 // ------------------------------------------------------------------------------------
 // apply_pst second pass (DSPH.py:1045-1049) and the boundary kernels (sph_base.py:622-655)
 // ------------------------------------------------------------------------------------
+// `sig` (solid only): the live deviator, reflected with the particle (M s M,
+// sph_base.py:329, 358, 367); the solid also measures the step's leftover here, after the
+// grips and the boundaries have set the final acceleration (banked one step later, as
+// SOLID.py:2222-2236 reads it).
 template <typename R>
-__global__ __launch_bounds__(SPH_BLOCK) void k_finish(DGrid g, DFluid F, DState<R> st, DScratch s, int pstride) {
+__global__ __launch_bounds__(SPH_BLOCK) void k_finish(DGrid g, DFluid F, DState<R> st, DScratch s, int pstride,
+                                                      R* sig) {
     __shared__ double red[SPH_BLOCK / SPH_WAVE];
     int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
-    double ww = 0.0;
+    double ww = 0.0, wl = 0.0;
     if (i < st.n) {
         const DBuf<R>& b = st.b;
         double4 x = b.x[i], u = b.u[i], x0 = b.x0[i], v = b.v[i];
@@ -1025,6 +1046,11 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_finish(DGrid g, DFluid F, DState<
             xs[k] = -xs[k];
             if (F.disp_pos) us[k] = -us[k] - 2.0 * x0s[k];
             vs[k] = -vs[k]; as[k] = -as[k]; a_changed = true;
+            if (sig) {   // order xx, yy, xy, zz, xz, yz
+                R* q = sig + 6 * (size_t)i;
+                if (k == 1) { q[2] = -q[2]; q[5] = -q[5]; }
+                else        { q[4] = -q[4]; q[5] = -q[5]; }
+            }
         };
         if (axi) {
             vs[2] = 0.0; as[2] = 0.0; a_changed = true;
@@ -1065,10 +1091,21 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_finish(DGrid g, DFluid F, DState<
         if (a_changed) {
             typename R4t<R>::type ao; ao.x = (R)as[0]; ao.y = (R)as[1]; ao.z = (R)as[2]; ao.w = 0;
             st.acc[i] = ao;
+            as[0] = (double)ao.x; as[1] = (double)ao.y; as[2] = (double)ao.z;
+        }
+        if (sig) {
+            double dt = st.scal[SPH_S_DT];
+            double fx = as[0] - F.g[0], fy = as[1] - F.g[1], fz = as[2] - F.g[2];
+            double g2 = F.g[0] * F.g[0] + F.g[1] * F.g[1] + F.g[2] * F.g[2];
+            wl = 0.5 * (double)b.m[i] * dt * dt * (fx * fx + fy * fy + fz * fz - g2);
         }
     }
     double bs = block_sum(ww, red);
     if (threadIdx.x == 0) s.part[PART_WALL * pstride + blockIdx.x] = bs;
+    if (sig) {
+        double bl = block_sum(wl, red);
+        if (threadIdx.x == 0) s.part[PART_LEFT * pstride + blockIdx.x] = bl;
+    }
 }
 
 __global__ __launch_bounds__(1024) void k_bank(double* scal, DScratch s, int nblk, int pstride, int ma_global) {
@@ -1166,7 +1203,11 @@ struct sph_ctx {
     int* slow_cnt;        // [1]
     std::string err;
     bool timing = false;
-    TimerKind tk[4];
+    TimerKind tk[7];
+    // solid step scratch, allocated at the first solid call
+    double* cons_part = nullptr;   // [ncons * 3][nblk] grip force partials
+    int cons_cap = 0;
+    int* host_stages = nullptr;    // pinned: the hourglass sub-step count read back per step
 };
 
 static int set_err(sph_ctx* c, int code, const char* fmt, const char* a = "", long long b = 0) {
@@ -1310,6 +1351,8 @@ extern "C" int sph_ctx_destroy(sph_ctx* c) {
     void* ptrs[] = {s.key, s.cellw, s.cellid, s.count, s.start, s.cursor, s.bsum, s.tmp, s.perm, s.part, s.maxv,
                     c->slow_idx, c->slow_cnt};
     for (void* p : ptrs) if (p) (void)hipFree(p);
+    if (c->cons_part) (void)hipFree(c->cons_part);
+    if (c->host_stages) (void)hipHostFree(c->host_stages);
     for (auto& t : c->tk) for (auto ev : t.ev) (void)hipEventDestroy(ev);
     delete c;
     return SPH_OK;
@@ -1329,7 +1372,7 @@ extern "C" int sph_timing(sph_ctx* c, int32_t enable) {
 }
 
 extern "C" int sph_timing_read(sph_ctx* c, int32_t which, double* avg_ms, int64_t* launches) {
-    if (!c || which < 0 || which > 3 || !avg_ms || !launches) return SPH_EINVAL;
+    if (!c || which < 0 || which > 6 || !avg_ms || !launches) return SPH_EINVAL;
     TimerKind& t = c->tk[which];
     double tot = 0.0;
     for (int k = 0; k < t.used; ++k) {
@@ -1477,6 +1520,7 @@ template <typename R> static PassArgs<R> pass_args(sph_ctx* c, sph_state* st, co
     a.s = c->s;
     a.X = (const double4*)st->buf[st->cur].x;
     a.pstride = c->pstride;
+    a.bcf = nullptr;
     return a;
 }
 
@@ -1512,7 +1556,7 @@ template <typename R> static int pass_shift(sph_ctx* c, sph_state* st, const DFl
 
 template <typename R> static int finish(sph_ctx* c, sph_state* st, const DFluid& F, hipStream_t hs) {
     hipLaunchKernelGGL((k_finish<R>), dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st), c->s,
-                       c->pstride);
+                       c->pstride, (R*)nullptr);
     hipLaunchKernelGGL(k_bank, dim3(1), dim3(1024), 0, hs, st->scal, c->s, c->nblk, c->pstride,
                        F.ma_mode == 1 ? 1 : 0);
     return check_launch(c, "finish");
@@ -1614,21 +1658,4 @@ extern "C" int sph_reduce(sph_ctx* c, const sph_state* st, int32_t which, const 
     return check_launch(c, "reduce");
 }
 
-// ---- hypoelastic solid step: entry points (kernels in progress) ---------------------
-extern "C" int sph_solid_sort(sph_ctx* c, sph_state* st, sph_solid* sd, void* stream) {
-    (void)st; (void)sd; (void)stream;
-    if (!c) return SPH_EINVAL;
-    return set_err(c, SPH_EINVAL, "the solid step is not built in this library%s", "", 0);
-}
-extern "C" int sph_solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const sph_fluid_params* P,
-                              const sph_solid_params* S, void* stream) {
-    (void)st; (void)sd; (void)P; (void)S; (void)stream;
-    if (!c) return SPH_EINVAL;
-    return set_err(c, SPH_EINVAL, "the solid step is not built in this library%s", "", 0);
-}
-extern "C" int sph_solid_reduce(sph_ctx* c, const sph_state* st, const sph_solid* sd, int32_t which,
-                                double* out, void* stream) {
-    (void)st; (void)sd; (void)which; (void)out; (void)stream;
-    if (!c) return SPH_EINVAL;
-    return set_err(c, SPH_EINVAL, "the solid step is not built in this library%s", "", 0);
-}
+#include "sph_solid.h"

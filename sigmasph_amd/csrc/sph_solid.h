@@ -1,0 +1,1347 @@
+// sph_solid.h -- the hypoelastic solid step (included once, at the end of sigmasph.hip).
+//
+// Kernels per step, in launch order:
+//   k_solid_begin   load factor from the clock; banks last step's deferred works
+//   sort            + the solid's sorted fields
+//   pass SA         StressBody: E_i, grad v, continuity, ALE transport of s, eps_p, e
+//                   per pair; lambda, L, grad v, the Jaumann update and return map, the
+//                   internal energy and temperature, the volume and the EOS per particle
+//   pass SB         SolidForceBody: <grad lambda>, the stress divergence with its work
+//                   conjugate, Morris, the explicit hourglass damper, ALE momentum and
+//                   the pair sound speed per pair; accelerations per particle
+//   k_dt_solid      dt, its hourglass cap or the super-time-step count; banks SA's works
+//   advect          plain kick-drift, or half kick / [grad v pass] / s hourglass
+//                   sub-steps (HgBody) / half kick + drift
+//   k_constraints   grips; pass C (shifting) and k_finish as for the fluid
+//   k_bank_solid    wall, leftover, damper and grip-force totals; the clock
+// References: SOLID.py:5060-5231 (substep order), the tasks named at each body.
+//
+// Layouts: sig [n][6] = xx, yy, xy, zz, xz, yz; grad_v [n][10] = G00 G01 G10 G11 G22
+// G02 G12 G20 G21 (pad): the in-plane block first, so a single-layer deck reads a
+// contiguous head of each row.
+
+// deferred banks (scalar slots past the public ones)
+#define SPH_S_PEND_LEFT 22
+#define SPH_S_PEND_VISC 23
+#define SPH_S_PEND_HG   24
+#define SPH_S_KE_POST   25
+#define SPH_MAX_CONS    16
+
+// scene constants of the solid step, folded on the host
+struct DSolidP {
+    double alpha_hg, hg_rate, inv_wdx0, inv_V0, ale_diff, ramp_time, damping, l_eig_tol;
+    double hg_k;     // alpha / h: the damper's prefactor before rho0/rho c_p w_j/V0 W/W0
+    int kc, jaumann, transport, transport_ep, sts, sts_stages, sts_max, sts_refresh;
+    int dt_pair_mode, visc_work, thermal, plastic, ncons, hg_explicit;
+};
+
+template <typename R> struct DSolid {
+    const R* sig_in; R* sig_out;     // history buffers: live, updated
+    const R* ep_in; R* ep_out;
+    const R* e_in; R* e_out;
+    R* T; R* de_int;
+    const int32_t* eos; const int32_t* bcf; const int32_t* bcid;
+    const typename R4t<R>::type* bcd;
+    R* gv;                            // [n][10]
+    double4* y0; double4* y1; double4* vs;
+    const sph_material* mat;
+    double* cons;
+};
+
+template <typename R> struct SolidArgs {
+    DFluid F;
+    DSolidP S;
+    DState<R> st;
+    DSolid<R> sd;
+    DScratch s;
+    const double4* X;
+    int pstride;
+    // hourglass sub-step pass only
+    const double4* ysrc; double4* ydst; int stage;
+    __device__ bool any_out() const { return st.scal[SPH_S_OUT_OF_BOX] > 0.5; }
+};
+
+template <typename R> struct Sym6 { R xx, yy, xy, zz, xz, yz; };
+
+template <typename R> __device__ __forceinline__ Sym6<R> load_sig(const R* p, int i, bool two_d) {
+    Sym6<R> q;
+    const R* r = p + 6 * (size_t)i;
+    q.xx = r[0]; q.yy = r[1]; q.xy = r[2]; q.zz = r[3];
+    if (two_d) { q.xz = 0; q.yz = 0; }
+    else { q.xz = r[4]; q.yz = r[5]; }
+    return q;
+}
+// M s M with M = diag(1, sy, sz)
+template <typename R> __device__ __forceinline__ void mirror_sig(Sym6<R>& q, int mir) {
+    if (mir & 1) { q.xy = -q.xy; q.yz = -q.yz; }
+    if (mir & 2) { q.xz = -q.xz; q.yz = -q.yz; }
+}
+
+// full 3x3 grad v from the stored row
+template <typename R> __device__ __forceinline__ void load_gv(const R* p, int i, bool two_d, R G[3][3]) {
+    const R* r = p + 10 * (size_t)i;
+    G[0][0] = r[0]; G[0][1] = r[1]; G[1][0] = r[2]; G[1][1] = r[3]; G[2][2] = r[4];
+    if (two_d) { G[0][2] = G[1][2] = G[2][0] = G[2][1] = 0; }
+    else { G[0][2] = r[5]; G[1][2] = r[6]; G[2][0] = r[7]; G[2][1] = r[8]; }
+}
+template <typename R> __device__ __forceinline__ void mirror_gv(R G[3][3], int mir) {
+    R m[3] = {(R)1, (mir & 1) ? (R)-1 : (R)1, (mir & 2) ? (R)-1 : (R)1};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) G[a][b] *= m[a] * m[b];
+}
+
+__device__ __forceinline__ double solid_eos(const sph_material& M, double rho, int allow_neg) {
+    // eos_pressure_i for the linear / Tait rows (SOLID.py:3009-3038)
+    double p = M.eos_kind == SPH_EOS_LINEAR ? M.eos_S * (rho / M.rho0 - 1.0)
+                                            : M.eos_S * (pow(rho / M.rho0, M.eos_gamma) - 1.0);
+    if (!allow_neg) p = fmax(p, 0.0);
+    return p;
+}
+
+// tait_specific_energy (EOS001_tait.py:102-121) with the clamp of cold_specific_energy
+__device__ __forceinline__ double cold_energy(const sph_material& M, double rho, int allow_neg) {
+    double x = rho / M.rho0, e;
+    if (fabs(M.eos_gamma - 1.0) < 1e-6) e = (M.eos_S / M.rho0) * (log(x) + 1.0 / x - 1.0);
+    else e = (M.eos_S / M.rho0) * ((pow(x, M.eos_gamma - 1.0) - 1.0) / (M.eos_gamma - 1.0) + 1.0 / x - 1.0);
+    if (!allow_neg && rho < M.rho0) e = 0.0;
+    return e;
+}
+
+// _jc_residual (MAT002_johnson_cook.py:251-277)
+__device__ __forceinline__ double jc_residual(double d, double vm, double G, const sph_material& M,
+                                              double m_temp, double eps_p, double dt, double* dg) {
+    double et = eps_p + d;
+    double ss = M.jc_A, dss = 0.0;
+    if (et > 0.0 && M.jc_B > 0.0 && M.jc_n > 0.0) {
+        ss += M.jc_B * pow(et, M.jc_n);
+        if (et > 1e-12) dss = M.jc_n * M.jc_B * pow(et, M.jc_n - 1.0);
+    }
+    double rate = 1.0;
+    if (dt > 0.0 && M.jc_eps0_dot > 0.0) rate = (d / dt) / M.jc_eps0_dot;
+    double mr = 1.0, dmr = 0.0;
+    if (rate > 1.0 && M.jc_C > 0.0) { mr = 1.0 + M.jc_C * log(rate); dmr = M.jc_C / fmax(1e-12, d); }
+    *dg = -3.0 * G - (dss * mr + ss * dmr) * m_temp;
+    return vm - 3.0 * G * d - ss * mr * m_temp;
+}
+
+// The return maps on the trial deviator (in f64): j2_radial_return
+// (MAT001_linear_plasticity.py:65-81) and jc_radial_return (MAT002_johnson_cook.py:280-354,
+// bracketed Newton-bisection, at most 40 iterations).  Returns d_eps_p; scales q.
+__device__ double return_map(const sph_material& M, double q[6], double eps_p, double temp, double dt) {
+    double ss = q[0] * q[0] + q[1] * q[1] + q[3] * q[3] + 2.0 * (q[2] * q[2] + q[4] * q[4] + q[5] * q[5]);
+    double vm = sqrt(1.5 * ss);
+    double sc = 1.0, d = 0.0;
+    if (M.plastic_kind == SPH_PLASTIC_LINEAR) {
+        double sy = M.sigma_y0, H = M.hardening, G = M.G;
+        if (sy > 0.0) {
+            double sig_y = sy + H * eps_p;
+            if (vm > sig_y && (3.0 * G + H) > 0.0) {
+                d = (vm - sig_y) / (3.0 * G + H);
+                sc = (sig_y + H * d) / vm;
+            }
+        }
+    } else if (M.plastic_kind == SPH_PLASTIC_JC) {
+        double G = M.G;
+        if (G > 0.0 && M.jc_A > 0.0) {
+            double m_temp = 1.0;
+            if (M.jc_Tm > M.jc_T0 && temp > M.jc_T0) {
+                double ts = fmin(1.0, fmax(0.0, (temp - M.jc_T0) / (M.jc_Tm - M.jc_T0)));
+                m_temp = fmax(0.0, 1.0 - pow(ts, M.jc_m));
+            }
+            double s0 = M.jc_A;
+            if (eps_p > 0.0 && M.jc_B > 0.0 && M.jc_n > 0.0) s0 += M.jc_B * pow(eps_p, M.jc_n);
+            double sy0 = s0 * m_temp;
+            if (vm > sy0 && sy0 > 0.0) {
+                double lo = 0.0, hi = (vm - sy0) / (3.0 * G), dg;
+                double d_ref = 0.0;
+                if (dt > 0.0 && M.jc_eps0_dot > 0.0 && M.jc_C > 0.0) d_ref = dt * M.jc_eps0_dot;
+                if (d_ref > 0.0 && d_ref < hi) {
+                    double g = jc_residual(d_ref, vm, G, M, m_temp, eps_p, dt, &dg);
+                    if (g > 0.0) lo = d_ref; else hi = d_ref;
+                }
+                d = lo > 0.0 ? lo : hi;
+                for (int it = 0; it < 40; ++it) {
+                    double g = jc_residual(d, vm, G, M, m_temp, eps_p, dt, &dg);
+                    if (g > 0.0) lo = d; else hi = d;
+                    if (fabs(g) <= 1e-6 * vm || hi - lo <= 1e-6 * hi) break;
+                    double dn = d - g / dg;
+                    if (!(dn > lo && dn < hi)) dn = lo > 0.0 ? sqrt(lo * hi) : 0.5 * hi;
+                    if (fabs(dn - d) <= 1e-6 * d) { d = dn; break; }
+                    d = dn;
+                }
+                sc = fmax(0.0, vm - 3.0 * G * d) / vm;
+            }
+        }
+    }
+    if (d > 0.0)
+        for (int k = 0; k < 6; ++k) q[k] *= sc;
+    return d;
+}
+
+// ------------------------------------------------------------------------------------
+// pass SA
+// ------------------------------------------------------------------------------------
+template <typename R> struct StressBody {
+    typedef SolidArgs<R> Args;
+    const Args& a;
+    double w_diff, w_plastic, w_iepair, w_ieadv, w_iediff, w_ieale;
+    double4 vi4;
+    double xiy;
+    R rho_i, inv_rho_i, duix, duiy, duiz, inv_dt, epi, ei, kdelta;
+    int eos_i;
+    Sym6<R> si;
+    R e00, e11, e22, e01, e02, e12, rate, drate;
+    R G[3][3];
+    R as_xx, as_yy, as_xy, as_zz, as_xz, as_yz, a_ep, a_eadv, a_ediff;
+
+    __device__ StressBody(const Args& a_)
+        : a(a_), w_diff(0), w_plastic(0), w_iepair(0), w_ieadv(0), w_iediff(0), w_ieale(0) {}
+
+    __device__ void begin(int i, double& x, double& y, double& z) {
+        const DBuf<R>& b = a.st.b;
+        const bool two_d = (a.F.flags & SPH_TWO_D) != 0;
+        double4 xi = b.x[i];
+        x = xi.x; y = xi.y; z = xi.z;
+        xiy = xi.y;
+        vi4 = b.v[i];
+        rho_i = a.st.rho[i];
+        inv_rho_i = frcp<R>(rho_i);
+        inv_dt = (R)(1.0 / a.st.scal[SPH_S_DT]);
+        duix = duiy = duiz = 0;
+        if (a.F.pst_en) {
+            typename R4t<R>::type sh = b.shift[i];
+            duix = sh.x * inv_dt; duiy = sh.y * inv_dt; duiz = sh.z * inv_dt;
+        }
+        eos_i = a.sd.eos[i];
+        {
+            // same-material pairs only: pair_delta (SOLID.py:4102-4114) of two equal rows
+            double di = a.sd.mat[eos_i].delta, den = di + di;
+            double delta = den > 1e-12 ? 2.0 * di * di / den : 0.0;
+            kdelta = (R)(delta * a.F.h * a.F.c0);
+        }
+        si = load_sig(a.sd.sig_in, i, two_d);
+        epi = a.sd.ep_in[i];
+        ei = a.sd.e_in[i];
+        e00 = e11 = e22 = e01 = e02 = e12 = 0;
+        rate = drate = 0;
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) G[p][q] = 0;
+        as_xx = as_yy = as_xy = as_zz = as_xz = as_yz = 0;
+        a_ep = a_eadv = a_ediff = 0;
+    }
+
+    __device__ void pair(int j, int mir, double rxd, double ryd, double rzd) {
+        const DFluid& F = a.F;
+        const DSolidP& S = a.S;
+        const DBuf<R>& b = a.st.b;
+        const bool two_d = (F.flags & SPH_TWO_D) != 0;
+        R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
+        R r2 = rx * rx + ry * ry + rz * rz;
+        R rn = fsqrt<R>(r2);
+        R c = gradw_c<R>(F, rn);
+        R wj = wq<R>(F, b, b.V, j);
+        // E_i (compute_L_task, DSPH.py:563-589)
+        R sc = -wj * c;
+        e00 += sc * rx * rx; e11 += sc * ry * ry; e01 += sc * rx * ry;
+        if (!two_d) { e22 += sc * rz * rz; e02 += sc * rx * rz; e12 += sc * ry * rz; }
+        R gx = c * rx, gy = c * ry, gz = c * rz;
+        R rho_j = a.st.rho[j];
+        double4 vj4 = b.v[j];
+        if (mir & 1) vj4.y = -vj4.y;
+        if (mir & 2) vj4.z = -vj4.z;
+        R dvx = (R)(vi4.x - vj4.x), dvy = (R)(vi4.y - vj4.y), dvz = (R)(vi4.z - vj4.z);
+        // grad v (compute_grad_v_task, SOLID.py:1721-1731): G += w_j (v_j - v_i) (x) grad W
+        {
+            R ux = -wj * dvx, uy = -wj * dvy;
+            G[0][0] += ux * gx; G[0][1] += ux * gy; G[1][0] += uy * gx; G[1][1] += uy * gy;
+            if (!two_d) {
+                R uz = -wj * dvz;
+                G[0][2] += ux * gz; G[1][2] += uy * gz;
+                G[2][0] += uz * gx; G[2][1] += uz * gy; G[2][2] += uz * gz;
+            }
+        }
+        const bool same = a.sd.eos[j] == eos_i;
+        R djx = 0, djy = 0, djz = 0;
+        if (F.pst_en) {
+            typename R4t<R>::type sh = b.shift[j];
+            djx = sh.x * inv_dt; djy = sh.y * inv_dt; djz = sh.z * inv_dt;
+            if (mir & 1) djy = -djy;
+            if (mir & 2) djz = -djz;
+            // continuity in ALE form (compute_depsvol_dt_task, DSPH.py:1240-1262)
+            R cx = dvx + duix - djx, cy = dvy + duiy - djy, cz = dvz + duiz - djz;
+            R fl = (rho_j * djx - rho_i * duix) * gx + (rho_j * djy - rho_i * duiy) * gy +
+                   (rho_j * djz - rho_i * duiz) * gz;
+            rate += wj * fl * inv_rho_i;
+            rate += wj * (cx * gx + cy * gy + cz * gz);
+        } else {
+            rate += wj * (dvx * gx + dvy * gy + dvz * gz);
+        }
+        if (same && kdelta > (R)0) {
+            // delta-SPH, gated on the pair sharing a material (DSPH.py:1279-1300)
+            R pg = (R)2 * (rho_j - rho_i) * (-c * r2) * frcp<R>(r2 + (R)F.eta2);
+            R dr = kdelta * wj * pg * inv_rho_i;
+            rate += dr;
+            drate += dr;
+        }
+        if (S.transport) {
+            // ALE transport of s, eps_p and e (SOLID.py:1799-2045): the conservative
+            // difference form reduces to Sum_j A_j (phi_j - phi_i) with
+            // A_j = w_j rho_j (du_j . grad W) / rho_i
+            R A = wj * rho_j * (djx * gx + djy * gy + djz * gz) * inv_rho_i;
+            Sym6<R> sj = load_sig(a.sd.sig_in, j, two_d);
+            mirror_sig(sj, mir);
+            as_xx += A * (sj.xx - si.xx); as_yy += A * (sj.yy - si.yy);
+            as_xy += A * (sj.xy - si.xy); as_zz += A * (sj.zz - si.zz);
+            if (!two_d) { as_xz += A * (sj.xz - si.xz); as_yz += A * (sj.yz - si.yz); }
+            if (S.transport_ep) a_ep += A * (a.sd.ep_in[j] - epi);
+            R ej = a.sd.e_in[j];
+            a_eadv += A * (ej - ei);
+            if (S.ale_diff > 0.0 && same) {
+                // upwind-equivalent diffusion: xi dx0/2 |du_ij| w_j psi_e . grad W rho_bar / rho_i
+                R pg = (R)2 * (ej - ei) * (-c * r2) * frcp<R>(r2 + (R)F.eta2);
+                R sx = duix + djx, sy = duiy + djy, sz = duiz + djz;
+                R speed = (R)0.5 * fsqrt<R>(sx * sx + sy * sy + sz * sz);
+                R rho_bar = (R)0.5 * (rho_i + rho_j);
+                a_ediff += (R)(S.ale_diff * 0.5 * F.dx0) * speed * wj * pg * rho_bar * inv_rho_i;
+            }
+        }
+    }
+
+    __device__ void end(int i) {
+        const DFluid& F = a.F;
+        const DSolidP& S = a.S;
+        const DBuf<R>& b = a.st.b;
+        const DState<R>& st = a.st;
+        const DSolid<R>& sd = a.sd;
+        const bool two_d = (F.flags & SPH_TWO_D) != 0;
+        const sph_material& M = sd.mat[eos_i];
+        const double dt = st.scal[SPH_S_DT];
+        // ---- lambda and L (DSPH.py:639-689)
+        double L6[6], lam;
+        if (two_d) {
+            double aa = e00, bb = e01, d = e11;
+            double det = sqrt((aa - d) * (aa - d) + 4.0 * bb * bb);
+            lam = fmin(1.0, fmax(0.0, 0.5 * (aa + d - det)));
+            spd_inverse2(e00, e01, e01, e11, F.tol_lam, L6);
+        } else {
+            double A6[6] = {(double)e00, (double)e11, (double)e22, (double)e01, (double)e02, (double)e12};
+            double me;
+            spd_inverse3(A6, F.tol_lam, L6, &me);
+            lam = fmin(1.0, fmax(0.0, me));
+        }
+        st.lam[i] = (R)lam;
+        R* Lo = st.L6 + 8 * (size_t)i;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) Lo[k] = (R)L6[k];
+        // ---- grad v (SOLID.py:1743-1794)
+        double Gd[3][3];
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) Gd[p][q] = (double)G[p][q];
+        if (S.kc) {
+            double K6[6], me;
+            if (two_d) {
+                spd_inverse2(e00, e01, e01, e11, S.l_eig_tol, K6);
+                K6[2] = 1.0; K6[4] = K6[5] = 0.0;
+            } else {
+                double A6[6] = {(double)e00, (double)e11, (double)e22, (double)e01, (double)e02, (double)e12};
+                spd_inverse3(A6, S.l_eig_tol, K6, &me);
+            }
+            double K[3][3] = {{K6[0], K6[3], K6[4]}, {K6[3], K6[1], K6[5]}, {K6[4], K6[5], K6[2]}};
+            double H[3][3];
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) H[p][q] = Gd[p][0] * K[0][q] + Gd[p][1] * K[1][q] + Gd[p][2] * K[2][q];
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) Gd[p][q] = H[p][q];
+        }
+        if (two_d)
+            for (int k = 0; k < 3; ++k) Gd[2][k] = Gd[k][2] = 0.0;
+        if (F.flags & SPH_AXISYMMETRIC) Gd[2][2] = vi4.y / fmax(xiy, F.axi_rmin);
+        {
+            R* go = sd.gv + 10 * (size_t)i;
+            go[0] = (R)Gd[0][0]; go[1] = (R)Gd[0][1]; go[2] = (R)Gd[1][0]; go[3] = (R)Gd[1][1];
+            go[4] = (R)Gd[2][2]; go[5] = (R)Gd[0][2]; go[6] = (R)Gd[1][2]; go[7] = (R)Gd[2][0];
+            go[8] = (R)Gd[2][1]; go[9] = 0;
+        }
+        // ---- update_stress (SOLID.py:2691-3004)
+        double Sm[3][3] = {{(double)si.xx, (double)si.xy, (double)si.xz},
+                           {(double)si.xy, (double)si.yy, (double)si.yz},
+                           {(double)si.xz, (double)si.yz, (double)si.zz}};
+        double ed[3][3], sp[3][3];
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) {
+                ed[p][q] = 0.5 * (Gd[p][q] + Gd[q][p]);
+                sp[p][q] = 0.5 * (Gd[p][q] - Gd[q][p]);
+            }
+        double tr = ed[0][0] + ed[1][1] + ed[2][2];
+        double sn[3][3];
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) {
+                double sdot = 2.0 * M.G * (ed[p][q] - (p == q ? tr / 3.0 : 0.0));
+                if (S.jaumann) {
+                    double ws = sp[p][0] * Sm[0][q] + sp[p][1] * Sm[1][q] + sp[p][2] * Sm[2][q];
+                    double sw = Sm[p][0] * sp[0][q] + Sm[p][1] * sp[1][q] + Sm[p][2] * sp[2][q];
+                    sdot += ws - sw;
+                }
+                sn[p][q] = sdot;
+            }
+        if (S.transport) {
+            sn[0][0] += (double)as_xx; sn[1][1] += (double)as_yy; sn[2][2] += (double)as_zz;
+            sn[0][1] += (double)as_xy; sn[1][0] += (double)as_xy;
+            sn[0][2] += (double)as_xz; sn[2][0] += (double)as_xz;
+            sn[1][2] += (double)as_yz; sn[2][1] += (double)as_yz;
+        }
+        double q6[6];   // xx, yy, xy, zz, xz, yz of the re-projected trial deviator
+        {
+            double t[3][3];
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) t[p][q] = Sm[p][q] + dt * sn[p][q];
+            double u[3][3];
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) u[p][q] = 0.5 * (t[p][q] + t[q][p]);
+            double tt = (u[0][0] + u[1][1] + u[2][2]) / 3.0;
+            q6[0] = u[0][0] - tt; q6[1] = u[1][1] - tt; q6[2] = u[0][1]; q6[3] = u[2][2] - tt;
+            q6[4] = u[0][2]; q6[5] = u[1][2];
+        }
+        if (M.G <= 0.0)
+            for (int k = 0; k < 6; ++k) q6[k] = 0.0;
+        double ep = (double)epi;
+        if (S.transport_ep) ep = fmax(0.0, ep + dt * (double)a_ep);
+        double Ti = (double)sd.T[i];
+        if (S.plastic) {
+            double dep = return_map(M, q6, ep, Ti, dt);
+            ep += dep;
+            if (dep > 0.0) {
+                double ss = q6[0] * q6[0] + q6[1] * q6[1] + q6[3] * q6[3] +
+                            2.0 * (q6[2] * q6[2] + q6[4] * q6[4] + q6[5] * q6[5]);
+                w_plastic += (double)b.V[i] * sqrt(1.5 * ss) * dep;
+            }
+        }
+        {
+            R* so = sd.sig_out + 6 * (size_t)i;
+            so[0] = (R)q6[0]; so[1] = (R)q6[1]; so[2] = (R)q6[2]; so[3] = (R)q6[3];
+            so[4] = (R)q6[4]; so[5] = (R)q6[5];
+        }
+        sd.ep_out[i] = (R)ep;
+        // ---- update_internal_energy (SOLID.py:2071-2237) with last step's de_int
+        const double m = (double)b.m[i];
+        double de = (double)sd.de_int[i];
+        w_iepair += dt * m * de;
+        if (S.transport) {
+            double adv = (double)a_eadv, dif = (double)a_ediff, ale = adv + dif;
+            de += ale;
+            w_ieadv += dt * m * adv;
+            w_iediff += dt * m * dif;
+            w_ieale += dt * m * ale;
+        }
+        double e_new = (double)ei + dt * de;
+        sd.e_out[i] = (R)e_new;
+        // ---- update_temperature (SOLID.py:2239-2293) on the volume before its update
+        double Vfull_old = (double)b.V[i] + (sizeof(R) == 4 ? (double)b.V_lo[i] : 0.0);
+        if (S.thermal && M.cv > 0.0) {
+            double e_th = e_new - cold_energy(M, m / Vfull_old, F.allow_neg) - M.e0;
+            sd.T[i] = (R)(M.T0 + e_th / M.cv);
+        }
+        // ---- compute_volume_evolution (DSPH.py:1432-1495)
+        R dv = rate;
+        if (F.flags & SPH_AXISYMMETRIC) dv -= (R)(vi4.y / fmax(xiy, F.axi_rmin));
+        st.div[i] = dv;
+        w_diff += -(double)b.V[i] * (double)b.p[i] * (double)drate;
+        R Vi = b.V[i];
+        R dVdt = -Vi * dv;
+        R Vn;
+        double Vfull;
+        if (sizeof(R) == 4 && F.comp_vol) {
+            double sum = (double)Vi + (double)b.V_lo[i] + (double)((R)dt * dVdt);
+            Vn = (R)sum;
+            b.V_lo[i] = (R)(sum - (double)Vn);
+            Vfull = sum;
+        } else {
+            Vn = Vi + (R)dt * dVdt;
+            Vfull = (double)Vn;
+        }
+        st.V_new[i] = Vn;
+        st.rho_new[i] = (R)(m / Vfull);
+        // ---- the pressure from the particle's EOS row (SOLID.py:3212-3252)
+        b.p[i] = (R)solid_eos(M, m / Vfull, F.allow_neg);
+    }
+
+    __device__ void reduce(double* red, int slot) {
+        const int ps = a.pstride;
+        double v0 = block_sum(w_diff, red);
+        double v1 = block_sum(w_plastic, red);
+        double v2 = block_sum(w_iepair, red);
+        double v3 = block_sum(w_ieadv, red);
+        double v4 = block_sum(w_iediff, red);
+        double v5 = block_sum(w_ieale, red);
+        if (threadIdx.x == 0) {
+            a.s.part[PART_WDIFF * ps + slot] = v0;
+            a.s.part[PART_PLASTIC * ps + slot] = v1;
+            a.s.part[PART_IEPAIR * ps + slot] = v2;
+            a.s.part[PART_IEADV * ps + slot] = v3;
+            a.s.part[PART_IEDIFF * ps + slot] = v4;
+            a.s.part[PART_IEALE * ps + slot] = v5;
+        }
+    }
+};
+
+// ------------------------------------------------------------------------------------
+// pass SB: compute_lam second pass + compute_stress_accel_task (SOLID.py:3544-3852) +
+//          compute_dt_task (4116-4156) per pair; the stress accelerations per particle
+// ------------------------------------------------------------------------------------
+template <typename R> struct SolidForceBody {
+    typedef SolidArgs<R> Args;
+    const Args& a;
+    double c2loc, amag, w_visc, w_hg;
+    double4 vi4, xi4;
+    R rho_i, lam_i, inv_dt, c2max, p_i;
+    int eos_i;
+    Sym6<R> si;
+    R Gi[3][3];
+    R glx, gly, glz;
+    R swx, swy, swz;          // Sum w_j grad W
+    R fjx, fjy, fjz;          // Sum w_j sigma_j grad W
+    R wsig;                   // Sum w_j ((sigma_i + sigma_j) grad W) . (v_i - v_j)
+    R avx, avy, avz, wmor;    // Morris: Sum f_v grad W, Sum f_v grad W . dv
+    R hx, hy, hz, whg;        // hourglass: Sum (damp / prefactor) rhat, its work
+    R aax, aay, aaz;
+
+    __device__ SolidForceBody(const Args& a_) : a(a_), c2loc(a_.F.csig2), amag(0.0), w_visc(0.0), w_hg(0.0) {}
+
+    __device__ void begin(int i, double& x, double& y, double& z) {
+        const DBuf<R>& b = a.st.b;
+        const bool two_d = (a.F.flags & SPH_TWO_D) != 0;
+        xi4 = b.x[i];
+        x = xi4.x; y = xi4.y; z = xi4.z;
+        vi4 = b.v[i];
+        rho_i = a.st.rho_new[i];
+        p_i = b.p[i];
+        lam_i = a.st.lam[i];
+        inv_dt = (R)(1.0 / a.st.scal[SPH_S_DT]);
+        c2max = (R)a.F.csig2;
+        eos_i = a.sd.eos[i];
+        si = load_sig(a.sd.sig_in, i, two_d);
+        if (a.S.hg_explicit) load_gv(a.sd.gv, i, two_d, Gi);
+        glx = gly = glz = swx = swy = swz = fjx = fjy = fjz = wsig = 0;
+        avx = avy = avz = wmor = hx = hy = hz = whg = aax = aay = aaz = 0;
+    }
+
+    __device__ void pair(int j, int mir, double rxd, double ryd, double rzd) {
+        const DFluid& F = a.F;
+        const DSolidP& S = a.S;
+        const DBuf<R>& b = a.st.b;
+        const DState<R>& st = a.st;
+        const bool two_d = (F.flags & SPH_TWO_D) != 0;
+        R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
+        R r2 = rx * rx + ry * ry + rz * rz;
+        R rn = fsqrt<R>(r2);
+        R c = gradw_c<R>(F, rn);
+        R gx = c * rx, gy = c * ry, gz = c * rz;
+        R wo = wq<R>(F, b, b.V, j);
+        R wj = wq<R>(F, b, st.V_new, j);
+        R dl = st.lam[j] - lam_i;
+        glx += wo * dl * gx; gly += wo * dl * gy; glz += wo * dl * gz;
+        // stress divergence: sigma = -p I + s
+        Sym6<R> sj = load_sig(a.sd.sig_in, j, two_d);
+        mirror_sig(sj, mir);
+        R p_j = b.p[j];
+        R sjx = (sj.xx - p_j) * gx + sj.xy * gy + sj.xz * gz;
+        R sjy = sj.xy * gx + (sj.yy - p_j) * gy + sj.yz * gz;
+        R sjz = sj.xz * gx + sj.yz * gy + (sj.zz - p_j) * gz;
+        R six = (si.xx - p_i) * gx + si.xy * gy + si.xz * gz;
+        R siy = si.xy * gx + (si.yy - p_i) * gy + si.yz * gz;
+        R siz = si.xz * gx + si.yz * gy + (si.zz - p_i) * gz;
+        swx += wj * gx; swy += wj * gy; swz += wj * gz;
+        fjx += wj * sjx; fjy += wj * sjy; fjz += wj * sjz;
+        double4 vj4 = b.v[j];
+        if (mir & 1) vj4.y = -vj4.y;
+        if (mir & 2) vj4.z = -vj4.z;
+        R dvx = (R)(vi4.x - vj4.x), dvy = (R)(vi4.y - vj4.y), dvz = (R)(vi4.z - vj4.z);
+        wsig += wj * ((six + sjx) * dvx + (siy + sjy) * dvy + (siz + sjz) * dvz);
+        if (F.visc > 0.0) {
+            R vr = dvx * rx + dvy * ry + dvz * rz;
+            R fv = wj * vr * frcp<R>(r2 + (R)F.eta2);
+            avx += fv * gx; avy += fv * gy; avz += fv * gz;
+            wmor += fv * (gx * dvx + gy * dvy + gz * dvz);
+        }
+        if (S.hg_explicit && rn > (R)1e-9) {
+            // the rate damper on v (SOLID.py:3778-3825): dv_hg = (v_j - v_i) - <G> (x_j - x_i)
+            R Gj[3][3];
+            load_gv(a.sd.gv, j, two_d, Gj);
+            mirror_gv(Gj, mir);
+            R inv_rn = frcp<R>(rn);
+            R ex = -rx * inv_rn, ey = -ry * inv_rn, ez = -rz * inv_rn;
+            R hx_ = -dvx + (R)0.5 * ((Gi[0][0] + Gj[0][0]) * rx + (Gi[0][1] + Gj[0][1]) * ry + (Gi[0][2] + Gj[0][2]) * rz);
+            R hy_ = -dvy + (R)0.5 * ((Gi[1][0] + Gj[1][0]) * rx + (Gi[1][1] + Gj[1][1]) * ry + (Gi[1][2] + Gj[1][2]) * rz);
+            R hz_ = -dvz + (R)0.5 * ((Gi[2][0] + Gj[2][0]) * rx + (Gi[2][1] + Gj[2][1]) * ry + (Gi[2][2] + Gj[2][2]) * rz);
+            R hg = hx_ * ex + hy_ * ey + hz_ * ez;
+            R q = rn * (R)F.inv_h, om = (R)1 - q;
+            R W = (R)F.alpha * om * om * om * om * ((R)4 * q + (R)1);
+            R k = wj * W * hg;
+            hx += k * ex; hy += k * ey; hz += k * ez;
+            whg += k * (ex * dvx + ey * dvy + ez * dvz);
+        }
+        R rho_j = st.rho_new[j];
+        if (F.pst_en && F.ale_mom) {
+            typename R4t<R>::type sh = b.shift[j];
+            R djx = sh.x * inv_dt, djy = sh.y * inv_dt, djz = sh.z * inv_dt;
+            if (mir & 1) djy = -djy;
+            if (mir & 2) djz = -djz;
+            R t = wj * rho_j * (djx * gx + djy * gy + djz * gz);
+            aax -= t * dvx; aay -= t * dvy; aaz -= t * dvz;
+        }
+        if (F.dt_pair && (S.dt_pair_mode != 1 || a.sd.eos[j] == eos_i)) {
+            R drho = fabs(rho_i - rho_j);
+            if (drho > (R)(1e-3 * F.rho0)) {
+                R dp = fabs(p_i - p_j);
+                if (dp > c2max * drho) c2max = dp / drho;
+            }
+        }
+    }
+
+    __device__ void end(int i) {
+        const DFluid& F = a.F;
+        const DSolidP& S = a.S;
+        const DBuf<R>& b = a.st.b;
+        const DState<R>& st = a.st;
+        const DSolid<R>& sd = a.sd;
+        const sph_material& M = sd.mat[eos_i];
+        const R* Li = st.L6 + 8 * (size_t)i;
+        R lgx = Li[0] * glx + Li[3] * gly + Li[4] * glz;
+        R lgy = Li[3] * glx + Li[1] * gly + Li[5] * glz;
+        R lgz = Li[4] * glx + Li[5] * gly + Li[2] * glz;
+        if (F.flags & SPH_TWO_D) lgz = 0;
+        typename R4t<R>::type lg; lg.x = lgx; lg.y = lgy; lg.z = lgz; lg.w = 0;
+        st.lam_grad[i] = lg;
+        // f_ij / mw_i = w_i w_j (sigma_i + sigma_j) grad W / mw_i, with w_i / mw_i = 1 / rho_i
+        R wi = wq<R>(F, b, st.V_new, i), mwi = mwq<R>(F, b, i);
+        R kf = wi / mwi;
+        R six = (si.xx - p_i) * swx + si.xy * swy + si.xz * swz;
+        R siy = si.xy * swx + (si.yy - p_i) * swy + si.yz * swz;
+        R siz = si.xz * swx + si.yz * swy + (si.zz - p_i) * swz;
+        double ax = (double)(kf * (six + fjx)), ay = (double)(kf * (siy + fjy)), az = (double)(kf * (siz + fjz));
+        double deint = -0.5 * (double)kf * (double)wsig;
+        const double kv = 10.0 * F.visc;   // 2 (dim + 2), dim = 3
+        ax += kv * (double)avx; ay += kv * (double)avy; az += kv * (double)avz;
+        double dvisc = 0.0;
+        if (F.visc > 0.0) {
+            double wm = -0.5 * kv * (double)wmor;
+            deint += wm;
+            if (S.visc_work) dvisc = -wm;
+        }
+        double dhg = 0.0;
+        if (S.hg_explicit) {
+            // alpha (rho0 / rho_i) (c_p / h) (1 / V0) (1 / W(dx0))
+            double k = S.alpha_hg * (M.rho0 / (double)rho_i) * (M.c_p * F.inv_h) * S.inv_V0 * S.inv_wdx0;
+            ax += k * (double)hx; ay += k * (double)hy; az += k * (double)hz;
+            dhg = -0.5 * k * (double)whg;
+        }
+        if (F.pst_en && F.ale_mom) {
+            double ir = 1.0 / (double)rho_i;
+            ax += (double)aax * ir; ay += (double)aay * ir; az += (double)aaz * ir;
+        }
+        if (F.flags & SPH_AXISYMMETRIC) {
+            // the geometric source (SOLID.py:3458-3480): s_xr/(rho r), (s_rr - s_tt)/(rho r)
+            double inv = 1.0 / ((double)rho_i * fmax(xi4.y, F.axi_rmin));
+            ax += (double)si.xy * inv;
+            ay += ((double)si.yy - (double)si.zz) * inv;
+        }
+        ax += F.g[0]; ay += F.g[1]; az += F.g[2];
+        if (S.damping > 0.0) { ax -= S.damping * vi4.x; ay -= S.damping * vi4.y; az -= S.damping * vi4.z; }
+        typename R4t<R>::type a4; a4.x = (R)ax; a4.y = (R)ay; a4.z = (R)az; a4.w = 0;
+        st.acc[i] = a4;
+        sd.de_int[i] = (R)deint;
+        const double mw = (double)mwi;
+        w_visc += (double)b.m[i] * dvisc;
+        w_hg += mw * dhg;
+        c2loc = fmax(c2loc, (double)c2max);
+        if (finite3(xi4.x, xi4.y, xi4.z))
+            amag = fmax(amag, sqrt((double)a4.x * a4.x + (double)a4.y * a4.y + (double)a4.z * a4.z));
+    }
+
+    __device__ void reduce(double* red, int slot) {
+        double bc = block_max(c2loc, red);
+        double ba = block_max(amag, red);
+        double bv = block_sum(w_visc, red);
+        double bh = block_sum(w_hg, red);
+        if (threadIdx.x == 0) {
+            atomic_max_pos(&a.s.maxv[MAX_C2], bc);
+            atomic_max_pos(&a.s.maxv[MAX_ACC], ba);
+            a.s.part[PART_VISCS * a.pstride + slot] = bv;
+            a.s.part[PART_HGW * a.pstride + slot] = bh;
+        }
+    }
+};
+
+// ------------------------------------------------------------------------------------
+// grad v alone (the super-time-stepped damper's refresh, SOLID.py:5165-5166)
+// ------------------------------------------------------------------------------------
+template <typename R> struct GradVBody {
+    typedef SolidArgs<R> Args;
+    const Args& a;
+    double4 vi4;
+    double xiy;
+    R G[3][3];
+    R e00, e11, e22, e01, e02, e12;
+    __device__ GradVBody(const Args& a_) : a(a_) {}
+    __device__ void begin(int i, double& x, double& y, double& z) {
+        double4 xi = a.st.b.x[i];
+        x = xi.x; y = xi.y; z = xi.z;
+        xiy = xi.y;
+        vi4 = a.st.b.v[i];
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) G[p][q] = 0;
+        e00 = e11 = e22 = e01 = e02 = e12 = 0;
+    }
+    __device__ void pair(int j, int mir, double rxd, double ryd, double rzd) {
+        const DFluid& F = a.F;
+        const DBuf<R>& b = a.st.b;
+        const bool two_d = (F.flags & SPH_TWO_D) != 0;
+        R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
+        R r2 = rx * rx + ry * ry + rz * rz;
+        R c = gradw_c<R>(F, fsqrt<R>(r2));
+        R wj = wq<R>(F, b, b.V, j);
+        R gx = c * rx, gy = c * ry, gz = c * rz;
+        double4 vj4 = b.v[j];
+        if (mir & 1) vj4.y = -vj4.y;
+        if (mir & 2) vj4.z = -vj4.z;
+        R ux = -wj * (R)(vi4.x - vj4.x), uy = -wj * (R)(vi4.y - vj4.y);
+        G[0][0] += ux * gx; G[0][1] += ux * gy; G[1][0] += uy * gx; G[1][1] += uy * gy;
+        if (!two_d) {
+            R uz = -wj * (R)(vi4.z - vj4.z);
+            G[0][2] += ux * gz; G[1][2] += uy * gz;
+            G[2][0] += uz * gx; G[2][1] += uz * gy; G[2][2] += uz * gz;
+        }
+        if (a.S.kc) {
+            R sc = -wj * c;
+            e00 += sc * rx * rx; e11 += sc * ry * ry; e01 += sc * rx * ry;
+            if (!two_d) { e22 += sc * rz * rz; e02 += sc * rx * rz; e12 += sc * ry * rz; }
+        }
+    }
+    __device__ void end(int i) {
+        const DFluid& F = a.F;
+        const bool two_d = (F.flags & SPH_TWO_D) != 0;
+        double Gd[3][3];
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) Gd[p][q] = (double)G[p][q];
+        if (a.S.kc) {
+            double K6[6], me;
+            if (two_d) {
+                spd_inverse2(e00, e01, e01, e11, a.S.l_eig_tol, K6);
+                K6[2] = 1.0; K6[4] = K6[5] = 0.0;
+            } else {
+                double A6[6] = {(double)e00, (double)e11, (double)e22, (double)e01, (double)e02, (double)e12};
+                spd_inverse3(A6, a.S.l_eig_tol, K6, &me);
+            }
+            double K[3][3] = {{K6[0], K6[3], K6[4]}, {K6[3], K6[1], K6[5]}, {K6[4], K6[5], K6[2]}};
+            double H[3][3];
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) H[p][q] = Gd[p][0] * K[0][q] + Gd[p][1] * K[1][q] + Gd[p][2] * K[2][q];
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) Gd[p][q] = H[p][q];
+        }
+        if (two_d)
+            for (int k = 0; k < 3; ++k) Gd[2][k] = Gd[k][2] = 0.0;
+        if (F.flags & SPH_AXISYMMETRIC) Gd[2][2] = vi4.y / fmax(xiy, F.axi_rmin);
+        R* go = a.sd.gv + 10 * (size_t)i;
+        go[0] = (R)Gd[0][0]; go[1] = (R)Gd[0][1]; go[2] = (R)Gd[1][0]; go[3] = (R)Gd[1][1];
+        go[4] = (R)Gd[2][2]; go[5] = (R)Gd[0][2]; go[6] = (R)Gd[1][2]; go[7] = (R)Gd[2][0];
+        go[8] = (R)Gd[2][1]; go[9] = 0;
+    }
+    __device__ void reduce(double*, int) {}
+};
+
+// ------------------------------------------------------------------------------------
+// one forward-Euler hourglass sub-step on the stage velocities (_hg_rate_accel,
+// SOLID.py:4260-4300; sts_stage at (mu, nu) = (1, 0), 4492-4503)
+// ------------------------------------------------------------------------------------
+template <typename R> struct HgBody {
+    typedef SolidArgs<R> Args;
+    const Args& a;
+    double4 yi;
+    R Gi[3][3];
+    R hx, hy, hz;
+    __device__ HgBody(const Args& a_) : a(a_) {}
+    __device__ void begin(int i, double& x, double& y, double& z) {
+        double4 xi = a.st.b.x[i];
+        x = xi.x; y = xi.y; z = xi.z;
+        yi = a.ysrc[i];
+        load_gv(a.sd.gv, i, (a.F.flags & SPH_TWO_D) != 0, Gi);
+        hx = hy = hz = 0;
+    }
+    __device__ void pair(int j, int mir, double rxd, double ryd, double rzd) {
+        const DFluid& F = a.F;
+        const DBuf<R>& b = a.st.b;
+        const bool two_d = (F.flags & SPH_TWO_D) != 0;
+        R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
+        R rn = fsqrt<R>(rx * rx + ry * ry + rz * rz);
+        if (!(rn > (R)1e-9)) return;
+        double4 yj = a.ysrc[j];
+        if (mir & 1) yj.y = -yj.y;
+        if (mir & 2) yj.z = -yj.z;
+        R Gj[3][3];
+        load_gv(a.sd.gv, j, two_d, Gj);
+        mirror_gv(Gj, mir);
+        R inv_rn = frcp<R>(rn);
+        R ex = -rx * inv_rn, ey = -ry * inv_rn, ez = -rz * inv_rn;
+        R dx_ = (R)(yj.x - yi.x) + (R)0.5 * ((Gi[0][0] + Gj[0][0]) * rx + (Gi[0][1] + Gj[0][1]) * ry + (Gi[0][2] + Gj[0][2]) * rz);
+        R dy_ = (R)(yj.y - yi.y) + (R)0.5 * ((Gi[1][0] + Gj[1][0]) * rx + (Gi[1][1] + Gj[1][1]) * ry + (Gi[1][2] + Gj[1][2]) * rz);
+        R dz_ = (R)(yj.z - yi.z) + (R)0.5 * ((Gi[2][0] + Gj[2][0]) * rx + (Gi[2][1] + Gj[2][1]) * ry + (Gi[2][2] + Gj[2][2]) * rz);
+        R hg = dx_ * ex + dy_ * ey + dz_ * ez;
+        R q = rn * (R)F.inv_h, om = (R)1 - q;
+        R W = (R)F.alpha * om * om * om * om * ((R)4 * q + (R)1);
+        R k = wq<R>(F, b, b.V, j) * W * hg;
+        hx += k * ex; hy += k * ey; hz += k * ez;
+    }
+    __device__ void end(int i) {
+        const DFluid& F = a.F;
+        const DSolidP& S = a.S;
+        const DBuf<R>& b = a.st.b;
+        const sph_material& M = a.sd.mat[a.sd.eos[i]];
+        double rho_i = (double)b.m[i] / (double)b.V[i];
+        double k = S.alpha_hg * (M.rho0 / rho_i) * (M.c_p * F.inv_h) * S.inv_V0 * S.inv_wdx0;
+        const double sub = a.st.scal[SPH_S_DT] / a.st.scal[SPH_S_STS_STAGES];
+        double4 y = yi;
+        y.x = 1.0 * y.x + sub * (k * (double)hx);
+        y.y = 1.0 * y.y + sub * (k * (double)hy);
+        y.z = 1.0 * y.z + sub * (k * (double)hz);
+        a.ydst[i] = y;
+    }
+    __device__ void reduce(double*, int) {}
+};
+
+// ------------------------------------------------------------------------------------
+// streaming kernels of the solid step
+// ------------------------------------------------------------------------------------
+// update_load_factor (SOLID.py:5001-5017) from the clock; banks what the previous step
+// left for this one (SOLID.py:2146-2236 read them one step late); clears the grips.
+__global__ void k_solid_begin(double* scal, DSolidP S, double* cons, int ncons) {
+    if (threadIdx.x != 0) return;
+    double dt = scal[SPH_S_DT];
+    scal[SPH_S_VISC_WORK] += dt * scal[SPH_S_PEND_VISC];
+    scal[SPH_S_HG_WORK] += dt * scal[SPH_S_PEND_HG];
+    scal[SPH_S_LEFTOVER] += scal[SPH_S_PEND_LEFT];
+    scal[SPH_S_PEND_VISC] = scal[SPH_S_PEND_HG] = scal[SPH_S_PEND_LEFT] = 0.0;
+    double t = scal[SPH_S_SIM_TIME];
+    if (S.ramp_time <= 0.0) {
+        scal[SPH_S_LOAD_S] = 1.0;
+        scal[SPH_S_LOAD_SDOT] = 0.0;
+    } else {
+        double u = fmin(1.0, fmax(0.0, t / S.ramp_time));
+        scal[SPH_S_LOAD_S] = u * u * (3.0 - 2.0 * u);
+        scal[SPH_S_LOAD_SDOT] = 6.0 * u * (1.0 - u) / S.ramp_time;
+    }
+    for (int k = 0; k < 6 * ncons; ++k) cons[k] = 0.0;
+}
+
+// compute_adaptive_dt (DSPH.py:1365-1430) with the solid's caps: clamp_dt_hourglass
+// (SOLID.py:4158-4161) or sts_stage_count (4510-4539); banks pass SA's works.
+__global__ __launch_bounds__(1024) void k_dt_solid(DFluid F, DSolidP S, double* scal, DScratch s, int nparts,
+                                                   int pstride) {
+    __shared__ double red[1024 / SPH_WAVE];
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int slots[8] = {PART_WDIFF, PART_PLASTIC, PART_IEPAIR, PART_IEADV, PART_IEDIFF, PART_IEALE,
+                          PART_VISCS, PART_HGW};
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x)
+        for (int q = 0; q < 8; ++q) acc[q] += s.part[slots[q] * pstride + k];
+    double t[8];
+    for (int q = 0; q < 8; ++q) t[q] = block_sum(acc[q], red);
+    if (threadIdx.x == 0) {
+        double dt_prev = scal[SPH_S_DT];
+        scal[SPH_S_DIFF_WORK] += dt_prev * t[0];
+        scal[SPH_S_PLASTIC_WORK] += t[1];
+        scal[SPH_S_IE_PAIR] += t[2];
+        scal[SPH_S_IE_ALE_ADV] += t[3];
+        scal[SPH_S_IE_ALE_DIFF] += t[4];
+        scal[SPH_S_IE_ALE] += t[5];
+        scal[SPH_S_PEND_VISC] = S.visc_work ? t[6] : 0.0;
+        scal[SPH_S_PEND_HG] = S.hg_explicit ? t[7] : 0.0;
+        double c2 = fmax(s.maxv[MAX_C2], F.csig2);
+        double am = fmax(s.maxv[MAX_ACC], 1e-10);
+        double dt = fmin(F.cfl * F.h / sqrt(c2), F.cfl * sqrt(F.h / am));
+        int st = 0;
+        if (S.hg_explicit) {
+            dt = fmin(dt, 1.6 / S.hg_rate);
+        } else if (S.sts) {
+            st = 1;
+            if (S.hg_rate > 0.0) {
+                double want = S.hg_rate * dt;
+                st = S.sts_stages > 0 ? S.sts_stages : (int)fmax(1.0, ceil(want / 1.6));
+                st = min(st, S.sts_max);
+                double cap = st * 1.6 / S.hg_rate;
+                if (dt > cap) dt = cap;
+            }
+        }
+        scal[SPH_S_DT] = dt;
+        scal[SPH_S_STS_STAGES] = (double)st;
+        scal[SPH_S_MAX_C2] = c2;
+        scal[SPH_S_MAX_ACC] = am;
+        s.maxv[MAX_C2] = 0.0;
+        s.maxv[MAX_ACC] = 0.0;
+    }
+}
+
+// advect (DSPH.py:1652-1665) for the solid: the kick and drift, the volume of pass SA
+template <typename R>
+__global__ __launch_bounds__(SPH_BLOCK) void k_advect_solid(DGrid g, DFluid F, DState<R> st) {
+    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    if (i >= st.n) return;
+    const DBuf<R>& b = st.b;
+    double dt = st.scal[SPH_S_DT];
+    typename R4t<R>::type a = st.acc[i];
+    double4 v = b.v[i];
+    v.x = fma(dt, (double)a.x, v.x); v.y = fma(dt, (double)a.y, v.y); v.z = fma(dt, (double)a.z, v.z);
+    b.v[i] = v;
+    double4 x = b.x[i];
+    if (F.disp_pos) {
+        double4 u = b.u[i], x0 = b.x0[i];
+        u.x = fma(dt, v.x, u.x); u.y = fma(dt, v.y, u.y); u.z = fma(dt, v.z, u.z);
+        b.u[i] = u;
+        x.x = x0.x + u.x; x.y = x0.y + u.y; x.z = x0.z + u.z;
+    } else {
+        x.x += dt * v.x; x.y += dt * v.y; x.z += dt * v.z;
+    }
+    x.x = wrap_pos(g, 0, x.x); x.y = wrap_pos(g, 1, x.y); x.z = wrap_pos(g, 2, x.z);
+    b.x[i] = x;
+    b.V[i] = st.V_new[i];
+}
+
+// advect_velocity (SOLID.py:4578-4588) + hg_ke_snapshot (2324-2335); loads the stages
+template <typename R>
+__global__ __launch_bounds__(SPH_BLOCK) void k_sts_kick1(DState<R> st, DSolid<R> sd, DScratch s, int pstride) {
+    __shared__ double red[SPH_BLOCK / SPH_WAVE];
+    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    double ke = 0.0;
+    if (i < st.n) {
+        const DBuf<R>& b = st.b;
+        double h = 0.5 * st.scal[SPH_S_DT];
+        typename R4t<R>::type a = st.acc[i];
+        double4 v = b.v[i];
+        v.x += h * (double)a.x; v.y += h * (double)a.y; v.z += h * (double)a.z;
+        b.v[i] = v;
+        sd.vs[i] = v;
+        sd.y0[i] = v;
+        b.V[i] = st.V_new[i];
+        ke = 0.5 * (double)b.m[i] * (v.x * v.x + v.y * v.y + v.z * v.z);
+    }
+    double bs = block_sum(ke, red);
+    if (threadIdx.x == 0) s.part[PART_KE * pstride + blockIdx.x] = bs;
+}
+
+// sts_store + hg_ke_bank + advect_velocity_finish + advect_position (SOLID.py:4508,
+// 2337-2349, 4590-4615)
+template <typename R>
+__global__ __launch_bounds__(SPH_BLOCK) void k_sts_kick2(DGrid g, DFluid F, DState<R> st, DSolid<R> sd,
+                                                         DScratch s, int pstride) {
+    __shared__ double red[SPH_BLOCK / SPH_WAVE];
+    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    double ke = 0.0, cross = 0.0;
+    if (i < st.n) {
+        const DBuf<R>& b = st.b;
+        const int stages = (int)st.scal[SPH_S_STS_STAGES];
+        double4 v = (stages & 1) ? sd.y1[i] : sd.y0[i];
+        double m = (double)b.m[i];
+        ke = 0.5 * m * (v.x * v.x + v.y * v.y + v.z * v.z);
+        double dt = st.scal[SPH_S_DT], h = 0.5 * dt;
+        typename R4t<R>::type a = st.acc[i];
+        double ax = a.x, ay = a.y, az = a.z;
+        double4 vs = sd.vs[i];
+        cross = m * ((v.x - vs.x) * (h * ax) + (v.y - vs.y) * (h * ay) + (v.z - vs.z) * (h * az));
+        v.x += h * ax; v.y += h * ay; v.z += h * az;
+        b.v[i] = v;
+        double4 x = b.x[i];
+        double drx = dt * v.x, dry = dt * v.y, drz = dt * v.z;
+        if (F.disp_pos) {
+            double4 u = b.u[i], x0 = b.x0[i];
+            u.x += drx; u.y += dry; u.z += drz;
+            b.u[i] = u;
+            x.x = x0.x + u.x; x.y = x0.y + u.y; x.z = x0.z + u.z;
+        } else {
+            x.x += drx; x.y += dry; x.z += drz;
+        }
+        x.x = wrap_pos(g, 0, x.x); x.y = wrap_pos(g, 1, x.y); x.z = wrap_pos(g, 2, x.z);
+        b.x[i] = x;
+    }
+    double b1 = block_sum(ke, red), b2 = block_sum(cross, red);
+    if (threadIdx.x == 0) {
+        s.part[PART_KE2 * pstride + blockIdx.x] = b1;
+        s.part[PART_CROSS * pstride + blockIdx.x] = b2;
+    }
+}
+
+// clear_constraint_forces + apply_constraints (SOLID.py:5019-5055), and max |v| for a
+// global shifting Mach number (taken after the grips, as DSPH.py:712-723 runs there)
+template <typename R>
+__global__ __launch_bounds__(SPH_BLOCK) void k_constraints(DFluid F, DState<R> st, DSolid<R> sd, DScratch s,
+                                                           double* cons_part, int ncons, int nblk) {
+    __shared__ double red[SPH_BLOCK / SPH_WAVE];
+    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    double fx = 0.0, fy = 0.0, fz = 0.0, vmax = 0.0;
+    int cid = -1;
+    if (i < st.n) {
+        const DBuf<R>& b = st.b;
+        int f = sd.bcf[i];
+        double4 v = b.v[i];
+        if (f != 0) {
+            const double ls = st.scal[SPH_S_LOAD_S], lsd = st.scal[SPH_S_LOAD_SDOT];
+            const double scale = (F.flags & SPH_AXISYMMETRIC) ? 2.0 * M_PI : 1.0;
+            cid = sd.bcid[i];
+            typename R4t<R>::type a = st.acc[i];
+            typename R4t<R>::type d4 = sd.bcd[i];
+            double4 x = b.x[i], u = b.u[i], x0 = b.x0[i];
+            double m = (double)b.m[i];
+            double av[3] = {(double)a.x, (double)a.y, (double)a.z};
+            double dv[3] = {(double)d4.x, (double)d4.y, (double)d4.z};
+            double xs[3] = {x.x, x.y, x.z}, us[3] = {u.x, u.y, u.z}, x0s[3] = {x0.x, x0.y, x0.z};
+            double vv[3] = {v.x, v.y, v.z}, fo[3] = {0, 0, 0};
+            for (int k = 0; k < 3; ++k) {
+                if (!(f & (1 << k))) continue;
+                if (ncons > 0 && cid >= 0) {
+                    fo[k] = scale * m * av[k];
+                    sd.cons[6 * cid + 3 + k] = dv[k] * ls;
+                }
+                us[k] = dv[k] * ls;
+                xs[k] = x0s[k] + us[k];
+                vv[k] = dv[k] * lsd;
+                av[k] = 0.0;
+            }
+            fx = fo[0]; fy = fo[1]; fz = fo[2];
+            v = make_double4(vv[0], vv[1], vv[2], 0.0);
+            b.v[i] = v;
+            b.x[i] = make_double4(xs[0], xs[1], xs[2], 0.0);
+            b.u[i] = make_double4(us[0], us[1], us[2], 0.0);
+            typename R4t<R>::type ao; ao.x = (R)av[0]; ao.y = (R)av[1]; ao.z = (R)av[2]; ao.w = 0;
+            st.acc[i] = ao;
+        }
+        if (F.ma_mode == 1) {
+            double4 x = b.x[i];
+            if (finite3(x.x, x.y, x.z)) vmax = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+        }
+    }
+    for (int c = 0; c < ncons; ++c) {
+        bool mine = cid == c;
+        double a0 = block_sum(mine ? fx : 0.0, red);
+        double a1 = block_sum(mine ? fy : 0.0, red);
+        double a2 = block_sum(mine ? fz : 0.0, red);
+        if (threadIdx.x == 0) {
+            cons_part[(3 * c + 0) * nblk + blockIdx.x] = a0;
+            cons_part[(3 * c + 1) * nblk + blockIdx.x] = a1;
+            cons_part[(3 * c + 2) * nblk + blockIdx.x] = a2;
+        }
+    }
+    if (F.ma_mode == 1) {
+        double bm = block_max(vmax, red);
+        if (threadIdx.x == 0) atomic_max_pos(&s.maxv[MAX_V], bm);
+    }
+}
+
+// the step's totals: wall work, the leftover (deferred one step), the damper's kinetic
+// energy and the split kick's cross term (SOLID.py:4590-4606), the grip forces, the clock
+__global__ __launch_bounds__(1024) void k_bank_solid(double* scal, DScratch s, DSolidP S, int nblk, int pstride,
+                                                     const double* cons_part, double* cons, int ncons,
+                                                     int ma_global) {
+    __shared__ double red[1024 / SPH_WAVE];
+    double a[5] = {0, 0, 0, 0, 0};
+    const int slots[5] = {PART_WALL, PART_LEFT, PART_KE, PART_KE2, PART_CROSS};
+    const int use = S.sts ? 5 : 2;
+    for (int k = threadIdx.x; k < nblk; k += blockDim.x)
+        for (int q = 0; q < use; ++q) a[q] += s.part[slots[q] * pstride + k];
+    double t[5] = {0, 0, 0, 0, 0};
+    for (int q = 0; q < use; ++q) t[q] = block_sum(a[q], red);
+    for (int c = 0; c < 3 * ncons; ++c) {
+        double v = 0.0;
+        for (int k = threadIdx.x; k < nblk; k += blockDim.x) v += cons_part[c * nblk + k];
+        double tv = block_sum(v, red);
+        if (threadIdx.x == 0) cons[6 * (c / 3) + (c % 3)] = tv;
+    }
+    if (threadIdx.x == 0) {
+        scal[SPH_S_WALL_WORK] += t[0];
+        scal[SPH_S_PEND_LEFT] = t[1];
+        if (S.sts) {
+            scal[SPH_S_KE_PRE] = t[2];
+            scal[SPH_S_KE_POST] = t[3];
+            scal[SPH_S_HG_WORK] += t[2] - t[3];
+            scal[SPH_S_LEFTOVER] += t[4];
+        }
+        scal[SPH_S_SIM_TIME] += scal[SPH_S_DT];
+        if (ma_global) s.maxv[MAX_V] = 0.0;
+    }
+}
+
+// f64 solid reductions: Sum m e (compute_internal_energy), the elastic energy
+// (compute_elastic_energy, SOLID.py:2295-2322)
+template <typename R>
+__global__ __launch_bounds__(SPH_BLOCK) void k_solid_reduce(int n, DBuf<R> b, DSolid<R> sd, int which, DScratch s,
+                                                            int pstride, int mixed) {
+    __shared__ double red[SPH_BLOCK / SPH_WAVE];
+    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    double r0 = 0.0;
+    if (i < n) {
+        double m = (double)b.m[i];
+        if (which == SPH_R_INTERNAL) {
+            r0 = m * (double)sd.e_in[i];
+        } else {
+            const sph_material& M = sd.mat[sd.eos[i]];
+            if (M.G > 0.0) {
+                double V = (double)b.V[i] + (mixed ? (double)b.V_lo[i] : 0.0);
+                double rho = m / V, th = (rho - M.rho0) / M.rho0;
+                const R* q = sd.sig_in + 6 * (size_t)i;
+                double ss = (double)q[0] * q[0] + (double)q[1] * q[1] + (double)q[3] * q[3] +
+                            2.0 * ((double)q[2] * q[2] + (double)q[4] * q[4] + (double)q[5] * q[5]);
+                r0 = 0.5 * M.K * th * th * V + 0.25 * ss / M.G * V;
+            }
+        }
+    }
+    double a0 = block_sum(r0, red);
+    if (threadIdx.x == 0) {
+        s.part[PART_R0 * pstride + blockIdx.x] = a0;
+        s.part[PART_R1 * pstride + blockIdx.x] = 0.0;
+        s.part[PART_R2 * pstride + blockIdx.x] = 0.0;
+    }
+}
+
+// ====================================================================================
+// host side of the solid step
+// ====================================================================================
+static DSolidP make_solid_params(const sph_ctx* c, const sph_fluid_params* P, const sph_solid_params* S) {
+    DSolidP d;
+    memset(&d, 0, sizeof d);
+    d.alpha_hg = S->hourglass;
+    d.hg_rate = S->hg_rate;
+    d.inv_wdx0 = S->w_dx0 > 0.0 ? 1.0 / S->w_dx0 : 0.0;
+    d.inv_V0 = 1.0 / S->V0;
+    d.ale_diff = S->ale_diffusion;
+    d.ramp_time = S->ramp_time;
+    d.damping = S->damping;
+    d.l_eig_tol = S->l_eig_tol;
+    d.kc = S->kernel_correction;
+    d.jaumann = S->jaumann;
+    d.transport = P->pst_enabled && S->ale_history;
+    d.transport_ep = d.transport && S->plastic;
+    d.sts = S->sts && S->hourglass > 0.0;
+    d.hg_explicit = !S->sts && S->hourglass > 0.0;
+    d.sts_stages = S->sts_stages;
+    d.sts_max = S->sts_max_stages;
+    d.sts_refresh = S->sts_refresh;
+    d.dt_pair_mode = S->dt_pair_mode;
+    d.visc_work = S->visc_work;
+    d.thermal = S->thermal;
+    d.plastic = S->plastic;
+    d.ncons = S->ncons;
+    (void)c;
+    return d;
+}
+
+template <typename R> static DSolid<R> dsolid(const sph_state* st, const sph_solid* sd) {
+    DSolid<R> d;
+    int h = sd->hcur, k = st->cur;
+    d.sig_in = (const R*)sd->sig[h]; d.sig_out = (R*)sd->sig[1 - h];
+    d.ep_in = (const R*)sd->eps_p[h]; d.ep_out = (R*)sd->eps_p[1 - h];
+    d.e_in = (const R*)sd->e_int[h]; d.e_out = (R*)sd->e_int[1 - h];
+    d.T = (R*)sd->T[k]; d.de_int = (R*)sd->de_int[k];
+    d.eos = sd->eos_id[k]; d.bcf = sd->bc_flag[k]; d.bcid = sd->bc_id[k];
+    d.bcd = (const typename R4t<R>::type*)sd->bc_disp[k];
+    d.gv = (R*)sd->grad_v;
+    d.y0 = (double4*)sd->hg_y[0]; d.y1 = (double4*)sd->hg_y[1]; d.vs = (double4*)sd->v_sync;
+    d.mat = sd->mat;
+    d.cons = sd->cons;
+    return d;
+}
+
+template <typename R>
+static SolidArgs<R> solid_args(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F, const DSolidP& S) {
+    SolidArgs<R> a;
+    a.F = F; a.S = S;
+    a.st = dstate<R>(c, st);
+    a.sd = dsolid<R>(st, sd);
+    a.s = c->s;
+    a.X = (const double4*)st->buf[st->cur].x;
+    a.pstride = c->pstride;
+    a.ysrc = nullptr; a.ydst = nullptr; a.stage = 0;
+    return a;
+}
+
+static bool solid_ok(const sph_solid* sd, bool need_step) {
+    for (int k = 0; k < 2; ++k)
+        if (!sd->sig[k] || !sd->eps_p[k] || !sd->e_int[k] || !sd->T[k] || !sd->de_int[k] || !sd->eos_id[k] ||
+            !sd->bc_flag[k] || !sd->bc_id[k] || !sd->bc_disp[k])
+            return false;
+    if (sd->hcur != 0 && sd->hcur != 1) return false;
+    if (need_step && (!sd->grad_v || !sd->hg_y[0] || !sd->hg_y[1] || !sd->v_sync || !sd->mat || !sd->cons))
+        return false;
+    return true;
+}
+
+extern "C" int sph_solid_sort(sph_ctx* c, sph_state* st, sph_solid* sd, void* stream) {
+    if (!c || !st || !sd) return SPH_EINVAL;
+    int rc = check_state(c, st, false);
+    if (rc) return rc;
+    if (!solid_ok(sd, false)) return set_err(c, SPH_EINVAL, "null or invalid solid field%s", "", 0);
+    if (!buf_ok(st->buf[1 - st->cur], c->dims.precision == SPH_PREC_MIXED))
+        return set_err(c, SPH_EINVAL, "null field in alternate buffer%s", "", 0);
+    if (!st->rho) return set_err(c, SPH_EINVAL, "null rho%s", "", 0);
+    hipStream_t hs = (hipStream_t)stream;
+    int n = (int)c->dims.n;
+    int nb = c->nblk;
+    bool mixed = c->dims.precision == SPH_PREC_MIXED;
+    const sph_sorted& A = st->buf[st->cur];
+    const sph_sorted& B = st->buf[1 - st->cur];
+    const int k = st->cur, h = sd->hcur;
+    timer_begin(c, 3, hs);
+    hipLaunchKernelGGL(k_zero_scal, dim3(1), dim3(64), 0, hs, st->scal);
+    hipLaunchKernelGGL(k_keys, dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, n, (const double4*)A.x,
+                       (const void*)A.m, mixed ? 0 : 1, c->s, st->scal);
+    int total = (int)c->nbuck + 1;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(c->scan_blocks), dim3(SPH_BLOCK), 0, hs, total, c->s.count, c->s.start,
+                       c->s.bsum);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, hs, c->scan_blocks, c->s.bsum);
+    hipLaunchKernelGGL(k_scan_add, dim3((total + SPH_BLOCK - 1) / SPH_BLOCK), dim3(SPH_BLOCK), 0, hs, total,
+                       c->s.start, c->s.cursor, (const int*)c->s.bsum);
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(SPH_BLOCK), 0, hs, n, c->s);
+    hipLaunchKernelGGL(k_rank, dim3(nb), dim3(SPH_BLOCK), 0, hs, n, c->s, (const int32_t*)A.pid);
+    PermTable T;
+    int rb = mixed ? 4 : 8, r4 = mixed ? 16 : 32;
+    int nf = 0;
+    auto add = [&](const void* s_, void* d_, int bytes) { T.f[nf].src = s_; T.f[nf].dst = d_; T.f[nf].bytes = bytes; ++nf; };
+    add(A.pid, B.pid, 4); add(A.obj, B.obj, 4);
+    add(A.x, B.x, 32); add(A.x0, B.x0, 32); add(A.u, B.u, 32); add(A.v, B.v, 32);
+    add(A.shift, B.shift, r4);
+    add(A.V, B.V, rb); add(A.m, B.m, rb); add(A.p, B.p, rb);
+    if (mixed) add(A.V_lo, B.V_lo, rb);
+    add(sd->sig[h], sd->sig[1 - h], 6 * rb);
+    add(sd->eps_p[h], sd->eps_p[1 - h], rb);
+    add(sd->e_int[h], sd->e_int[1 - h], rb);
+    add(sd->T[k], sd->T[1 - k], rb);
+    add(sd->de_int[k], sd->de_int[1 - k], rb);
+    add(sd->eos_id[k], sd->eos_id[1 - k], 4);
+    add(sd->bc_flag[k], sd->bc_flag[1 - k], 4);
+    add(sd->bc_id[k], sd->bc_id[1 - k], 4);
+    add(sd->bc_disp[k], sd->bc_disp[1 - k], r4);
+    T.nf = nf;
+    T.real_bytes = rb;
+    T.m = A.m; T.V = A.V; T.V_lo = A.V_lo;
+    T.rho = st->rho;
+    hipLaunchKernelGGL(k_permute, dim3(nb), dim3(SPH_BLOCK), 0, hs, n, T, c->s, c->grid.nbox);
+    timer_end(c, 3, hs);
+    rc = check_launch(c, "solid sort");
+    if (rc) return rc;
+    st->cur = 1 - st->cur;
+    sd->hcur = 1 - sd->hcur;
+    return SPH_OK;
+}
+
+template <typename R>
+static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F, const DSolidP& S,
+                      hipStream_t hs) {
+    const int nb = c->nblk;
+    int rc;
+    // pass SA: reads history[hcur], writes history[1 - hcur]
+    timer_begin(c, 4, hs);
+    launch_pair_pass<StressBody<R>>(c, solid_args<R>(c, st, sd, F, S), hs);
+    timer_end(c, 4, hs);
+    if ((rc = check_launch(c, "solid stress pass"))) return rc;
+    sd->hcur = 1 - sd->hcur;
+    // pass SB on the updated stress, volume and pressure
+    timer_begin(c, 5, hs);
+    launch_pair_pass<SolidForceBody<R>>(c, solid_args<R>(c, st, sd, F, S), hs);
+    timer_end(c, 5, hs);
+    hipLaunchKernelGGL(k_dt_solid, dim3(1), dim3(1024), 0, hs, F, S, st->scal, c->s, c->tiles.ntiles + c->nblk,
+                       c->pstride);
+    if ((rc = check_launch(c, "solid force pass"))) return rc;
+    if (S.sts) {
+        // the stage count is a device number: read it back to size the sub-step loop
+        HIPCHECK(c, hipMemcpyAsync(c->host_stages, (const char*)st->scal + sizeof(double) * SPH_S_STS_STAGES,
+                                   sizeof(double), hipMemcpyDeviceToHost, hs));
+        hipLaunchKernelGGL((k_sts_kick1<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, dstate<R>(c, st), dsolid<R>(st, sd),
+                           c->s, c->pstride);
+        if (S.sts_refresh) launch_pair_pass<GradVBody<R>>(c, solid_args<R>(c, st, sd, F, S), hs);
+        if ((rc = check_launch(c, "solid half kick"))) return rc;
+        HIPCHECK(c, hipStreamSynchronize(hs));
+        int stages = (int)(*(const double*)c->host_stages);
+        if (stages < 1 || stages > S.sts_max)
+            return set_err(c, SPH_EINVAL, "hourglass sub-step count out of range%s: %lld", "", stages);
+        SolidArgs<R> a = solid_args<R>(c, st, sd, F, S);
+        for (int k = 0; k < stages; ++k) {
+            a.ysrc = (k & 1) ? a.sd.y1 : a.sd.y0;
+            a.ydst = (k & 1) ? a.sd.y0 : a.sd.y1;
+            a.stage = k;
+            timer_begin(c, 6, hs);
+            launch_pair_pass<HgBody<R>>(c, a, hs);
+            timer_end(c, 6, hs);
+        }
+        hipLaunchKernelGGL((k_sts_kick2<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st),
+                           dsolid<R>(st, sd), c->s, c->pstride);
+    } else {
+        hipLaunchKernelGGL((k_advect_solid<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st));
+    }
+    if ((rc = check_launch(c, "solid advect"))) return rc;
+    hipLaunchKernelGGL((k_constraints<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, F, dstate<R>(c, st),
+                       dsolid<R>(st, sd), c->s, c->cons_part, S.ncons, nb);
+    if (F.ma_mode == 1) hipLaunchKernelGGL(k_publish_vmax, dim3(1), dim3(64), 0, hs, st->scal, c->s);
+    if (F.pst_en) {
+        PassArgs<R> pa = pass_args<R>(c, st, F);
+        pa.bcf = sd->bc_flag[st->cur];
+        timer_begin(c, 2, hs);
+        launch_pair_pass<ShiftBody<R>>(c, pa, hs);
+        timer_end(c, 2, hs);
+    }
+    hipLaunchKernelGGL((k_finish<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st), c->s,
+                       c->pstride, (R*)sd->sig[sd->hcur]);
+    hipLaunchKernelGGL(k_bank_solid, dim3(1), dim3(1024), 0, hs, st->scal, c->s, S, nb, c->pstride,
+                       (const double*)c->cons_part, sd->cons, S.ncons, F.ma_mode == 1 ? 1 : 0);
+    return check_launch(c, "solid finish");
+}
+
+extern "C" int sph_solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const sph_fluid_params* P,
+                              const sph_solid_params* S, void* stream) {
+    if (!c || !st || !sd || !P || !S) return SPH_EINVAL;
+    int rc = check_fluid(c, st, P);
+    if (rc) return rc;
+    if (!solid_ok(sd, true)) return set_err(c, SPH_EINVAL, "null or invalid solid field%s", "", 0);
+    if (S->nmat <= 0 || S->ncons < 0 || S->ncons > SPH_MAX_CONS)
+        return set_err(c, SPH_EINVAL, "material or constraint count out of range%s: %lld", "", S->ncons);
+    if (!(S->V0 > 0.0)) return set_err(c, SPH_EINVAL, "reference cell volume must be positive%s", "", 0);
+    if (S->hourglass > 0.0 && !(S->hg_rate > 0.0 && S->w_dx0 > 0.0))
+        return set_err(c, SPH_EINVAL, "hourglass rate and W(dx0) must be positive%s", "", 0);
+    if (S->sts && (S->sts_max_stages < 1 || S->sts_max_stages > 1024 || S->sts_stages < 0 ||
+                   S->sts_stages > S->sts_max_stages))
+        return set_err(c, SPH_EINVAL, "hourglass sub-step limits out of range%s", "", 0);
+    hipStream_t hs = (hipStream_t)stream;
+    if (!c->cons_part) {
+        HIPCHECK(c, hipMalloc((void**)&c->cons_part, sizeof(double) * 3 * SPH_MAX_CONS * (size_t)c->nblk));
+        c->cons_cap = SPH_MAX_CONS;
+    }
+    if (!c->host_stages) HIPCHECK(c, hipHostMalloc((void**)&c->host_stages, sizeof(double)));
+    DFluid F = make_fluid(c, P);
+    DSolidP D = make_solid_params(c, P, S);
+    hipLaunchKernelGGL(k_solid_begin, dim3(1), dim3(64), 0, hs, st->scal, D, sd->cons, S->ncons);
+    if ((rc = sph_solid_sort(c, st, sd, stream))) return rc;
+    if (c->dims.precision == SPH_PREC_MIXED) return solid_step<float>(c, st, sd, F, D, hs);
+    return solid_step<double>(c, st, sd, F, D, hs);
+}
+
+extern "C" int sph_solid_reduce(sph_ctx* c, const sph_state* st, const sph_solid* sd, int32_t which, double* out,
+                                void* stream) {
+    if (!c || !st || !sd) return SPH_EINVAL;
+    int rc = check_state(c, st, false);
+    if (rc) return rc;
+    if (!out || (which != SPH_R_INTERNAL && which != SPH_R_ELASTIC))
+        return set_err(c, SPH_EINVAL, "bad solid reduce%s", "", 0);
+    if (!solid_ok(sd, false) || !sd->mat) return set_err(c, SPH_EINVAL, "null solid field%s", "", 0);
+    hipStream_t hs = (hipStream_t)stream;
+    int n = (int)c->dims.n;
+    bool mixed = c->dims.precision == SPH_PREC_MIXED;
+    if (mixed)
+        hipLaunchKernelGGL(k_solid_reduce<float>, dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, n,
+                           dbuf<float>(st->buf[st->cur]), dsolid<float>(st, sd), which, c->s, c->pstride, 1);
+    else
+        hipLaunchKernelGGL(k_solid_reduce<double>, dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, n,
+                           dbuf<double>(st->buf[st->cur]), dsolid<double>(st, sd), which, c->s, c->pstride, 0);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, hs, c->s, c->nblk, c->pstride, out);
+    return check_launch(c, "solid reduce");
+}

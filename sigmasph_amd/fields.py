@@ -88,3 +88,51 @@ class Scalar:
 
     def __int__(self):
         return int(self[None])
+
+
+# the stored order of a symmetric deviator row: xx, yy, xy, zz, xz, yz
+_SYM = ((0, 2, 4), (2, 1, 5), (4, 5, 3))
+# the stored order of a velocity-gradient row: G00 G01 G10 G11 G22 G02 G12 G20 G21
+_GV = ((0, 1, 5), (2, 3, 6), (7, 8, 4))
+
+
+class MatrixField:
+    """A per-particle 3x3 field (the reference's `ti.Matrix.field(3, 3)`), stored
+    compactly: `layout` maps (row, col) to the column of the stored row."""
+
+    def __init__(self, getter, layout, name=""):
+        self._get = getter
+        self._layout = layout
+        self.name = name
+
+    @property
+    def shape(self):
+        return (self._get().shape[0], 3, 3)
+
+    def to_numpy(self):
+        t = self._get().detach().cpu().numpy()
+        out = np.empty((t.shape[0], 3, 3), dtype=t.dtype)
+        for a in range(3):
+            for b in range(3):
+                out[:, a, b] = t[:, self._layout[a][b]]
+        return out
+
+    def from_numpy(self, arr):
+        arr = np.asarray(arr)
+        t = self._get()
+        host = t.detach().cpu().numpy().copy()
+        for a in range(3):
+            for b in range(3):
+                host[:, self._layout[a][b]] = arr[:, a, b]
+        t.copy_(torch.as_tensor(host, dtype=t.dtype).to(t.device))
+
+    def __getitem__(self, idx):
+        return self.to_numpy()[idx]
+
+
+def sym_field(getter, name=""):
+    return MatrixField(getter, _SYM, name)
+
+
+def grad_field(getter, name=""):
+    return MatrixField(getter, _GV, name)

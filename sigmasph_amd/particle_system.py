@@ -23,7 +23,7 @@ import numpy as np
 import torch
 
 from . import _native as N
-from .fields import Field, Scalar
+from .fields import Field, Scalar, sym_field, grad_field
 
 
 def lattice_points(lower, size, diameter, dim=3):
@@ -417,6 +417,9 @@ class ParticleSystem(SceneLayout):
             setattr(self.state, name, t.data_ptr())
         self.state.scal = self.scal.data_ptr()
 
+        if self.is_solid:
+            self._init_solid_device(n)
+
         self._lib = N.lib()
         self.ctx = ctypes.c_void_p()
         N.check(self._lib.sph_ctx_create(ctypes.byref(self.ctx), ctypes.byref(self.dims)), None,
@@ -442,6 +445,50 @@ class ParticleSystem(SceneLayout):
         self.lam = Field(lambda: self._derived["lam"], 1, "lam")
         self.divergence = Field(lambda: self._derived["div"], 1, "divergence")
 
+    def _init_solid_device(self, n):
+        """Device buffers of the solid step (the sorted-field registry entries of
+        SOLID.py:443, 488-499, 836, 1240, 1271, 1307; see sph_solid in the header)."""
+        dev, R = self.device, self.real
+        z = lambda *shape, dt=R: torch.zeros(*shape, dtype=dt, device=dev)
+        self._hist = [{"sig": z(n, 6), "eps_p": z(n), "e_int": z(n)} for _ in range(2)]
+        self._sbufs = [{"T": z(n), "de_int": z(n), "eos_id": z(n, dt=torch.int32),
+                        "bc_flag": z(n, dt=torch.int32), "bc_id": z(n, dt=torch.int32),
+                        "bc_disp": z(n, 4)} for _ in range(2)]
+        self._sderived = {"grad_v": z(n, 10), "hg_y0": z(n, 4, dt=torch.float64),
+                          "hg_y1": z(n, 4, dt=torch.float64), "v_sync": z(n, 4, dt=torch.float64)}
+        self.cons = z(16, 6, dt=torch.float64)
+        h0, b0 = self._hist[0], self._sbufs[0]
+        h0["e_int"].copy_(torch.from_numpy(self.e_init))
+        b0["T"].copy_(torch.from_numpy(self.T_init))
+        b0["eos_id"].copy_(torch.from_numpy(self.eos_init))
+        b0["bc_flag"].copy_(torch.from_numpy(self.bc_flag_init))
+        b0["bc_id"].copy_(torch.from_numpy(self.bc_id_init))
+        b0["bc_disp"][:, :3].copy_(torch.from_numpy(self.bc_disp_init))
+        sd = N.SphSolid()
+        for k in range(2):
+            for name in ("sig", "eps_p", "e_int"):
+                getattr(sd, name)[k] = self._hist[k][name].data_ptr()
+            for name in ("T", "de_int", "eos_id", "bc_flag", "bc_id", "bc_disp"):
+                getattr(sd, name)[k] = self._sbufs[k][name].data_ptr()
+            sd.hg_y[k] = self._sderived["hg_y%d" % k].data_ptr()
+        sd.hcur = 0
+        sd.grad_v = self._sderived["grad_v"].data_ptr()
+        sd.v_sync = self._sderived["v_sync"].data_ptr()
+        sd.cons = self.cons.data_ptr()
+        self.solid = sd
+        H = lambda: self._hist[self.solid.hcur]
+        B = lambda: self._sbufs[self.state.cur]
+        self.sigma_dev = sym_field(lambda: H()["sig"], "sigma_dev")
+        self.eps_plastic = Field(lambda: H()["eps_p"], 1, "eps_plastic")
+        self.e_int = Field(lambda: H()["e_int"], 1, "e_int")
+        self.temperature = Field(lambda: B()["T"], 1, "temperature")
+        self.de_int = Field(lambda: B()["de_int"], 1, "de_int")
+        self.eos_id = Field(lambda: B()["eos_id"], 1, "eos_id")
+        self.bc_flag = Field(lambda: B()["bc_flag"], 1, "bc_flag")
+        self.bc_id = Field(lambda: B()["bc_id"], 1, "bc_id")
+        self.bc_disp = Field(lambda: B()["bc_disp"], 3, "bc_disp")
+        self.grad_v = grad_field(lambda: self._sderived["grad_v"], "grad_v")
+
     # ---------------------------------------------------------------------------------
     def _live(self):
         return self._bufs[self.state.cur]
@@ -452,6 +499,10 @@ class ParticleSystem(SceneLayout):
     # ---------------------------------------------------------------------------------
     def initialize_particle_system(self):
         """update_grid_id + prefix sum + counting_sort (particle_system.py:1342-1345)."""
+        if self.is_solid:
+            N.check(self._lib.sph_solid_sort(self.ctx, ctypes.byref(self.state), ctypes.byref(self.solid),
+                                             self.stream()), self.ctx, "sph_solid_sort")
+            return
         N.check(self._lib.sph_sort(self.ctx, ctypes.byref(self.state), self.stream()), self.ctx, "sph_sort")
 
     def build_solver(self):
@@ -460,7 +511,8 @@ class ParticleSystem(SceneLayout):
             from .DSPH import DSPHSolver
             return DSPHSolver(self)
         if method == "hypoElastic":
-            raise NotImplementedError("the hypoelastic solid step is not built yet")
+            from .SOLID import HypoElasticSolver
+            return HypoElasticSolver(self)
         raise ValueError("simulationMethod must be 'deltaPlusSPH' or 'hypoElastic', got: %r" % (method,))
 
     def neighbor_pairs(self, cap=None):

@@ -1,0 +1,86 @@
+"""The solid deck's host side without a GPU: material resolution, derived constants,
+the dogbone layout and its grips, and the parameter structs the kernels take."""
+import math
+
+import numpy as np
+import pytest
+
+from sigmasph_amd import _native as N
+from sigmasph_amd.config import SimConfig, SceneError
+from sigmasph_amd.materials import resolve, MaterialError
+from sigmasph_amd.particle_system import SceneLayout
+from sigmasph_amd.SOLID import SolidSettings
+from sigmasph_amd.scenes import dogbone_c2, solid_block
+
+
+def test_dogbone_signal_speed_and_dt():
+    # c_p = sqrt((K + 4G/3)/rho) = 5766 mm/ms and dt = 0.3 * 0.3 / 5766 = 1.56e-5 ms
+    lay = SceneLayout(SimConfig(dogbone_c2()))
+    st = SolidSettings(lay)
+    assert st.c_p == pytest.approx(5766.0, abs=1.0)
+    assert st.dt_initial == pytest.approx(1.56e-5, rel=2e-3)
+    M = st.default_material
+    assert M.K == pytest.approx(160.0, rel=1e-12) and M.G == pytest.approx(77.0, rel=1e-12)
+    assert M.jc["C"] == 0.07 and M.jc["eps0_dot"] == 1.0 and M.plastic_kind == N.PLASTIC_JC
+    S = st.solid_params()
+    assert S.sts == 1 and S.hourglass == 4.0 and S.ncons == 2 and S.nmat == 1
+    assert S.hg_rate == pytest.approx(4.0 * st.c_p / lay.support_radius * st.hg_neighbour_sum)
+
+
+def test_dogbone_layout_and_grips():
+    lay = SceneLayout(SimConfig(dogbone_c2()))
+    x = lay.x_init
+    assert lay.axisymmetric and lay.two_d
+    assert x[:, 1].min() == pytest.approx(0.05) and np.all(x[:, 2] == 0.0)
+    gauge = np.abs(x[:, 0]) < 14.0
+    assert x[gauge, 1].max() <= 3.0 + 1e-12           # the 6 mm gauge
+    assert x[:, 1].max() <= 5.0 + 1e-12               # the 10 mm heads
+    left = lay.bc_id_init == 0
+    right = lay.bc_id_init == 1
+    assert left.sum() == right.sum() > 0
+    assert np.all(lay.bc_flag_init[left] == 3)         # components "xy"
+    np.testing.assert_array_equal(lay.bc_disp_init[right], np.tile([1.5, 0.0, 0.0], (right.sum(), 1)))
+    assert np.all(x[left, 0] < -19.0) and np.all(x[right, 0] > 19.0)
+    assert lay.e_init.max() == 0.0 and np.all(lay.T_init == 293.0)
+
+
+def test_linear_plasticity_and_library_merge():
+    mats = resolve([{"name": "s", "density0": 7.9e-6, "eos": {"type": "linear"},
+                     "strength": {"type": "hypoElastic", "youngsModulus": 200.0, "poissonRatio": 0.25,
+                                  "plasticity": {"model": "linear", "yieldStrain": 0.002,
+                                                 "tangentModulusRatio": 0.1}}},
+                    {"name": "cu", "library": "copper", "strength": {"plasticity": {"C": 0.05}}}])
+    s = mats["s"]
+    assert s.K == pytest.approx(200.0 / (3 * 0.5)) and s.G == pytest.approx(80.0)
+    assert s.sigma_y0 == pytest.approx(0.4) and s.hardening == pytest.approx(200.0 * 0.1 / 0.9)
+    cu = mats["cu"]
+    assert cu.index == 1 and cu.jc["C"] == 0.05 and cu.jc["A"] == 0.09 and cu.cv == 383.0
+    with pytest.raises(MaterialError):
+        resolve([{"name": "bad", "density0": 1.0, "eos": {"type": "jwl"}}])
+
+
+def test_unsupported_solid_features_raise():
+    sc = solid_block()
+    sc["Hourglass"]["superTimeStepping"]["mode"] = "rkl"
+    with pytest.raises(NotImplementedError):
+        SolidSettings(SceneLayout(SimConfig(sc)))
+    sc = solid_block(extra={"Domain": {"conservativeAxialMomentum": True}})
+    with pytest.raises(NotImplementedError):
+        SolidSettings(SceneLayout(SimConfig(sc)))
+    sc = solid_block()
+    sc["Hourglass"]["unknownKey"] = 1
+    with pytest.raises(SceneError):
+        SimConfig(sc)
+
+
+def test_solid_params_defaults():
+    st = SolidSettings(SceneLayout(SimConfig(solid_block(sts=False))))
+    S = st.solid_params()
+    P = st.params()
+    assert S.sts == 0 and S.jaumann == 1 and S.ale_history == 1 and S.dt_pair_mode == 1
+    assert S.visc_work == 1 and S.thermal == 1 and S.plastic == 1
+    assert P.c_signal == st.c_p and P.dt_pair == 1
+    assert st.dt_hg_cap == pytest.approx(1.6 / st.hg_rate)
+    s, sdot = st.load_factor(0.25)
+    u = 0.25 / st.ramp_time
+    assert s == pytest.approx(u * u * (3 - 2 * u)) and sdot == pytest.approx(6 * u * (1 - u) / st.ramp_time)

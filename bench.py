@@ -1,13 +1,18 @@
 """Throughput of the sigmaSPH particle time step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
-                    [--workload tgv160|dambreak] [--no-cpu-baseline]
+                    [--workload dogbone|tgv160|dambreak] [--no-cpu-baseline]
 
-Workload (default): the 3-D weakly compressible Taylor-Green block, 160^3 = 4 096 000
-particles at dx = 1 mm in a periodic cube, +-0.05 dx seeded jitter, Tait c0 = 20 m/s,
-Wendland C2 with support 2.6 dx, delta-SPH 0.1, shifting on, CFL 0.2.  A "step" is one
-full particle time step (sort + the delta-plus-SPH substep + boundaries), inputs resident
-in HBM.  Prints ONE JSON line on stdout; diagnostics go to stderr.
+Workload (default, `dogbone`): the axisymmetric round-bar tension test -- the (x, r)
+half-plane of a steel bar at dx0 = 0.1 mm (10 mm heads 15 mm long, 6 mm gauge 30 mm
+long, 5 mm fillets; 26 880 particles), Johnson-Cook strength, linear EOS K = 160 GPa,
+G = 77 GPa, grips pulling the heads apart, the super-time-stepped hourglass damper,
+Colle shifting with ALE transport.  A "step" is one full HypoElasticSolver.step (load
+factor, sort, the solid substep with its hourglass sub-steps, grips, shifting,
+boundaries), inputs resident in HBM.  `tgv160` is the 160^3 = 4 096 000-particle periodic
+Taylor-Green fluid block (dx = 1 mm, Tait c0 = 20 m/s, support 2.6 dx, delta 0.1,
+shifting on); `dambreak` the 32 000-particle 3-D dam break.
+Prints ONE JSON line on stdout; diagnostics go to stderr.
 """
 import argparse
 import ctypes
@@ -15,6 +20,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -28,6 +35,9 @@ def log(*a):
 
 def make_scene(name):
     from sigmasph_amd import scenes
+    if name == "dogbone":
+        return (scenes.dogbone_c2(), "axisymmetric round-bar tensile dogbone, dx0=0.1 mm, "
+                "JC steel K=160 G=77 GPa, grips 1 mm/ms to 3 mm, hourglass STS")
     if name == "tgv160":
         return scenes.tgv_block(n=160), "taylor-green block 160^3, periodic, dx=1mm"
     if name == "dambreak":
@@ -44,22 +54,36 @@ def cpu_baseline(workload, precision, seconds=15.0):
     from sigmasph_amd.particle_system import SceneLayout
     from sigmasph_amd.DSPH import FluidSettings
     from sigmasph_amd import scenes
-    if workload == "tgv160":
-        sc, sample = scenes.tgv_block(n=48), "periodic taylor-green block 48^3 (110 592 particles), same numerics"
-    else:
-        sc, sample = scenes.dam_break_3d(), "the full 32 000-particle dam break"
-    lay = SceneLayout(SimConfig(sc))
-    fs = FluidSettings(lay)
-    P = fs.params()
-    s = O.OracleState.from_snapshot(lay.snapshot())
-    s.scal[0] = fs.dt_initial
     threads = min(os.cpu_count() or 1, 16)
     O.set_threads(threads)
-    O.fluid_step(lay.dims, P, s)          # warm-up (page-in, first sort)
+    if workload == "dogbone":
+        from sigmasph_amd.SOLID import SolidSettings
+        sc, sample = scenes.dogbone_c2(), "the full dogbone (26 880 particles), same numerics"
+        lay = SceneLayout(SimConfig(sc))
+        st = SolidSettings(lay)
+        P, S = st.params(), st.solid_params()
+        s = O.OracleState(lay.dims.n).make_solid(st.mat_table, st.ncons)
+        for k, v in lay.snapshot().items():
+            cur = getattr(s, k)
+            setattr(s, k, np.ascontiguousarray(np.asarray(v, dtype=cur.dtype).reshape(cur.shape)))
+        s.scal[0] = st.dt_initial
+        one = lambda: O.solid_step(lay.dims, P, S, s)
+    else:
+        if workload == "tgv160":
+            sc, sample = scenes.tgv_block(n=48), "periodic taylor-green block 48^3 (110 592 particles), same numerics"
+        else:
+            sc, sample = scenes.dam_break_3d(), "the full 32 000-particle dam break"
+        lay = SceneLayout(SimConfig(sc))
+        fs = FluidSettings(lay)
+        P = fs.params()
+        s = O.OracleState.from_snapshot(lay.snapshot())
+        s.scal[0] = fs.dt_initial
+        one = lambda: O.fluid_step(lay.dims, P, s)
+    one()                                 # warm-up (page-in, first sort)
     t0 = time.perf_counter()
     steps = 0
     while True:
-        O.fluid_step(lay.dims, P, s)
+        one()
         steps += 1
         el = time.perf_counter() - t0
         if el > seconds or steps >= 50:
@@ -75,7 +99,7 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
-    ap.add_argument("--workload", default="tgv160", choices=("tgv160", "dambreak"))
+    ap.add_argument("--workload", default="dogbone", choices=("dogbone", "tgv160", "dambreak"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -86,7 +110,7 @@ def main():
     import torch
     from sigmasph_amd.config import SimConfig
     from sigmasph_amd.particle_system import ParticleSystem
-    from sigmasph_amd.bytes_model import fluid_kernel_bytes
+    from sigmasph_amd.bytes_model import fluid_kernel_bytes, solid_kernel_bytes
     from sigmasph_amd import _native as N
 
     scene, desc = make_scene(args.workload)
@@ -107,20 +131,32 @@ def main():
         solver.step()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    kinds = ["pass_volume", "pass_force", "pass_shift", "sort"]
+    solid = ps.is_solid
+    kinds = ["pass_volume", "pass_force", "pass_shift", "sort", "solid_stress", "solid_force",
+             "hourglass_stage"]
     avg = {}
     for k, name in enumerate(kinds):
         ms = ctypes.c_double()
         cnt = ctypes.c_int64()
         N.check(lib.sph_timing_read(ps.ctx, k, ctypes.byref(ms), ctypes.byref(cnt)), ps.ctx, "timing")
-        avg[name] = (ms.value, cnt.value)
+        if cnt.value:
+            avg[name] = (ms.value, cnt.value)
     lib.sph_timing(ps.ctx, 0)
-    log("kernel averages (ms): " + ", ".join("%s %.3f (%d)" % (k, v[0], v[1]) for k, v in avg.items()))
+    log("kernel averages (ms): " + ", ".join("%s %.4f (%d)" % (k, v[0], v[1]) for k, v in avg.items()))
     ms_step = 1e3 * el / args.steps
     rate = n * args.steps / el
-    bm = fluid_kernel_bytes(args.precision, pst=bool(solver.pst_enabled),
-                            track_visc=bool(solver.track_visc_work))
-    dom = max(("pass_volume", "pass_force", "pass_shift"), key=lambda k: avg[k][0])
+    stages = None
+    if solid:
+        stages = int(ps.scal[N.SPH_S_STS_STAGES].item())
+        bm = solid_kernel_bytes(args.precision, two_d=ps.two_d, pst=bool(solver.settings.fs.pst_enabled),
+                                sts=bool(solver.hg_sts), stages=max(stages, 1))
+        pair_kinds = ("solid_stress", "solid_force", "hourglass_stage", "pass_shift")
+    else:
+        bm = fluid_kernel_bytes(args.precision, pst=bool(solver.pst_enabled),
+                                track_visc=bool(solver.track_visc_work))
+        pair_kinds = ("pass_volume", "pass_force", "pass_shift")
+    # the dominant kernel: the largest device time per step
+    dom = max((k for k in pair_kinds if k in avg), key=lambda k: avg[k][0] * avg[k][1])
     dur_s = avg[dom][0] * 1e-3
     achieved = bm[dom] * n / dur_s / 1e9 if dur_s > 0 else 0.0
     pairs = ps.neighbor_count() / n
@@ -138,7 +174,8 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32 pair sums + f64 integration",
         "data": "synthetic",
         "config": {"workload": desc, "particles": n, "precision": args.precision,
-                   "support": "2.6 dx", "parallelism": "single GPU"},
+                   "support": "3 dx0" if solid else "2.6 dx", "parallelism": "single GPU",
+                   "hourglass_stages_last_step": stages},
         "pair_interactions_per_s": (pairs * rate) if pairs else None,
         "hbm_fraction_step": bm["step"] * rate / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -37,3 +37,45 @@ def fluid_kernel_bytes(precision="fp64", pst=True, track_visc=False, cells_per_p
     out["finish"] = (4 * v3 + (s3 if pst else 0) + r) + 3 * v3
     out["step"] = sum(out.values())
     return out
+
+
+def solid_kernel_bytes(precision="fp64", two_d=True, pst=True, sts=True, stages=1,
+                       cells_per_particle=1.0 / 36.0):
+    """The same counting rule for the solid step.  A single-layer deck stores the full
+    3x3 grad v row but only its in-plane block and G22 are useful (5 values); the
+    deviator is 6 values (xx, yy, xy, zz, xz, yz) and 4 are useful in a single layer.
+    `stages` is the hourglass sub-step count of the step (the stage pass runs that often)."""
+    r = 8 if precision == "fp64" else 4
+    v3 = 24
+    s3 = 3 * r
+    gvn = 5 if two_d else 9
+    sgn = 4 if two_d else 6
+    cell = 4 * cells_per_particle
+    mixed = precision != "fp64"
+    out = {}
+    # pass SA: reads x, v, shift, V, rho, m, p, sig, eps_p, e_int, T, de_int, eos, [V_lo];
+    # writes lam, L6, grad v, sig', eps_p', e', T, div, V_new, rho_new, p, [V_lo]
+    rd = v3 + v3 + (s3 if pst else 0) + 4 * r + sgn * r + 4 * r + 4 + (r if mixed else 0) + 2 * cell
+    wr = r + 6 * r + gvn * r + sgn * r + 2 * r + r + 4 * r + (r if mixed else 0)
+    out["solid_stress"] = rd + wr
+    # pass SB: reads x, v, shift, V, V_new, rho_new, m, p, lam, L6, sig, eos, [grad v];
+    # writes acc, lam_grad, de_int
+    rd = v3 + v3 + (s3 if pst else 0) + 7 * r + 6 * r + sgn * r + 4 + (0 if sts else gvn * r) + 2 * cell
+    wr = s3 + s3 + r
+    out["solid_force"] = rd + wr
+    # one hourglass sub-step: reads x, y, grad v, V, m, eos; writes y
+    out["hourglass_stage"] = (v3 + v3 + gvn * r + 2 * r + 4 + 2 * cell) + v3 if sts else 0
+    # the grad v refresh: reads x, v, V; writes grad v
+    out["grad_v"] = (v3 + v3 + r + 2 * cell + gvn * r) if sts else 0
+    # kicks and drift (or the plain advect): reads v, acc, V_new, m, y, v_sync, u, x0;
+    # writes v, v_sync, y0, V, u, x
+    out["advect"] = (v3 + s3 + 2 * r + v3 + v3 + 2 * v3) + (v3 + v3 + v3 + r + 2 * v3) if sts else \
+        (s3 + 3 * v3 + r) + (3 * v3 + r)
+    # grips: reads flags (and, for grip particles only, ignored here) writes nothing else
+    out["constraints"] = 4 + v3
+    out["pass_shift"] = (v3 + r + r + v3 + r + s3 + 2 * cell + s3 + 4) if pst else 0
+    out["finish"] = (4 * v3 + (s3 if pst else 0) + r + s3) + 3 * v3
+    fields = 4 + 4 + 4 * v3 + s3 + 3 * r + (r if mixed else 0) + (sgn + 2) * r + 2 * r + 3 * 4 + s3
+    out["sort"] = (v3 + 4) + 3 * 2 * cell + (4 + 4) + (4 + 4 + 4) + 4 + 2 * fields
+    out["step"] = sum(v for k, v in out.items() if k != "hourglass_stage") + stages * out["hourglass_stage"]
+    return out

@@ -93,6 +93,19 @@ def cpu_baseline(workload, precision, seconds=15.0):
             "sample": "%s, %d steps in %.1f s" % (sample, steps, el)}
 
 
+def measured_traffic(workload, precision, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter runs
+    (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration), or None
+    when no counter run of this workload and precision is on record."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh).get("%s/%s" % (workload, precision))
+        return float(rec["kernels"][kernel]["bytes"]) if rec else None
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +172,7 @@ def main():
     dom = max((k for k in pair_kinds if k in avg), key=lambda k: avg[k][0] * avg[k][1])
     dur_s = avg[dom][0] * 1e-3
     achieved = bm[dom] * n / dur_s / 1e9 if dur_s > 0 else 0.0
+    traffic = measured_traffic(args.workload, args.precision, dom)
     pairs = ps.neighbor_count() / n
     out = {
         "metric": "particle-steps/s (1 GPU)",
@@ -179,7 +193,7 @@ def main():
         "pair_interactions_per_s": (pairs * rate) if pairs else None,
         "hbm_fraction_step": bm["step"] * rate / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_particle": bm[dom], "avg_ms": avg[dom][0]},
         "kernel_ms": {k: v[0] for k, v in avg.items()},
         "cpu_baseline": None,

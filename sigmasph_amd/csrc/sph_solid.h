@@ -32,6 +32,7 @@ struct DSolidP {
     double alpha_hg, hg_rate, inv_wdx0, inv_V0, ale_diff, ramp_time, damping, l_eig_tol;
     double hg_k;     // alpha / h: the damper's prefactor before rho0/rho c_p w_j/V0 W/W0
     int kc, jaumann, transport, transport_ep, sts, sts_stages, sts_max, sts_refresh;
+    int sts_bound;   // the most sub-steps any dt of the step can ask for (dt <= CFL h / c_signal)
     int dt_pair_mode, visc_work, thermal, plastic, ncons, hg_explicit;
 };
 
@@ -59,6 +60,8 @@ template <typename R> struct SolidArgs {
     // hourglass sub-step pass only
     const double4* ysrc; double4* ydst; int stage;
     __device__ bool any_out() const { return st.scal[SPH_S_OUT_OF_BOX] > 0.5; }
+    // a sub-step pass beyond this step's count has nothing to do
+    __device__ bool skip() const { return ydst != nullptr && stage >= (int)st.scal[SPH_S_STS_STAGES]; }
 };
 
 template <typename R> struct Sym6 { R xx, yy, xy, zz, xz, yz; };
@@ -308,6 +311,25 @@ template <typename R> struct StressBody {
                 R rho_bar = (R)0.5 * (rho_i + rho_j);
                 a_ediff += (R)(S.ale_diff * 0.5 * F.dx0) * speed * wj * pg * rho_bar * inv_rho_i;
             }
+        }
+    }
+
+    template <int LPP> __device__ void lane_reduce() {
+        if (LPP == 1) return;
+        const bool two_d = (a.F.flags & SPH_TWO_D) != 0;
+        e00 = lane_sum<LPP>(e00); e11 = lane_sum<LPP>(e11); e01 = lane_sum<LPP>(e01);
+        rate = lane_sum<LPP>(rate); drate = lane_sum<LPP>(drate);
+        G[0][0] = lane_sum<LPP>(G[0][0]); G[0][1] = lane_sum<LPP>(G[0][1]);
+        G[1][0] = lane_sum<LPP>(G[1][0]); G[1][1] = lane_sum<LPP>(G[1][1]);
+        as_xx = lane_sum<LPP>(as_xx); as_yy = lane_sum<LPP>(as_yy);
+        as_xy = lane_sum<LPP>(as_xy); as_zz = lane_sum<LPP>(as_zz);
+        a_ep = lane_sum<LPP>(a_ep); a_eadv = lane_sum<LPP>(a_eadv); a_ediff = lane_sum<LPP>(a_ediff);
+        if (!two_d) {
+            e22 = lane_sum<LPP>(e22); e02 = lane_sum<LPP>(e02); e12 = lane_sum<LPP>(e12);
+            G[0][2] = lane_sum<LPP>(G[0][2]); G[1][2] = lane_sum<LPP>(G[1][2]);
+            G[2][0] = lane_sum<LPP>(G[2][0]); G[2][1] = lane_sum<LPP>(G[2][1]);
+            G[2][2] = lane_sum<LPP>(G[2][2]);
+            as_xz = lane_sum<LPP>(as_xz); as_yz = lane_sum<LPP>(as_yz);
         }
     }
 
@@ -602,6 +624,20 @@ template <typename R> struct SolidForceBody {
         }
     }
 
+    template <int LPP> __device__ void lane_reduce() {
+        if (LPP == 1) return;
+        glx = lane_sum<LPP>(glx); gly = lane_sum<LPP>(gly); glz = lane_sum<LPP>(glz);
+        swx = lane_sum<LPP>(swx); swy = lane_sum<LPP>(swy); swz = lane_sum<LPP>(swz);
+        fjx = lane_sum<LPP>(fjx); fjy = lane_sum<LPP>(fjy); fjz = lane_sum<LPP>(fjz);
+        wsig = lane_sum<LPP>(wsig);
+        avx = lane_sum<LPP>(avx); avy = lane_sum<LPP>(avy); avz = lane_sum<LPP>(avz);
+        wmor = lane_sum<LPP>(wmor);
+        hx = lane_sum<LPP>(hx); hy = lane_sum<LPP>(hy); hz = lane_sum<LPP>(hz);
+        whg = lane_sum<LPP>(whg);
+        aax = lane_sum<LPP>(aax); aay = lane_sum<LPP>(aay); aaz = lane_sum<LPP>(aaz);
+        c2max = lane_max<LPP>(c2max);
+    }
+
     __device__ void end(int i) {
         const DFluid& F = a.F;
         const DSolidP& S = a.S;
@@ -723,6 +759,15 @@ template <typename R> struct GradVBody {
             if (!two_d) { e22 += sc * rz * rz; e02 += sc * rx * rz; e12 += sc * ry * rz; }
         }
     }
+    template <int LPP> __device__ void lane_reduce() {
+        if (LPP == 1) return;
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) G[p][q] = lane_sum<LPP>(G[p][q]);
+        e00 = lane_sum<LPP>(e00); e11 = lane_sum<LPP>(e11); e22 = lane_sum<LPP>(e22);
+        e01 = lane_sum<LPP>(e01); e02 = lane_sum<LPP>(e02); e12 = lane_sum<LPP>(e12);
+    }
     __device__ void end(int i) {
         const DFluid& F = a.F;
         const bool two_d = (F.flags & SPH_TWO_D) != 0;
@@ -798,6 +843,10 @@ template <typename R> struct HgBody {
         R k = wq<R>(F, b, b.V, j) * W * hg;
         hx += k * ex; hy += k * ey; hz += k * ez;
     }
+    template <int LPP> __device__ void lane_reduce() {
+        if (LPP == 1) return;
+        hx = lane_sum<LPP>(hx); hy = lane_sum<LPP>(hy); hz = lane_sum<LPP>(hz);
+    }
     __device__ void end(int i) {
         const DFluid& F = a.F;
         const DSolidP& S = a.S;
@@ -841,14 +890,16 @@ __global__ void k_solid_begin(double* scal, DSolidP S, double* cons, int ncons) 
 
 // compute_adaptive_dt (DSPH.py:1365-1430) with the solid's caps: clamp_dt_hourglass
 // (SOLID.py:4158-4161) or sts_stage_count (4510-4539); banks pass SA's works.
-__global__ __launch_bounds__(1024) void k_dt_solid(DFluid F, DSolidP S, double* scal, DScratch s, int nparts,
-                                                   int pstride) {
+__global__ __launch_bounds__(1024) void k_dt_solid(DFluid F, DSolidP S, double* scal, DScratch s, int nparts_a,
+                                                   int nparts_b, int pstride) {
     __shared__ double red[1024 / SPH_WAVE];
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const int slots[8] = {PART_WDIFF, PART_PLASTIC, PART_IEPAIR, PART_IEADV, PART_IEDIFF, PART_IEALE,
                           PART_VISCS, PART_HGW};
-    for (int k = threadIdx.x; k < nparts; k += blockDim.x)
-        for (int q = 0; q < 8; ++q) acc[q] += s.part[slots[q] * pstride + k];
+    for (int k = threadIdx.x; k < nparts_a; k += blockDim.x)
+        for (int q = 0; q < 6; ++q) acc[q] += s.part[slots[q] * pstride + k];
+    for (int k = threadIdx.x; k < nparts_b; k += blockDim.x)
+        for (int q = 6; q < 8; ++q) acc[q] += s.part[slots[q] * pstride + k];
     double t[8];
     for (int q = 0; q < 8; ++q) t[q] = block_sum(acc[q], red);
     if (threadIdx.x == 0) {
@@ -873,6 +924,7 @@ __global__ __launch_bounds__(1024) void k_dt_solid(DFluid F, DSolidP S, double* 
                 double want = S.hg_rate * dt;
                 st = S.sts_stages > 0 ? S.sts_stages : (int)fmax(1.0, ceil(want / 1.6));
                 st = min(st, S.sts_max);
+                st = min(st, S.sts_bound);   // never binds: dt <= CFL h / c_signal
                 double cap = st * 1.6 / S.hg_rate;
                 if (dt > cap) dt = cap;
             }
@@ -1105,6 +1157,69 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_solid_reduce(int n, DBuf<R> b, DS
     }
 }
 
+// ------------------------------------------------------------------------------------
+// the step's neighbour list
+//
+// Every pair pass of the solid step before the drift (SA, SB, the grad-v refresh and the
+// s hourglass sub-steps) visits the same pairs: the positions do not move between them.
+// One traversal writes them once, j | mirror << 30, column-major (entry k of particle i
+// at k * npad + i, so the lanes of a wave read consecutive words), in the traversal's own
+// order; the passes then read the list instead of filtering ~3x as many candidates.  A
+// particle with more than `nmax` neighbours keeps the generic traversal.
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SPH_BLOCK) void k_build_list(DGrid g, int n, const int* __restrict__ start,
+                                                          const int4* __restrict__ cellid,
+                                                          const double4* __restrict__ X, const double* scal,
+                                                          NList nl) {
+    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double4 xi = X[i];
+    int c = 0;
+    auto f = [&](int j, int mir, double, double, double) {
+        if (c < nl.nmax) nl.idx[(size_t)c * nl.npad + i] = j | (mir << 30);
+        ++c;
+    };
+    Traverse tr{g, start, cellid, X, scal[SPH_S_OUT_OF_BOX] > 0.5};
+    tr(i, xi.x, xi.y, xi.z, f);
+    nl.cnt[i] = c;
+}
+
+// One pair pass over the list: LPP consecutive lanes share a particle and take every
+// LPP-th entry; the group's sums are combined by shuffles and its first lane runs `end`.
+template <class Body, int LPP>
+__global__ __launch_bounds__(SPH_BLOCK) void k_list(DGrid g, int n, typename Body::Args a, NList nl) {
+    __shared__ double red[SPH_BLOCK / SPH_WAVE];
+    if (a.skip()) return;   // uniform over the grid
+    Body body(a);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / LPP, lane = t % LPP;
+    const bool act = i < n;
+    if (act) {
+        double xi, yi, zi;
+        body.begin(i, xi, yi, zi);
+        const int cnt = nl.cnt[i];
+        if (cnt <= nl.nmax) {
+            for (int k = lane; k < cnt; k += LPP) {
+                const int e = nl.idx[(size_t)k * nl.npad + i];
+                const int j = e & 0x3fffffff, mir = (e >> 30) & 3;
+                const double4 xj = a.X[j];
+                // as the traversal forms it: r = x_i - M x_j
+                double dx = min_image(g, 0, xi - xj.x);
+                double dy = (mir & 1) ? -(-yi - xj.y) : min_image(g, 1, yi - xj.y);
+                double dz = (mir & 2) ? -(-zi - xj.z) : min_image(g, 2, zi - xj.z);
+                body.pair(j, mir, dx, dy, dz);
+            }
+        } else if (lane == 0) {
+            auto f = [&](int j, int mir, double rx, double ry, double rz) { body.pair(j, mir, rx, ry, rz); };
+            Traverse tr{g, a.s.start, a.s.cellid, a.X, a.any_out()};
+            tr(i, xi, yi, zi, f);
+        }
+    }
+    body.template lane_reduce<LPP>();
+    if (act && lane == 0) body.end(i);
+    body.reduce(red, blockIdx.x);
+}
+
 // ====================================================================================
 // host side of the solid step
 // ====================================================================================
@@ -1127,13 +1242,25 @@ static DSolidP make_solid_params(const sph_ctx* c, const sph_fluid_params* P, co
     d.hg_explicit = !S->sts && S->hourglass > 0.0;
     d.sts_stages = S->sts_stages;
     d.sts_max = S->sts_max_stages;
+    d.sts_bound = 1;
+    if (d.sts && S->hg_rate > 0.0) {
+        if (S->sts_stages > 0) {
+            d.sts_bound = std::min(S->sts_stages, S->sts_max_stages);
+        } else {
+            // the device's acoustic limit with c^2 at its floor c_signal^2, in the same
+            // operations: every dt of a step is at most this
+            double csig2 = P->c_signal * P->c_signal;
+            double dt_max = P->cfl * c->dims.h / sqrt(csig2);
+            double want = S->hg_rate * dt_max;
+            d.sts_bound = std::min(S->sts_max_stages, (int)fmax(1.0, ceil(want / 1.6)));
+        }
+    }
     d.sts_refresh = S->sts_refresh;
     d.dt_pair_mode = S->dt_pair_mode;
     d.visc_work = S->visc_work;
     d.thermal = S->thermal;
     d.plastic = S->plastic;
     d.ncons = S->ncons;
-    (void)c;
     return d;
 }
 
@@ -1235,43 +1362,58 @@ extern "C" int sph_solid_sort(sph_ctx* c, sph_state* st, sph_solid* sd, void* st
     return SPH_OK;
 }
 
+// one list pass; returns the number of workgroups (the partial-sum slots it wrote)
+template <class Body>
+static int launch_list_pass(sph_ctx* c, const typename Body::Args& a, hipStream_t hs, int kind) {
+    const int n = (int)c->dims.n;
+    const int lpp = c->lpp[kind];
+    const int blocks = (int)(((int64_t)n * lpp + SPH_BLOCK - 1) / SPH_BLOCK);
+    dim3 gb(blocks), bb(SPH_BLOCK);
+    switch (lpp) {
+    case 8: hipLaunchKernelGGL((k_list<Body, 8>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
+    case 4: hipLaunchKernelGGL((k_list<Body, 4>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
+    case 2: hipLaunchKernelGGL((k_list<Body, 2>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
+    default: hipLaunchKernelGGL((k_list<Body, 1>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
+    }
+    return blocks;
+}
+
 template <typename R>
 static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F, const DSolidP& S,
                       hipStream_t hs) {
     const int nb = c->nblk;
     int rc;
+    // the neighbour list of the step's positions
+    hipLaunchKernelGGL(k_build_list, dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, (int)c->dims.n,
+                       (const int*)c->s.start, (const int4*)c->s.cellid, (const double4*)st->buf[st->cur].x,
+                       (const double*)st->scal, c->nl);
     // pass SA: reads history[hcur], writes history[1 - hcur]
     timer_begin(c, 4, hs);
-    launch_pair_pass<StressBody<R>>(c, solid_args<R>(c, st, sd, F, S), hs);
+    int slots_a = launch_list_pass<StressBody<R>>(c, solid_args<R>(c, st, sd, F, S), hs, PASS_STRESS);
     timer_end(c, 4, hs);
     if ((rc = check_launch(c, "solid stress pass"))) return rc;
     sd->hcur = 1 - sd->hcur;
     // pass SB on the updated stress, volume and pressure
     timer_begin(c, 5, hs);
-    launch_pair_pass<SolidForceBody<R>>(c, solid_args<R>(c, st, sd, F, S), hs);
+    int slots_b = launch_list_pass<SolidForceBody<R>>(c, solid_args<R>(c, st, sd, F, S), hs, PASS_SFORCE);
     timer_end(c, 5, hs);
-    hipLaunchKernelGGL(k_dt_solid, dim3(1), dim3(1024), 0, hs, F, S, st->scal, c->s, c->tiles.ntiles + c->nblk,
+    hipLaunchKernelGGL(k_dt_solid, dim3(1), dim3(1024), 0, hs, F, S, st->scal, c->s, slots_a, slots_b,
                        c->pstride);
     if ((rc = check_launch(c, "solid force pass"))) return rc;
     if (S.sts) {
-        // the stage count is a device number: read it back to size the sub-step loop
-        HIPCHECK(c, hipMemcpyAsync(c->host_stages, (const char*)st->scal + sizeof(double) * SPH_S_STS_STAGES,
-                                   sizeof(double), hipMemcpyDeviceToHost, hs));
         hipLaunchKernelGGL((k_sts_kick1<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, dstate<R>(c, st), dsolid<R>(st, sd),
                            c->s, c->pstride);
-        if (S.sts_refresh) launch_pair_pass<GradVBody<R>>(c, solid_args<R>(c, st, sd, F, S), hs);
+        if (S.sts_refresh) launch_list_pass<GradVBody<R>>(c, solid_args<R>(c, st, sd, F, S), hs, PASS_GRADV);
         if ((rc = check_launch(c, "solid half kick"))) return rc;
-        HIPCHECK(c, hipStreamSynchronize(hs));
-        int stages = (int)(*(const double*)c->host_stages);
-        if (stages < 1 || stages > S.sts_max)
-            return set_err(c, SPH_EINVAL, "hourglass sub-step count out of range%s: %lld", "", stages);
+        // the count is a device number (k_dt_solid); launch the most any dt can need and
+        // let the passes beyond it return at once, so the host never waits on the device
         SolidArgs<R> a = solid_args<R>(c, st, sd, F, S);
-        for (int k = 0; k < stages; ++k) {
+        for (int k = 0; k < S.sts_bound; ++k) {
             a.ysrc = (k & 1) ? a.sd.y1 : a.sd.y0;
             a.ydst = (k & 1) ? a.sd.y0 : a.sd.y1;
             a.stage = k;
             timer_begin(c, 6, hs);
-            launch_pair_pass<HgBody<R>>(c, a, hs);
+            launch_list_pass<HgBody<R>>(c, a, hs, PASS_HG);
             timer_end(c, 6, hs);
         }
         hipLaunchKernelGGL((k_sts_kick2<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st),
@@ -1287,7 +1429,7 @@ static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F,
         PassArgs<R> pa = pass_args<R>(c, st, F);
         pa.bcf = sd->bc_flag[st->cur];
         timer_begin(c, 2, hs);
-        launch_pair_pass<ShiftBody<R>>(c, pa, hs);
+        launch_pair_pass<ShiftBody<R>>(c, pa, hs, PASS_SHIFT);
         timer_end(c, 2, hs);
     }
     hipLaunchKernelGGL((k_finish<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st), c->s,
@@ -1316,7 +1458,19 @@ extern "C" int sph_solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const sp
         HIPCHECK(c, hipMalloc((void**)&c->cons_part, sizeof(double) * 3 * SPH_MAX_CONS * (size_t)c->nblk));
         c->cons_cap = SPH_MAX_CONS;
     }
-    if (!c->host_stages) HIPCHECK(c, hipHostMalloc((void**)&c->host_stages, sizeof(double)));
+    if (!c->nl.idx) {
+        // room for twice the lattice's neighbour count (plus the images near a plane)
+        const sph_dims& d = c->dims;
+        double q = d.dx0 > 0.0 ? d.h / d.dx0 : 2.6;
+        double expect = (d.flags & SPH_TWO_D) ? M_PI * q * q : 4.0 / 3.0 * M_PI * q * q * q;
+        int nmax = (int)(2.0 * expect) + 16;
+        nmax = (nmax + 7) / 8 * 8;
+        int npad = (int)((d.n + 63) / 64 * 64);
+        HIPCHECK(c, hipMalloc((void**)&c->nl.idx, sizeof(int32_t) * (size_t)nmax * npad));
+        HIPCHECK(c, hipMalloc((void**)&c->nl.cnt, sizeof(int32_t) * (size_t)npad));
+        c->nl.nmax = nmax;
+        c->nl.npad = npad;
+    }
     DFluid F = make_fluid(c, P);
     DSolidP D = make_solid_params(c, P, S);
     hipLaunchKernelGGL(k_solid_begin, dim3(1), dim3(64), 0, hs, st->scal, D, sd->cons, S->ncons);

@@ -88,6 +88,7 @@ extern "C" {
 /* ---- materials of the solid step (materials.py:96-97, 506-548) -------------------- */
 #define SPH_EOS_LINEAR       0  /* p = K (rho/rho0 - 1): Tait at gamma = 1, S = K      */
 #define SPH_EOS_TAIT         1  /* p = S ((rho/rho0)^gamma - 1)                         */
+#define SPH_EOS_MG           3  /* polynomial Mie-Grueneisen (EOS003_mie_gruneisen.py)   */
 #define SPH_PLASTIC_NONE     0
 #define SPH_PLASTIC_LINEAR   1  /* J2 with linear isotropic hardening (MAT001)          */
 #define SPH_PLASTIC_JC       2  /* Johnson-Cook (MAT002)                                */
@@ -179,6 +180,9 @@ typedef struct {
     double rho0, K, G, c_p, c0, sigma_y0, hardening, cv, T0, e0;
     double eos_S, eos_gamma, delta;      /* EOS stiffness and exponent; delta-SPH coefficient */
     double jc_A, jc_B, jc_n, jc_C, jc_eps0_dot, jc_T0, jc_Tm, jc_m;
+    /* Mie-Grueneisen: P = C0 + C1 mu + C2 mu^2 + C3 mu^3 + (C4 + C5 mu) rho0 e */
+    double mg_C0, mg_C1, mg_C2, mg_C3, mg_C4, mg_C5, mg_c0_ref;
+    double mg_linear_exp;                /* 1: C2 = C3 = 0 for mu < 0              */
     int32_t eos_kind;                    /* SPH_EOS_*     */
     int32_t plastic_kind;                /* SPH_PLASTIC_* */
 } sph_material;
@@ -195,6 +199,9 @@ typedef struct {
     double ramp_time;        /* Load.rampTime (SOLID.py:5001-5017)                          */
     double damping;          /* dampingCoefficient (SOLID.py:729)                           */
     double l_eig_tol;        /* L_EIG_TOL of the corrected grad v (SOLID.py:361)            */
+    double q_l2;             /* C_Q l^2 of the VNR bulk viscosity, 0 = off (SOLID.py:667-676)*/
+    double q_cl;             /* C_Q l, its signal-speed factor (SOLID.py:3938-3974)         */
+    double av_alpha, av_beta, av_eps;   /* Monaghan's Pi_ij (SOLID.py:709-712)           */
     int32_t kernel_correction;
     int32_t jaumann;         /* OBJECTIVE_RATE "jaumann" (1) or "none" (0)                  */
     int32_t ale_history;     /* PST_ALE_HISTORY: transport s, eps_p and e with the shift    */
@@ -217,7 +224,7 @@ typedef struct {
  *    sph_sort gathers buf[hcur] into buf[1-hcur]; the stress pass reads buf[hcur] and
  *    writes the updated values to buf[1-hcur]; each flips `hcur`.
  *  - sorted fields T, de_int, eos_id, bc_*: two buffers that follow sph_state.cur.
- *  - grad_v, hg_y, v_sync: derived within a step. */
+ *  - grad_v, q_visc, hg_y, v_sync: derived within a step. */
 typedef struct {
     void*    sig[2];         /* [n][6] deviatoric stress xx,yy,zz,xy,xz,yz (SOLID.py:421)   */
     void*    eps_p[2];       /* [n] equivalent plastic strain                               */
@@ -230,7 +237,8 @@ typedef struct {
     int32_t* bc_flag[2];     /* [n] constrained components, bit k = axis k                  */
     int32_t* bc_id[2];       /* [n] constraint index, -1 for none                           */
     void*    bc_disp[2];     /* [n][4] prescribed displacement at load factor 1             */
-    void*    grad_v;         /* [n][9] velocity gradient G[a][b] = dv_a/dx_b, row-major     */
+    void*    grad_v;         /* [n][10] velocity gradient G00 G01 G10 G11 G22 G02 G12 G20 G21 */
+    void*    q_visc;         /* [n] VNR bulk viscosity q (read when q_l2 > 0)               */
     double*  hg_y[2];        /* [n][4] hourglass sub-step velocities                        */
     double*  v_sync;         /* [n][4] velocity after the first half kick                   */
     const sph_material* mat; /* device [nmat]                                               */

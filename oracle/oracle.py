@@ -33,7 +33,7 @@ class OrState(ctypes.Structure):
                 ("eps_p", _dp), ("e_int", _dp), ("T", _dp), ("de_int", _dp), ("de_hg", _dp),
                 ("eos_id", _ip), ("bc_flag", _ip), ("bc_id", _ip),
                 ("bc_disp", _dp), ("grad_v", _dp), ("cons", _dp),
-                ("mat", ctypes.c_void_p)]
+                ("mat", ctypes.c_void_p), ("q", _dp), ("av_mu", _dp)]
 
 
 def build():
@@ -89,6 +89,10 @@ def lib():
         L.oracle_j2_return.restype = None
         L.oracle_jc_return.argtypes = [_dp] + [ctypes.c_double] * 12 + [_dp, _ip]
         L.oracle_jc_return.restype = None
+        L.oracle_mg_pressure.argtypes = [vp, ctypes.c_double, ctypes.c_double, _dp]
+        L.oracle_mg_pressure.restype = ctypes.c_double
+        L.oracle_cold_energy.argtypes = [vp, ctypes.c_double, ctypes.c_int]
+        L.oracle_cold_energy.restype = ctypes.c_double
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_max_threads.restype = ctypes.c_int
         _lib = L
@@ -105,7 +109,7 @@ class OracleState:
     FIELDS3 = ("x", "x0", "u", "v", "shift", "acc", "lam_grad")
     FIELDS1 = ("V", "m", "p", "lam", "div", "de_visc")
     SOLID_D = {"sig": 6, "eps_p": 0, "e_int": 0, "T": 0, "de_int": 0, "de_hg": 0,
-               "bc_disp": 3, "grad_v": 9}
+               "bc_disp": 3, "grad_v": 9, "q": 0, "av_mu": 0}
     SOLID_I = ("eos_id", "bc_flag", "bc_id")
 
     def __init__(self, n):
@@ -272,6 +276,17 @@ def spd_inverse2(E2, tol):
 
 def spd_weight(e, tol):
     return lib().oracle_spd_weight(float(e), float(tol))
+
+
+def mg_pressure(mat, rho, e):
+    """(p, c^2) of a Mie-Grueneisen material row (an SphMaterial)."""
+    c2 = ctypes.c_double(0.0)
+    p = lib().oracle_mg_pressure(ctypes.byref(mat), rho, e, ctypes.byref(c2))
+    return p, c2.value
+
+
+def cold_energy(mat, rho, allow_negative_pressure=True):
+    return lib().oracle_cold_energy(ctypes.byref(mat), rho, int(bool(allow_negative_pressure)))
 
 
 def tait_energy(rho, rho0, S, gamma):

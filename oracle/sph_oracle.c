@@ -50,6 +50,8 @@ typedef struct {
     double *grad_v;                      /* [n][9] row-major, G[a][b] = dv_a/dx_b */
     double *cons;                        /* [ncons][6] grip force (3), grip displacement (3) */
     const sph_material* mat;             /* [nmat] */
+    double *q;                           /* [n] VNR bulk viscosity */
+    double *av_mu;                       /* [n] max_j |mu_ij| of Monaghan's Pi */
 } or_state;
 
 typedef struct {
@@ -1357,13 +1359,25 @@ static void or_internal_energy(const sph_dims* d, const sph_fluid_params* P,
     free(rate);
 }
 
-/* cold_specific_energy (SOLID.py:2511-2567) for the Tait/linear rows */
-static double or_cold_energy(const sph_fluid_params* P, const or_state* s, int64_t i) {
-    const sph_material* M = or_mat(s, i);
-    double rho = s->m[i] / s->V[i];
-    double e = oracle_tait_energy(rho, M->rho0, M->eos_S, M->eos_gamma);
-    if (!P->allow_negative_pressure && rho < M->rho0) e = 0.0;
+/* cold_specific_energy (SOLID.py:2511-2567) */
+double oracle_cold_energy(const sph_material* M, double rho, int allow_negative_pressure) {
+    double e;
+    if (M->eos_kind == SPH_EOS_MG) {         /* mg_cold_specific_energy (EOS003:322-331) */
+        double c2 = M->mg_C2, c3 = M->mg_C3;
+        if (rho < M->rho0 && M->mg_linear_exp > 0.5) c2 = c3 = 0.0;
+        double x = rho / M->rho0;
+        double A = c3, B = c2 - 3.0 * c3, D = M->mg_C1 - 2.0 * c2 + 3.0 * c3;
+        double E = M->mg_C0 - M->mg_C1 + c2 - c3;
+        e = (0.5 * A * (x * x - 1.0) + B * (x - 1.0) + D * log(x) + E * (1.0 - 1.0 / x)) / M->rho0;
+    } else {
+        e = oracle_tait_energy(rho, M->rho0, M->eos_S, M->eos_gamma);
+    }
+    if (!allow_negative_pressure && rho < M->rho0) e = 0.0;
     return e;
+}
+
+static double or_cold_energy(const sph_fluid_params* P, const or_state* s, int64_t i) {
+    return oracle_cold_energy(or_mat(s, i), s->m[i] / s->V[i], P->allow_negative_pressure);
 }
 
 /* update_temperature (SOLID.py:2239-2293) */
@@ -1377,10 +1391,37 @@ static void or_temperature(const sph_fluid_params* P, or_state* s) {
     }
 }
 
-/* eos_pressure_i for the linear/Tait branch (SOLID.py:3009-3038, 3142-3181) */
-static double or_solid_pressure(const sph_fluid_params* P, const sph_material* M, double rho) {
-    double p = M->eos_kind == SPH_EOS_LINEAR ? M->eos_S * (rho / M->rho0 - 1.0)
-                                             : M->eos_S * (pow(rho / M->rho0, M->eos_gamma) - 1.0);
+/* mie_gruneisen_pressure and _sound_speed_sq (EOS003_mie_gruneisen.py:223-270) */
+static double or_mg_pressure(const sph_material* M, double rho, double e, double* c_sq) {
+    double mu = rho / M->rho0 - 1.0;
+    double ev = M->rho0 * e;
+    double pt = (M->mg_C4 + M->mg_C5 * mu) * ev, pc, dpc;
+    if (mu < 0.0 && M->mg_linear_exp > 0.5) { pc = M->mg_C0 + M->mg_C1 * mu; dpc = M->mg_C1; }
+    else {
+        pc = M->mg_C0 + mu * (M->mg_C1 + mu * (M->mg_C2 + mu * M->mg_C3));
+        dpc = M->mg_C1 + mu * (2.0 * M->mg_C2 + 3.0 * M->mg_C3 * mu);
+    }
+    double p = pc + pt;
+    if (c_sq) {
+        double dpr = (1.0 / M->rho0) * dpc + M->mg_C5 * e;
+        double dpe = (M->mg_C4 + M->mg_C5 * mu) * M->rho0;
+        double c2 = dpr + (p / (rho * rho)) * dpe;
+        double cf = 1e-3 * M->mg_c0_ref;
+        *c_sq = fmax(c2, cf * cf);
+    }
+    return p;
+}
+
+double oracle_mg_pressure(const sph_material* M, double rho, double e, double* c_sq) {
+    return or_mg_pressure(M, rho, e, c_sq);
+}
+
+/* eos_pressure_i (SOLID.py:3009-3038, 3142-3181) with e_for_eos = max(e, 0) (3075-3109) */
+static double or_solid_pressure(const sph_fluid_params* P, const sph_material* M, double rho, double e) {
+    double p;
+    if (M->eos_kind == SPH_EOS_MG) p = or_mg_pressure(M, rho, fmax(e, 0.0), NULL);
+    else p = M->eos_kind == SPH_EOS_LINEAR ? M->eos_S * (rho / M->rho0 - 1.0)
+                                           : M->eos_S * (pow(rho / M->rho0, M->eos_gamma) - 1.0);
     if (!P->allow_negative_pressure) p = fmax(p, 0.0);
     return p;
 }
@@ -1402,10 +1443,12 @@ static void or_stress_task(void* c, int64_t i, int64_t j, int mir, const double*
     or_sym6(s->sig + 6 * i, Si);
     or_sym6(s->sig + 6 * j, Sj);
     or_mirror_mat(mir, Sj);
+    double pti = s->p[i], ptj = s->p[j];
+    if (S->q_l2 > 0.0) { pti += s->q[i]; ptj += s->q[j]; }
     double sg[3][3];
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b)
-            sg[a][b] = (-s->p[i] * (a == b) + Si[a][b]) + (-s->p[j] * (a == b) + Sj[a][b]);
+            sg[a][b] = (-pti * (a == b) + Si[a][b]) + (-ptj * (a == b) + Sj[a][b]);
     double wi = or_w(d, P, s, i), wj = or_w(d, P, s, j), mwi = or_mw(d, P, s, i);
     double f[3];
     for (int a = 0; a < 3; ++a) f[a] = wi * wj * (sg[a][0] * gW[0] + sg[a][1] * gW[1] + sg[a][2] * gW[2]);
@@ -1423,6 +1466,23 @@ static void or_stress_task(void* c, int64_t i, int64_t j, int mir, const double*
         double wm = -0.5 * (am[0] * vij[0] + am[1] * vij[1] + am[2] * vij[2]);
         t->acc[3] += wm;
         if (S->visc_work) t->acc[4] += -wm;
+    }
+    if (S->av_alpha > 0.0 || S->av_beta > 0.0) {      /* Monaghan's Pi_ij (SOLID.py:3711-3765) */
+        double vr = vij[0] * r[0] + vij[1] * r[1] + vij[2] * r[2];
+        if (vr < 0.0) {
+            double h = d->h;
+            double mu = h * vr / (r2 + S->av_eps * h * h);
+            double rho_bar = 0.5 * (s->m[i] / s->V[i] + s->m[j] / s->V[j]);
+            double c_bar = 0.5 * (or_mat(s, i)->c_p + or_mat(s, j)->c_p);
+            double pi = (-S->av_alpha * c_bar * mu + S->av_beta * mu * mu) / rho_bar;
+            double mwj = or_mw(d, P, s, j);
+            double ap[3] = { -mwj * pi * gW[0], -mwj * pi * gW[1], -mwj * pi * gW[2] };
+            for (int k = 0; k < 3; ++k) t->acc[k] += ap[k];
+            if (-mu > t->acc[6]) t->acc[6] = -mu;
+            double wp = -0.5 * (ap[0] * vij[0] + ap[1] * vij[1] + ap[2] * vij[2]);
+            t->acc[3] += wp;
+            if (S->visc_work) t->acc[4] += -wp;
+        }
     }
     if (S->hourglass > 0.0 && !S->sts) {
         double rn = sqrt(r2);
@@ -1472,7 +1532,13 @@ static void or_stress_accelerations(const sph_dims* d, const sph_fluid_params* P
     double dt = s->scal[SPH_S_DT];
     for (int64_t i = 0; i < n; ++i) {
         for (int k = 0; k < 3; ++k) s->acc[3 * i + k] = P->g[k];
-        s->p[i] = or_solid_pressure(P, or_mat(s, i), s->m[i] / s->V[i]);
+        double rho = s->m[i] / s->V[i];
+        s->p[i] = or_solid_pressure(P, or_mat(s, i), rho, s->e_int[i]);
+        if (S->q_l2 > 0.0) {             /* VNR q from tr(grad v) (SOLID.py:3392-3407) */
+            const double* G = s->grad_v + 9 * i;
+            double dv = G[0] + G[4] + G[8];
+            s->q[i] = dv < 0.0 ? S->q_l2 * rho * dv * dv : 0.0;
+        }
         s->de_int[i] = 0.0; s->de_hg[i] = 0.0; s->de_visc[i] = 0.0;
     }
     #pragma omp parallel for schedule(dynamic, 256)
@@ -1493,6 +1559,7 @@ static void or_stress_accelerations(const sph_dims* d, const sph_fluid_params* P
         s->de_int[i] = t.acc[3];
         s->de_visc[i] = t.acc[4];
         s->de_hg[i] = t.acc[5];
+        if (s->av_mu) s->av_mu[i] = t.acc[6];
     }
 }
 
@@ -1633,6 +1700,39 @@ static void or_load_factor(const sph_solid_params* S, or_state* s) {
     s->scal[SPH_S_LOAD_SDOT] = 6.0 * u * (1.0 - u) / S->ramp_time;
 }
 
+/* clamp_dt_mie_gruneisen, clamp_dt_bulk_viscosity, clamp_dt_monaghan (SOLID.py:3886-4009) */
+static void or_solid_clamps(const sph_dims* d, const sph_fluid_params* P, const sph_solid_params* S,
+                            or_state* s) {
+    double mg = 0.0, qc = 0.0, av = 0.0;
+    for (int64_t i = 0; i < s->n; ++i) {
+        const sph_material* M = or_mat(s, i);
+        double rho = s->m[i] / s->V[i];
+        if (M->eos_kind == SPH_EOS_MG) {
+            double c2;
+            or_mg_pressure(M, rho, fmax(s->e_int[i], 0.0), &c2);
+            if (M->G > 0.0) c2 += (4.0 / 3.0) * M->G / rho;
+            if (c2 > mg) mg = c2;
+        }
+        if (S->q_l2 > 0.0) {
+            const double* G = s->grad_v + 9 * i;
+            double dv = G[0] + G[4] + G[8];
+            if (dv < 0.0) {
+                double q = S->q_cl * (-dv);
+                double c = q + sqrt(q * q + M->c_p * M->c_p);
+                if (c > qc) qc = c;
+            }
+        }
+        if ((S->av_alpha > 0.0 || S->av_beta > 0.0) && s->av_mu[i] > 0.0) {
+            double c = M->c_p + 1.2 * (S->av_alpha * M->c_p + S->av_beta * s->av_mu[i]);
+            if (c > av) av = c;
+        }
+    }
+    double* dt = &s->scal[SPH_S_DT];
+    if (mg > 0.0) *dt = fmin(*dt, P->cfl * d->h / sqrt(mg));
+    if (qc > 0.0) *dt = fmin(*dt, P->cfl * d->h / qc);
+    if (av > 0.0) *dt = fmin(*dt, P->cfl * d->h / av);
+}
+
 /* HypoElasticSolver.step (SOLID.py:5227-5231): load factor, SPHBase.step (sort, substep,
  * boundaries), then the clock. */
 void oracle_solid_step(const sph_dims* d, const sph_fluid_params* P, const sph_solid_params* S,
@@ -1647,6 +1747,7 @@ void oracle_solid_step(const sph_dims* d, const sph_fluid_params* P, const sph_s
     or_volume(d, P, s, &g_grid);
     or_stress_accelerations(d, P, S, s, &g_grid);
     or_adaptive_dt(d, P, s, &g_grid, S);
+    or_solid_clamps(d, P, S, s);
     int st = 0;
     if (!S->sts && S->hourglass > 0.0) {           /* clamp_dt_hourglass (4158-4161) */
         double cap = 1.6 / S->hg_rate;

@@ -5,17 +5,18 @@ Same construction, configuration reading and public methods as the reference's
 `sim_time`, `get_constraint_data()`, `energy_budget()` and the energy accessors.
 `step()` is one call into libsigmasph.so (`sph_solid_step`).
 
-The feature set is the one the tensile and plane-strain decks use: linear and Tait
-equations of state, J2 (linear hardening) and Johnson-Cook return maps, the Jaumann
-rate, ALE transport of the deviator, the plastic strain and the internal energy under
-the shift (with the upwind energy diffusion), Morris viscosity, the rate hourglass
-damper (explicit with its dt cap, or super-time-stepped in subcycle mode), the load
-ramp and displacement grips, axisymmetry and plane strain.  Everything else the
-reference offers on this class raises NotImplementedError when a scene asks for it:
-JWL / Mie-Grueneisen / HJC, damage and erosion, the VNR and Monaghan viscosities,
-the conservative axisymmetric forms, the viscous / adaptive / saturating hourglass
-forms, RKL super-time-stepping, the velocity limiters, and a table whose materials do
-not share one EOS sound speed and one acoustic impedance.
+The feature set is the one the tensile, plane-strain and impact decks use: linear,
+Tait and polynomial Mie-Grueneisen equations of state, J2 (linear hardening) and
+Johnson-Cook return maps, the Jaumann rate, ALE transport of the deviator, the plastic
+strain and the internal energy under the shift (with the upwind energy diffusion),
+Morris viscosity, the VNR bulk viscosity and Monaghan's pair viscosity with their dt
+limits, the rate hourglass damper (explicit with its dt cap, or super-time-stepped in
+subcycle mode), the load ramp and displacement grips, axisymmetry and plane strain.
+Everything else the reference offers on this class raises NotImplementedError when a
+scene asks for it: JWL / HJC, damage and erosion, the conservative axisymmetric forms,
+the viscous / adaptive / saturating hourglass forms, RKL super-time-stepping, the
+velocity limiters, and a table whose materials do not share one EOS sound speed and
+one acoustic impedance.
 """
 import ctypes
 import math
@@ -116,8 +117,19 @@ class SolidSettings:
         self.pst_ale_diffusion = 1.0 if ad is None else float(ad)
         mode = str(c("DT_PAIR_SOUND_SPEED") or "samematerial").lower()
         self.dt_pair_mode = {"all": 0, "samematerial": 1, "off": 2}[mode]
+        # VNR q and Monaghan's Pi (SOLID.py:660-712)
+        self.q_bulk = float(c("BULK_VISCOSITY_Q") or 0.0)
+        qlen = str(c("BULK_VISCOSITY_LENGTH") or "dx0").lower()
+        self.q_length = self.dx0 if qlen == "dx0" else ps.support_radius
+        self.q_l2 = self.q_bulk * self.q_length ** 2
+        self.av_alpha = float(c("MONAGHAN_ALPHA") or 0.0)
+        self.av_beta = float(c("MONAGHAN_BETA") or 0.0)
+        eps = c("MONAGHAN_EPS")
+        self.av_eps = 0.01 if eps is None else float(eps)
+        self.av_on = self.av_alpha > 0.0 or self.av_beta > 0.0
         track = c("TRACK_VISCOUS_WORK")
-        self.solid_visc_work_on = (True if track is None else bool(track)) and self.fs.viscosity > 0.0
+        self.solid_visc_work_on = ((True if track is None else bool(track))
+                                   and (self.fs.viscosity > 0.0 or self.av_on))
         self.thermal_on = any(m.cv > 0.0 for m in mats)
         self.plastic = any(m.plastic_kind != N.PLASTIC_NONE for m in mats)
         self.ncons = len(getattr(ps, "constraints", []) or [])
@@ -135,6 +147,8 @@ class SolidSettings:
         S.hourglass, S.hg_rate, S.w_dx0, S.V0 = self.hourglass, self.hg_rate, self.w_dx0, self.ps.V0
         S.ale_diffusion, S.ramp_time, S.damping = self.pst_ale_diffusion, self.ramp_time, self.damping
         S.l_eig_tol = self.l_eig_tol
+        S.q_l2, S.q_cl = self.q_l2, self.q_bulk * self.q_length
+        S.av_alpha, S.av_beta, S.av_eps = self.av_alpha, self.av_beta, self.av_eps
         S.kernel_correction = int(self.kernel_correction)
         S.jaumann = int(self.jaumann)
         S.ale_history = int(self.pst_ale_history)

@@ -57,7 +57,7 @@ SPH_R_KINETIC_OFFGRID = 6
 SPH_R_INTERNAL = 7
 SPH_R_ELASTIC = 8
 # material kinds
-EOS_LINEAR, EOS_TAIT = 0, 1
+EOS_LINEAR, EOS_TAIT, EOS_MG = 0, 1, 3
 PLASTIC_NONE, PLASTIC_LINEAR, PLASTIC_JC = 0, 1, 2
 
 _d3 = ctypes.c_double * 3
@@ -141,14 +141,15 @@ class SphMaterial(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in (
         "rho0", "K", "G", "c_p", "c0", "sigma_y0", "hardening", "cv", "T0", "e0",
         "eos_S", "eos_gamma", "delta",
-        "jc_A", "jc_B", "jc_n", "jc_C", "jc_eps0_dot", "jc_T0", "jc_Tm", "jc_m")] + [
+        "jc_A", "jc_B", "jc_n", "jc_C", "jc_eps0_dot", "jc_T0", "jc_Tm", "jc_m",
+        "mg_C0", "mg_C1", "mg_C2", "mg_C3", "mg_C4", "mg_C5", "mg_c0_ref", "mg_linear_exp")] + [
         ("eos_kind", ctypes.c_int32), ("plastic_kind", ctypes.c_int32)]
 
 
 class SphSolidParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in (
         "hourglass", "hg_rate", "w_dx0", "V0", "ale_diffusion", "ramp_time", "damping",
-        "l_eig_tol")] + [(n, ctypes.c_int32) for n in (
+        "l_eig_tol", "q_l2", "q_cl", "av_alpha", "av_beta", "av_eps")] + [(n, ctypes.c_int32) for n in (
         "kernel_correction", "jaumann", "ale_history", "sts", "sts_stages", "sts_max_stages",
         "sts_refresh", "dt_pair_mode", "visc_work", "thermal", "plastic", "nmat", "ncons",
         "pad_")]
@@ -164,6 +165,7 @@ class SphSolid(ctypes.Structure):
         ("T", _vp2), ("de_int", _vp2), ("eos_id", _vp2), ("bc_flag", _vp2),
         ("bc_id", _vp2), ("bc_disp", _vp2),
         ("grad_v", ctypes.c_void_p),
+        ("q_visc", ctypes.c_void_p),
         ("hg_y", _vp2),
         ("v_sync", ctypes.c_void_p),
         ("mat", ctypes.c_void_p),

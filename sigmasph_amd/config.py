@@ -86,6 +86,12 @@ PARAMS = [
     P("OBJECTIVE_RATE", "Solver", "objectiveRate", "str", "jaumann", choices=("jaumann", "none")),
     P("KERNEL_CORRECTION", "Solver", "kernelCorrection", "bool", False),
     P("L_EIG_TOL", "Solver", "lEigTol", "float", 0.1),
+    # Solver sub-cards (params.py:494-538)
+    P("BULK_VISCOSITY_Q", "Solver.bulkViscosity", "quadratic", "float", 0.0),
+    P("BULK_VISCOSITY_LENGTH", "Solver.bulkViscosity", "lengthScale", "str", "dx0", choices=("dx0", "h")),
+    P("MONAGHAN_ALPHA", "Solver.artificialViscosity", "alpha", "float", 0.0),
+    P("MONAGHAN_BETA", "Solver.artificialViscosity", "beta", "float", 0.0),
+    P("MONAGHAN_EPS", "Solver.artificialViscosity", "epsilon", "float", 0.01),
     # Load (params.py:977)
     P("gravitation", "Load", "gravitation", "vec3", required=True),
     P("rampTime", "Load", "rampTime", "float", 0.0),
@@ -130,7 +136,9 @@ PARAMS = [
 ]
 BY_NAME = {p.name: p for p in PARAMS}
 CARDS = ("Domain", "Time", "Solver", "Load", "Diffusion", "Shifting", "Hourglass")
-#: sub-cards whose presence switches a feature on (params.py:1326-1332)
+#: sub-cards; those whose presence switches a feature on name their flag
+#: (params.py:1326-1340)
+SUBCARDS = ("Hourglass.superTimeStepping", "Solver.bulkViscosity", "Solver.artificialViscosity")
 PRESENCE = {"Hourglass.superTimeStepping": "HOURGLASS_STS"}
 CONSTRAINT_KEYS = {"name", "region", "components", "displacement"}
 BLOCK_KEYS = {"objectId", "start", "end", "translation", "scale", "velocity", "density",
@@ -235,14 +243,15 @@ class SimConfig:
                     errors.append("card %s: expected an object" % section)
                     continue
                 fields = {p.field: p for p in PARAMS if p.card == section}
-                subs = {c.split(".", 1)[1]: c for c in PRESENCE if c.split(".")[0] == section}
+                subs = {c.split(".", 1)[1]: c for c in SUBCARDS if c.split(".")[0] == section}
                 for key, value in body.items():
                     if key in subs:
                         sub = subs[key]
                         if not isinstance(value, dict):
                             errors.append("card %s: %s must be an object" % (sub, key))
                             continue
-                        given[PRESENCE[sub]] = True
+                        if sub in PRESENCE:
+                            given[PRESENCE[sub]] = True
                         sfields = {p.field: p for p in PARAMS if p.card == sub}
                         for sk, sv in value.items():
                             sp = sfields.get(sk)

@@ -67,7 +67,7 @@ struct DScratch {
     int* tmp;        // [n] slot -> unsorted index
     int* perm;       // [n] sorted index -> unsorted index
     double* part;    // [SPH_NPART][nblk] per-block partials
-    double* maxv;    // [4] atomic maxima: c2, |a|, |v|
+    double* maxv;    // [8] atomic maxima: c2, |a|, |v|; the solid's MG c^2, VNR and Monaghan speeds
 };
 
 enum { PART_WDIFF = 0, PART_LEFT = 1, PART_VISC = 2, PART_WALL = 3, PART_R0 = 4, PART_R1 = 5,
@@ -76,7 +76,7 @@ enum { PART_WDIFF = 0, PART_LEFT = 1, PART_VISC = 2, PART_WALL = 3, PART_R0 = 4,
        PART_PLASTIC = 7, PART_IEPAIR = 8, PART_IEADV = 9, PART_IEDIFF = 10, PART_IEALE = 11,
        PART_HGW = 12, PART_KE = 13, PART_KE2 = 14, PART_CROSS = 15, PART_VISCS = 16,
        SPH_NPART = 17 };
-enum { MAX_C2 = 0, MAX_ACC = 1, MAX_V = 2 };
+enum { MAX_C2 = 0, MAX_ACC = 1, MAX_V = 2, MAX_MG = 3, MAX_QC = 4, MAX_AV = 5 };
 
 // a neighbour list: entry k of particle i at idx[k * npad + i], j | mirror << 30
 struct NList {
@@ -1404,12 +1404,12 @@ extern "C" int sph_ctx_create(sph_ctx** out, const sph_dims* d) {
     ALLOC(s.tmp, n * sizeof(int));
     ALLOC(s.perm, n * sizeof(int));
     ALLOC(s.part, (size_t)SPH_NPART * c->pstride * sizeof(double));
-    ALLOC(s.maxv, 4 * sizeof(double));
+    ALLOC(s.maxv, 8 * sizeof(double));
     ALLOC(c->slow_idx, n * sizeof(int));
     ALLOC(c->slow_cnt, sizeof(int));
 #undef ALLOC
     if (e == hipSuccess) e = hipMemset(s.count, 0, nb1 * sizeof(int));
-    if (e == hipSuccess) e = hipMemset(s.maxv, 0, 4 * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(s.maxv, 0, 8 * sizeof(double));
     if (e == hipSuccess) e = hipMemset(s.start, 0, nb1 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(c->slow_cnt, 0, sizeof(int));
     if (e == hipSuccess) e = hipDeviceSynchronize();

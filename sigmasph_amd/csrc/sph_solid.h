@@ -34,6 +34,9 @@ struct DSolidP {
     int kc, jaumann, transport, transport_ep, sts, sts_stages, sts_max, sts_refresh;
     int sts_bound;   // the most sub-steps any dt of the step can ask for (dt <= CFL h / c_signal)
     int dt_pair_mode, visc_work, thermal, plastic, ncons, hg_explicit;
+    double q_l2, q_cl;                   // VNR: C_Q l^2, C_Q l
+    double av_alpha, av_beta, av_epsh2;  // Monaghan: alpha, beta, eps h^2
+    int q_on, av_on;
 };
 
 template <typename R> struct DSolid {
@@ -45,6 +48,7 @@ template <typename R> struct DSolid {
     const typename R4t<R>::type* bcd;
     R* gv;                            // [n][10]
     double4* y0; double4* y1; double4* vs;
+    R* q;                             // [n] VNR q (when q_on)
     const sph_material* mat;
     double* cons;
 };
@@ -95,19 +99,58 @@ template <typename R> __device__ __forceinline__ void mirror_gv(R G[3][3], int m
         for (int b = 0; b < 3; ++b) G[a][b] *= m[a] * m[b];
 }
 
-__device__ __forceinline__ double solid_eos(const sph_material& M, double rho, int allow_neg) {
-    // eos_pressure_i for the linear / Tait rows (SOLID.py:3009-3038)
-    double p = M.eos_kind == SPH_EOS_LINEAR ? M.eos_S * (rho / M.rho0 - 1.0)
-                                            : M.eos_S * (pow(rho / M.rho0, M.eos_gamma) - 1.0);
+// The polynomial Mie-Grueneisen pressure p = C0 + C1 mu + C2 mu^2 + C3 mu^3 + (C4 + C5 mu) rho0 e
+// (linear cold curve in tension when mg_linear_exp is set) and its isentropic sound speed
+// c^2 = dp/drho|_e + p/rho^2 dp/de, floored at (1e-3 c0_ref)^2
+// (mie_gruneisen_pressure / _sound_speed_sq, EOS003_mie_gruneisen.py:223-270).
+__device__ __forceinline__ double mg_pressure(const sph_material& M, double rho, double e, double* c_sq) {
+    double mu = rho / M.rho0 - 1.0;
+    double gam = M.mg_C4 + M.mg_C5 * mu;
+    double pc, dpc;
+    if (mu < 0.0 && M.mg_linear_exp > 0.5) {
+        pc = M.mg_C0 + M.mg_C1 * mu;
+        dpc = M.mg_C1;
+    } else {
+        pc = M.mg_C0 + mu * (M.mg_C1 + mu * (M.mg_C2 + mu * M.mg_C3));
+        dpc = M.mg_C1 + mu * (2.0 * M.mg_C2 + 3.0 * M.mg_C3 * mu);
+    }
+    double p = pc + gam * M.rho0 * e;
+    if (c_sq) {
+        double c2 = dpc / M.rho0 + M.mg_C5 * e + (p / (rho * rho)) * gam * M.rho0;
+        double cf = 1e-3 * M.mg_c0_ref;
+        *c_sq = fmax(c2, cf * cf);
+    }
+    return p;
+}
+
+// eos_pressure_i (SOLID.py:3009-3038, 3142-3181): linear, Tait, or Mie-Grueneisen on
+// e_for_eos = max(e, 0) (3075-3109)
+__device__ __forceinline__ double solid_eos(const sph_material& M, double rho, double e, int allow_neg) {
+    double p;
+    if (M.eos_kind == SPH_EOS_MG) p = mg_pressure(M, rho, fmax(e, 0.0), nullptr);
+    else if (M.eos_kind == SPH_EOS_LINEAR) p = M.eos_S * (rho / M.rho0 - 1.0);
+    else p = M.eos_S * (pow(rho / M.rho0, M.eos_gamma) - 1.0);
     if (!allow_neg) p = fmax(p, 0.0);
     return p;
 }
 
-// tait_specific_energy (EOS001_tait.py:102-121) with the clamp of cold_specific_energy
+// the cold curve's specific energy with the clamp of cold_specific_energy
+// (SOLID.py:2511-2567): tait_specific_energy (EOS001_tait.py:102-121), or for a
+// Mie-Grueneisen row the integral of p_cold / rho^2 from rho0 (mg_cold_specific_energy,
+// EOS003_mie_gruneisen.py:322-331) in x = rho / rho0
 __device__ __forceinline__ double cold_energy(const sph_material& M, double rho, int allow_neg) {
     double x = rho / M.rho0, e;
-    if (fabs(M.eos_gamma - 1.0) < 1e-6) e = (M.eos_S / M.rho0) * (log(x) + 1.0 / x - 1.0);
-    else e = (M.eos_S / M.rho0) * ((pow(x, M.eos_gamma - 1.0) - 1.0) / (M.eos_gamma - 1.0) + 1.0 / x - 1.0);
+    if (M.eos_kind == SPH_EOS_MG) {
+        double c2 = M.mg_C2, c3 = M.mg_C3;
+        if (rho < M.rho0 && M.mg_linear_exp > 0.5) c2 = c3 = 0.0;
+        double A = c3, B = c2 - 3.0 * c3, D = M.mg_C1 - 2.0 * c2 + 3.0 * c3;
+        double E = M.mg_C0 - M.mg_C1 + c2 - c3;
+        e = (0.5 * A * (x * x - 1.0) + B * (x - 1.0) + D * log(x) + E * (1.0 - 1.0 / x)) / M.rho0;
+    } else if (fabs(M.eos_gamma - 1.0) < 1e-6) {
+        e = (M.eos_S / M.rho0) * (log(x) + 1.0 / x - 1.0);
+    } else {
+        e = (M.eos_S / M.rho0) * ((pow(x, M.eos_gamma - 1.0) - 1.0) / (M.eos_gamma - 1.0) + 1.0 / x - 1.0);
+    }
     if (!allow_neg && rho < M.rho0) e = 0.0;
     return e;
 }
@@ -190,6 +233,7 @@ template <typename R> struct StressBody {
     typedef SolidArgs<R> Args;
     const Args& a;
     double w_diff, w_plastic, w_iepair, w_ieadv, w_iediff, w_ieale;
+    double mg_c2, q_c;        // the MG and VNR dt limits' maxima
     double4 vi4;
     double xiy;
     R rho_i, inv_rho_i, duix, duiy, duiz, inv_dt, epi, ei, kdelta;
@@ -200,7 +244,7 @@ template <typename R> struct StressBody {
     R as_xx, as_yy, as_xy, as_zz, as_xz, as_yz, a_ep, a_eadv, a_ediff;
 
     __device__ StressBody(const Args& a_)
-        : a(a_), w_diff(0), w_plastic(0), w_iepair(0), w_ieadv(0), w_iediff(0), w_ieale(0) {}
+        : a(a_), w_diff(0), w_plastic(0), w_iepair(0), w_ieadv(0), w_iediff(0), w_ieale(0), mg_c2(0), q_c(0) {}
 
     __device__ void begin(int i, double& x, double& y, double& z) {
         const DBuf<R>& b = a.st.b;
@@ -488,7 +532,27 @@ template <typename R> struct StressBody {
         st.V_new[i] = Vn;
         st.rho_new[i] = (R)(m / Vfull);
         // ---- the pressure from the particle's EOS row (SOLID.py:3212-3252)
-        b.p[i] = (R)solid_eos(M, m / Vfull, F.allow_neg);
+        const double rho_n = m / Vfull;
+        b.p[i] = (R)solid_eos(M, rho_n, e_new, F.allow_neg);
+        if (M.eos_kind == SPH_EOS_MG) {
+            // clamp_dt_mie_gruneisen (SOLID.py:3886-3907): c^2 + (4/3) G / rho
+            double c2;
+            mg_pressure(M, rho_n, fmax(e_new, 0.0), &c2);
+            if (M.G > 0.0) c2 += (4.0 / 3.0) * M.G / rho_n;
+            mg_c2 = fmax(mg_c2, c2);
+        }
+        if (S.q_on) {
+            // VNR q from tr(grad v) (SOLID.py:3386-3402) and its dt limit
+            // (clamp_dt_bulk_viscosity, 3938-3974): Q + sqrt(Q^2 + c_p^2), Q = C_Q l |div v|
+            double dvv = Gd[0][0] + Gd[1][1] + Gd[2][2];
+            double qv = 0.0;
+            if (dvv < 0.0) {
+                qv = S.q_l2 * rho_n * dvv * dvv;
+                double Q = S.q_cl * (-dvv);
+                q_c = fmax(q_c, Q + sqrt(Q * Q + M.c_p * M.c_p));
+            }
+            sd.q[i] = (R)qv;
+        }
     }
 
     __device__ void reduce(double* red, int slot) {
@@ -499,7 +563,11 @@ template <typename R> struct StressBody {
         double v3 = block_sum(w_ieadv, red);
         double v4 = block_sum(w_iediff, red);
         double v5 = block_sum(w_ieale, red);
+        double m0 = block_max(mg_c2, red);
+        double m1 = a.S.q_on ? block_max(q_c, red) : 0.0;
         if (threadIdx.x == 0) {
+            if (m0 > 0.0) atomic_max_pos(&a.s.maxv[MAX_MG], m0);
+            if (m1 > 0.0) atomic_max_pos(&a.s.maxv[MAX_QC], m1);
             a.s.part[PART_WDIFF * ps + slot] = v0;
             a.s.part[PART_PLASTIC * ps + slot] = v1;
             a.s.part[PART_IEPAIR * ps + slot] = v2;
@@ -517,9 +585,9 @@ template <typename R> struct StressBody {
 template <typename R> struct SolidForceBody {
     typedef SolidArgs<R> Args;
     const Args& a;
-    double c2loc, amag, w_visc, w_hg;
+    double c2loc, amag, w_visc, w_hg, av_c;
     double4 vi4, xi4;
-    R rho_i, lam_i, inv_dt, c2max, p_i;
+    R rho_i, lam_i, inv_dt, c2max, p_i, pt_i;
     int eos_i;
     Sym6<R> si;
     R Gi[3][3];
@@ -530,8 +598,11 @@ template <typename R> struct SolidForceBody {
     R avx, avy, avz, wmor;    // Morris: Sum f_v grad W, Sum f_v grad W . dv
     R hx, hy, hz, whg;        // hourglass: Sum (damp / prefactor) rhat, its work
     R aax, aay, aaz;
+    R pix, piy, piz, wpi, mu_max;   // Monaghan: Sum -mw_j Pi_ij grad W, its work, max_j |mu_ij|
+    R cbi;                          // c_p of the particle's material
 
-    __device__ SolidForceBody(const Args& a_) : a(a_), c2loc(a_.F.csig2), amag(0.0), w_visc(0.0), w_hg(0.0) {}
+    __device__ SolidForceBody(const Args& a_)
+        : a(a_), c2loc(a_.F.csig2), amag(0.0), w_visc(0.0), w_hg(0.0), av_c(0.0) {}
 
     __device__ void begin(int i, double& x, double& y, double& z) {
         const DBuf<R>& b = a.st.b;
@@ -541,14 +612,17 @@ template <typename R> struct SolidForceBody {
         vi4 = b.v[i];
         rho_i = a.st.rho_new[i];
         p_i = b.p[i];
+        pt_i = a.S.q_on ? p_i + a.sd.q[i] : p_i;   // p_tot = p + q (SOLID.py:3566-3570)
         lam_i = a.st.lam[i];
         inv_dt = (R)(1.0 / a.st.scal[SPH_S_DT]);
         c2max = (R)a.F.csig2;
         eos_i = a.sd.eos[i];
+        cbi = (R)a.sd.mat[eos_i].c_p;
         si = load_sig(a.sd.sig_in, i, two_d);
         if (a.S.hg_explicit) load_gv(a.sd.gv, i, two_d, Gi);
         glx = gly = glz = swx = swy = swz = fjx = fjy = fjz = wsig = 0;
         avx = avy = avz = wmor = hx = hy = hz = whg = aax = aay = aaz = 0;
+        pix = piy = piz = wpi = mu_max = 0;
     }
 
     __device__ void pair(int j, int mir, double rxd, double ryd, double rzd) {
@@ -570,12 +644,13 @@ template <typename R> struct SolidForceBody {
         Sym6<R> sj = load_sig(a.sd.sig_in, j, two_d);
         mirror_sig(sj, mir);
         R p_j = b.p[j];
-        R sjx = (sj.xx - p_j) * gx + sj.xy * gy + sj.xz * gz;
-        R sjy = sj.xy * gx + (sj.yy - p_j) * gy + sj.yz * gz;
-        R sjz = sj.xz * gx + sj.yz * gy + (sj.zz - p_j) * gz;
-        R six = (si.xx - p_i) * gx + si.xy * gy + si.xz * gz;
-        R siy = si.xy * gx + (si.yy - p_i) * gy + si.yz * gz;
-        R siz = si.xz * gx + si.yz * gy + (si.zz - p_i) * gz;
+        R pt_j = S.q_on ? p_j + a.sd.q[j] : p_j;
+        R sjx = (sj.xx - pt_j) * gx + sj.xy * gy + sj.xz * gz;
+        R sjy = sj.xy * gx + (sj.yy - pt_j) * gy + sj.yz * gz;
+        R sjz = sj.xz * gx + sj.yz * gy + (sj.zz - pt_j) * gz;
+        R six = (si.xx - pt_i) * gx + si.xy * gy + si.xz * gz;
+        R siy = si.xy * gx + (si.yy - pt_i) * gy + si.yz * gz;
+        R siz = si.xz * gx + si.yz * gy + (si.zz - pt_i) * gz;
         swx += wj * gx; swy += wj * gy; swz += wj * gz;
         fjx += wj * sjx; fjy += wj * sjy; fjz += wj * sjz;
         double4 vj4 = b.v[j];
@@ -588,6 +663,21 @@ template <typename R> struct SolidForceBody {
             R fv = wj * vr * frcp<R>(r2 + (R)F.eta2);
             avx += fv * gx; avy += fv * gy; avz += fv * gz;
             wmor += fv * (gx * dvx + gy * dvy + gz * dvz);
+        }
+        R rho_j = st.rho_new[j];
+        if (S.av_on) {
+            // Monaghan's Pi_ij on approaching pairs (SOLID.py:3711-3765)
+            R vr = dvx * rx + dvy * ry + dvz * rz;
+            if (vr < (R)0) {
+                R mu = (R)F.h * vr * frcp<R>(r2 + (R)S.av_epsh2);
+                R rho_bar = (R)0.5 * (rho_i + rho_j);
+                R c_bar = (R)0.5 * (cbi + (R)a.sd.mat[a.sd.eos[j]].c_p);
+                R pi = ((R)(-S.av_alpha) * c_bar * mu + (R)S.av_beta * mu * mu) * frcp<R>(rho_bar);
+                R f = -mwq<R>(F, b, j) * pi;
+                pix += f * gx; piy += f * gy; piz += f * gz;
+                wpi += f * (gx * dvx + gy * dvy + gz * dvz);
+                mu_max = fmax(mu_max, -mu);
+            }
         }
         if (S.hg_explicit && rn > (R)1e-9) {
             // the rate damper on v (SOLID.py:3778-3825): dv_hg = (v_j - v_i) - <G> (x_j - x_i)
@@ -606,7 +696,6 @@ template <typename R> struct SolidForceBody {
             hx += k * ex; hy += k * ey; hz += k * ez;
             whg += k * (ex * dvx + ey * dvy + ez * dvz);
         }
-        R rho_j = st.rho_new[j];
         if (F.pst_en && F.ale_mom) {
             typename R4t<R>::type sh = b.shift[j];
             R djx = sh.x * inv_dt, djy = sh.y * inv_dt, djz = sh.z * inv_dt;
@@ -636,6 +725,11 @@ template <typename R> struct SolidForceBody {
         whg = lane_sum<LPP>(whg);
         aax = lane_sum<LPP>(aax); aay = lane_sum<LPP>(aay); aaz = lane_sum<LPP>(aaz);
         c2max = lane_max<LPP>(c2max);
+        if (a.S.av_on) {
+            pix = lane_sum<LPP>(pix); piy = lane_sum<LPP>(piy); piz = lane_sum<LPP>(piz);
+            wpi = lane_sum<LPP>(wpi);
+            mu_max = lane_max<LPP>(mu_max);
+        }
     }
 
     __device__ void end(int i) {
@@ -655,9 +749,9 @@ template <typename R> struct SolidForceBody {
         // f_ij / mw_i = w_i w_j (sigma_i + sigma_j) grad W / mw_i, with w_i / mw_i = 1 / rho_i
         R wi = wq<R>(F, b, st.V_new, i), mwi = mwq<R>(F, b, i);
         R kf = wi / mwi;
-        R six = (si.xx - p_i) * swx + si.xy * swy + si.xz * swz;
-        R siy = si.xy * swx + (si.yy - p_i) * swy + si.yz * swz;
-        R siz = si.xz * swx + si.yz * swy + (si.zz - p_i) * swz;
+        R six = (si.xx - pt_i) * swx + si.xy * swy + si.xz * swz;
+        R siy = si.xy * swx + (si.yy - pt_i) * swy + si.yz * swz;
+        R siz = si.xz * swx + si.yz * swy + (si.zz - pt_i) * swz;
         double ax = (double)(kf * (six + fjx)), ay = (double)(kf * (siy + fjy)), az = (double)(kf * (siz + fjz));
         double deint = -0.5 * (double)kf * (double)wsig;
         const double kv = 10.0 * F.visc;   // 2 (dim + 2), dim = 3
@@ -667,6 +761,17 @@ template <typename R> struct SolidForceBody {
             double wm = -0.5 * kv * (double)wmor;
             deint += wm;
             if (S.visc_work) dvisc = -wm;
+        }
+        if (S.av_on) {
+            ax += (double)pix; ay += (double)piy; az += (double)piz;
+            double wp = -0.5 * (double)wpi;
+            deint += wp;
+            if (S.visc_work) dvisc -= wp;
+            // clamp_dt_monaghan (SOLID.py:3976-4009): c_p + 1.2 (alpha c_p + beta mu_max)
+            if (mu_max > (R)0) {
+                double cp = M.c_p;
+                av_c = fmax(av_c, cp + 1.2 * (S.av_alpha * cp + S.av_beta * (double)mu_max));
+            }
         }
         double dhg = 0.0;
         if (S.hg_explicit) {
@@ -703,7 +808,9 @@ template <typename R> struct SolidForceBody {
         double ba = block_max(amag, red);
         double bv = block_sum(w_visc, red);
         double bh = block_sum(w_hg, red);
+        double bq = a.S.av_on ? block_max(av_c, red) : 0.0;
         if (threadIdx.x == 0) {
+            if (bq > 0.0) atomic_max_pos(&a.s.maxv[MAX_AV], bq);
             atomic_max_pos(&a.s.maxv[MAX_C2], bc);
             atomic_max_pos(&a.s.maxv[MAX_ACC], ba);
             a.s.part[PART_VISCS * a.pstride + slot] = bv;
@@ -915,6 +1022,10 @@ __global__ __launch_bounds__(1024) void k_dt_solid(DFluid F, DSolidP S, double* 
         double c2 = fmax(s.maxv[MAX_C2], F.csig2);
         double am = fmax(s.maxv[MAX_ACC], 1e-10);
         double dt = fmin(F.cfl * F.h / sqrt(c2), F.cfl * sqrt(F.h / am));
+        // clamp_dt_mie_gruneisen, _bulk_viscosity, _monaghan (SOLID.py:5126-5138)
+        if (s.maxv[MAX_MG] > 0.0) dt = fmin(dt, F.cfl * F.h / sqrt(s.maxv[MAX_MG]));
+        if (s.maxv[MAX_QC] > 0.0) dt = fmin(dt, F.cfl * F.h / s.maxv[MAX_QC]);
+        if (s.maxv[MAX_AV] > 0.0) dt = fmin(dt, F.cfl * F.h / s.maxv[MAX_AV]);
         int st = 0;
         if (S.hg_explicit) {
             dt = fmin(dt, 1.6 / S.hg_rate);
@@ -935,6 +1046,9 @@ __global__ __launch_bounds__(1024) void k_dt_solid(DFluid F, DSolidP S, double* 
         scal[SPH_S_MAX_ACC] = am;
         s.maxv[MAX_C2] = 0.0;
         s.maxv[MAX_ACC] = 0.0;
+        s.maxv[MAX_MG] = 0.0;
+        s.maxv[MAX_QC] = 0.0;
+        s.maxv[MAX_AV] = 0.0;
     }
 }
 
@@ -1261,6 +1375,13 @@ static DSolidP make_solid_params(const sph_ctx* c, const sph_fluid_params* P, co
     d.thermal = S->thermal;
     d.plastic = S->plastic;
     d.ncons = S->ncons;
+    d.q_l2 = S->q_l2;
+    d.q_cl = S->q_cl;
+    d.q_on = S->q_l2 > 0.0;
+    d.av_alpha = S->av_alpha;
+    d.av_beta = S->av_beta;
+    d.av_epsh2 = S->av_eps * c->dims.h * c->dims.h;
+    d.av_on = S->av_alpha > 0.0 || S->av_beta > 0.0;
     return d;
 }
 
@@ -1275,6 +1396,7 @@ template <typename R> static DSolid<R> dsolid(const sph_state* st, const sph_sol
     d.bcd = (const typename R4t<R>::type*)sd->bc_disp[k];
     d.gv = (R*)sd->grad_v;
     d.y0 = (double4*)sd->hg_y[0]; d.y1 = (double4*)sd->hg_y[1]; d.vs = (double4*)sd->v_sync;
+    d.q = (R*)sd->q_visc;
     d.mat = sd->mat;
     d.cons = sd->cons;
     return d;

@@ -1,7 +1,8 @@
 """Materials of the solid step: one resolved row per declared material.
 
 Follows the reference's resolution (materials.py:148-449, material_models/*.py): an
-entry declares `density0`, an `eos` (linear or Tait here) and, for a solid, a `strength`
+entry declares `density0`, an `eos` (linear, Tait or polynomial Mie-Grueneisen here) and,
+for a solid, a `strength`
 card with `youngsModulus`, `poissonRatio` and a `plasticity` model (linear J2 hardening
 or Johnson-Cook).  Each particle carries the integer row of its material (`eos_id`);
 every per-material constant is read through it, as in the reference's `mat_tab`,
@@ -15,30 +16,72 @@ import copy
 import ctypes
 import math
 
-from ._native import SphMaterial, EOS_LINEAR, EOS_TAIT, PLASTIC_NONE, PLASTIC_LINEAR, PLASTIC_JC
+from ._native import (SphMaterial, EOS_LINEAR, EOS_TAIT, EOS_MG, PLASTIC_NONE, PLASTIC_LINEAR,
+                      PLASTIC_JC)
 
 
 class MaterialError(ValueError):
     """A declared material whose constants cannot be resolved as written."""
 
 
-def _jc_preset(rho0, cv, E, nu, A, B, n, C, Tm, m):
-    return {"density0": rho0, "specificHeat": cv, "eos": {"type": "linear"},
+def _jc_preset(rho0, cv, E, nu, A, B, n, C, Tm, m, eos=None, T0=293.0):
+    return {"density0": rho0, "specificHeat": cv, "eos": eos or {"type": "linear"},
             "strength": {"type": "hypoElastic", "youngsModulus": E, "poissonRatio": nu,
                          "plasticity": {"model": "johnson_cook", "A": A, "B": B, "n": n,
-                                        "C": C, "eps0_dot": 0.001, "T0": 293.0, "Tm": Tm,
+                                        "C": C, "eps0_dot": 0.001, "T0": T0, "Tm": Tm,
                                         "m": m, "Cp": cv, "chi": 0.9}}}
 
 
-# material_models/material_library.py:28-180 (mm / ms / GPa / kg units)
+def _mg(c0, s, gamma0):
+    return {"type": "mie_gruneisen", "c0": c0, "s": s, "gamma0": gamma0}
+
+
+# material_models/material_library.py:28-474 (mm / ms / GPa / kg units).  The presets
+# that need a damage model or the HJC EOS (tantalum_spall, concrete_hjc_*) are not here.
 LIBRARY = {
     "aluminium": _jc_preset(2.70e-06, 896.0, 69.0, 0.33, 0.324, 0.114, 0.42, 0.002, 925.0, 1.34),
+    "al2017_t4": _jc_preset(2.79e-06, 875.0, 72.4, 0.33, 0.265, 0.426, 0.34, 0.015, 775.0, 1.00,
+                            eos=_mg(5328.0, 1.34, 2.0)),
     "construction_steel": _jc_preset(7.85e-06, 486.0, 210.0, 0.30, 0.350, 0.275, 0.36, 0.022,
                                      1780.0, 1.00),
     "stainless_steel": _jc_preset(7.90e-06, 500.0, 193.0, 0.30, 0.310, 1.000, 0.65, 0.020,
                                   1720.0, 1.00),
     "copper": _jc_preset(8.96e-06, 383.0, 124.0, 0.34, 0.090, 0.292, 0.31, 0.025, 1356.0, 1.09),
+    "gelatin_10": _jc_preset(1.03e-06, 3700.0, 1.5e-04, 0.45, 5.2e-04, 1.0e-05, 0.1, 0.0, 0.0, 1.0,
+                             eos=_mg(1520.0, 1.8, 0.17)),
+    "gelatin_20": _jc_preset(1.06e-06, 3500.0, 2.82e-04, 0.45, 3.5e-03, 1.0e-05, 0.1, 0.0, 0.0, 1.0,
+                             eos=_mg(1520.0, 1.87, 0.17)),
+    "aluminium_kim2025": _jc_preset(2.71e-06, 875.0, 73.416, 0.33, 0.175, 0.380, 0.34, 0.0015,
+                                    775.0, 1.0, eos=_mg(5300.0, 1.5, 1.7), T0=273.0),
 }
+# gelatin's Johnson-Cook Cp is 0 in the library (no adiabatic heating of the yield)
+for _g in ("gelatin_10", "gelatin_20"):
+    LIBRARY[_g]["strength"]["plasticity"]["Cp"] = 0.0
+
+# MATERIAL_ALIASES (material_library.py:476-510)
+ALIASES = {
+    "aluminum": "aluminium", "al": "aluminium", "al6061": "aluminium", "al6061-t6": "aluminium",
+    "al2017": "al2017_t4", "al2017-t4": "al2017_t4", "al2017t4": "al2017_t4",
+    "aluminium_2017": "al2017_t4", "aluminum_2017": "al2017_t4",
+    "steel": "construction_steel", "structural_steel": "construction_steel",
+    "mild_steel": "construction_steel", "s355": "construction_steel",
+    "cu": "copper", "ofhc_copper": "copper",
+    "gelatin": "gelatin_10", "ballistic_gelatin": "gelatin_10", "bg10": "gelatin_10",
+    "bg20": "gelatin_20", "10%_gelatin": "gelatin_10", "20%_gelatin": "gelatin_20",
+    "ss304": "stainless_steel", "304": "stainless_steel", "304_stainless": "stainless_steel",
+    "304_stainless_steel": "stainless_steel", "aisi304": "stainless_steel",
+    "aisi_304": "stainless_steel", "ss316": "stainless_steel", "316": "stainless_steel",
+    "ss316l": "stainless_steel", "316l": "stainless_steel", "stainless": "stainless_steel",
+}
+
+
+def normalize_name(name):
+    """normalize_material_name (material_library.py:513-519)."""
+    if not isinstance(name, str):
+        return ""
+    clean = name.strip().lower()
+    return ALIASES.get(clean, clean)
+
 
 ENTRY_KEYS = {"name", "library", "density0", "specificHeat", "delta", "eos", "strength"}
 
@@ -57,12 +100,13 @@ def expand_preset(entry):
     """expand_material_preset (material_library.py:548-580)."""
     lib = entry.get("library")
     if lib is not None:
-        if lib not in LIBRARY:
+        key = normalize_name(lib)
+        if key not in LIBRARY:
             raise MaterialError("unknown library material %r (have %s)"
-                                % (lib, ", ".join(sorted(LIBRARY))))
-        merged = _merge(LIBRARY[lib], {k: v for k, v in entry.items() if k != "library"})
+                                % (lib, ", ".join(sorted(list(LIBRARY) + list(ALIASES)))))
+        merged = _merge(LIBRARY[key], {k: v for k, v in entry.items() if k != "library"})
         return merged
-    name = entry.get("name")
+    name = normalize_name(entry.get("name"))
     if name in LIBRARY and ("density0" not in entry or "eos" not in entry
                             or "strength" not in entry):
         return _merge(LIBRARY[name], entry)
@@ -141,7 +185,38 @@ class Material:
                     self.plastic_kind = PLASTIC_LINEAR
             else:
                 raise MaterialError("%s: plasticity model %r is not part of this step" % (where, model))
-        if kind == "tait":
+        self.mg = dict(C0=0.0, C1=0.0, C2=0.0, C3=0.0, C4=0.0, C5=0.0, c0_ref=0.0, linear_exp=1.0, e0=0.0)
+        if kind in ("mie_gruneisen", "polynomial_mie_gruneisen", "mg"):
+            # derive_mie_gruneisen_eos (EOS003_mie_gruneisen.py:60-214)
+            p0 = float(eos.get("p0", eos.get("P0", 0.0)))
+            le = eos.get("linearExpansion", True)
+            le = le.lower() in ("true", "1", "yes") if isinstance(le, str) else bool(le)
+            if eos.get("C1") is not None:
+                C1 = float(eos["C1"])
+                if not C1 > 0.0:
+                    raise MaterialError("%s.eos: C1 must be positive" % where)
+                C0 = float(eos.get("C0", p0))
+                C2, C3 = float(eos.get("C2", 0.0)), float(eos.get("C3", 0.0))
+                C4 = float(eos.get("C4", 0.0))
+                C5 = float(eos.get("C5", C4))
+                c0 = math.sqrt(C1 / self.rho0)
+            else:
+                c0, sl, g0 = eos.get("c0"), eos.get("s"), eos.get("gamma0", eos.get("Gamma0"))
+                if c0 is None or sl is None or g0 is None:
+                    raise MaterialError("%s.eos: Mie-Grueneisen needs (c0, s, gamma0) or C0..C5" % where)
+                c0, sl, g0 = float(c0), float(sl), float(g0)
+                if not c0 > 0.0 or sl < 0.0 or g0 < 0.0:
+                    raise MaterialError("%s.eos: c0 > 0, s >= 0 and gamma0 >= 0 are required" % where)
+                C0 = p0
+                C1 = self.rho0 * c0 * c0 - 0.5 * g0 * p0
+                C2 = (2.0 * sl - 1.0) * C1
+                C3 = (sl - 1.0) * (3.0 * sl - 1.0) * C1
+                C4 = C5 = g0
+            self.mg = dict(C0=C0, C1=C1, C2=C2, C3=C3, C4=C4, C5=C5, c0_ref=c0,
+                           linear_exp=1.0 if le else 0.0, e0=float(eos.get("e0", 0.0)))
+            self.eos_kind, self.eos_gamma, self.eos_S = EOS_MG, 1.0, C1
+            self.K, self.c_eos = C1, c0
+        elif kind == "tait":
             c0, g = eos.get("c0"), eos.get("exponent")
             if c0 is None or g is None:
                 raise MaterialError("%s.eos: a Tait equation of state needs c0 and exponent" % where)
@@ -163,12 +238,12 @@ class Material:
             self.eos_kind, self.eos_gamma, self.eos_S = EOS_LINEAR, 1.0, K
             self.K, self.c_eos = K, math.sqrt(K / self.rho0)
         else:
-            raise MaterialError("%s.eos: type %r is not part of this step (linear, tait)"
-                                % (where, kind))
+            raise MaterialError("%s.eos: type %r is not part of this step (linear, tait, "
+                                "mie_gruneisen)" % (where, kind))
         self.c_p = math.sqrt((self.K + 4.0 * self.G / 3.0) / self.rho0)
         # reference_temperature / reference_energy (materials.py:429-445)
         self.T0 = self.jc["T0"] if self.plastic_kind == PLASTIC_JC else 293.0
-        self.e0 = 0.0
+        self.e0 = self.mg["e0"] if self.eos_kind == EOS_MG else 0.0
 
     def row(self):
         m = SphMaterial()
@@ -179,6 +254,9 @@ class Material:
         j = self.jc
         (m.jc_A, m.jc_B, m.jc_n, m.jc_C, m.jc_eps0_dot, m.jc_T0, m.jc_Tm,
          m.jc_m) = j["A"], j["B"], j["n"], j["C"], j["eps0_dot"], j["T0"], j["Tm"], j["m"]
+        g = self.mg
+        (m.mg_C0, m.mg_C1, m.mg_C2, m.mg_C3, m.mg_C4, m.mg_C5, m.mg_c0_ref,
+         m.mg_linear_exp) = g["C0"], g["C1"], g["C2"], g["C3"], g["C4"], g["C5"], g["c0_ref"], g["linear_exp"]
         m.eos_kind, m.plastic_kind = self.eos_kind, self.plastic_kind
         return m
 

@@ -455,7 +455,8 @@ class ParticleSystem(SceneLayout):
                         "bc_flag": z(n, dt=torch.int32), "bc_id": z(n, dt=torch.int32),
                         "bc_disp": z(n, 4)} for _ in range(2)]
         self._sderived = {"grad_v": z(n, 10), "hg_y0": z(n, 4, dt=torch.float64),
-                          "hg_y1": z(n, 4, dt=torch.float64), "v_sync": z(n, 4, dt=torch.float64)}
+                          "hg_y1": z(n, 4, dt=torch.float64), "v_sync": z(n, 4, dt=torch.float64),
+                          "q_visc": z(n)}
         self.cons = z(16, 6, dt=torch.float64)
         h0, b0 = self._hist[0], self._sbufs[0]
         h0["e_int"].copy_(torch.from_numpy(self.e_init))
@@ -474,6 +475,7 @@ class ParticleSystem(SceneLayout):
         sd.hcur = 0
         sd.grad_v = self._sderived["grad_v"].data_ptr()
         sd.v_sync = self._sderived["v_sync"].data_ptr()
+        sd.q_visc = self._sderived["q_visc"].data_ptr()
         sd.cons = self.cons.data_ptr()
         self.solid = sd
         H = lambda: self._hist[self.solid.hcur]
@@ -488,6 +490,7 @@ class ParticleSystem(SceneLayout):
         self.bc_id = Field(lambda: B()["bc_id"], 1, "bc_id")
         self.bc_disp = Field(lambda: B()["bc_disp"], 3, "bc_disp")
         self.grad_v = grad_field(lambda: self._sderived["grad_v"], "grad_v")
+        self.q_visc = Field(lambda: self._sderived["q_visc"], 1, "q_visc")
 
     # ---------------------------------------------------------------------------------
     def _live(self):

@@ -114,7 +114,7 @@ def solid_block(n=(20, 8, 1), dx=0.1, two_d="planeStrain", jitter=0.0, seed=0,
                 hourglass=4.0, sts=True, sts_stages=0, pst=True, ale_history=True,
                 viscosity=0.02, delta=0.1, cfl=0.3, jc=True, yield_stress=0.31, grips=None,
                 ramp_time=0.5, kernel_correction=False, damping=0.0, extra=None,
-                material=None, factor=6.0, y0=None):
+                material=None, factor=6.0, y0=None, density=7.9e-06):
     """A solid bar for tests: a single plane-strain or axisymmetric layer (`two_d`) or a
     3-D block (`two_d=None`), steel (Johnson-Cook or linear J2 hardening), Colle shifting
     with ALE history transport, the rate hourglass damper (super-time-stepped when
@@ -129,7 +129,7 @@ def solid_block(n=(20, 8, 1), dx=0.1, two_d="planeStrain", jitter=0.0, seed=0,
     if two_d:
         dom[two_d] = True
     blk = {"objectId": 0, "start": lo, "end": [hi[k] + 0.25 * dx for k in range(3)],
-           "density": 7.9e-06, "material": "steel", "jitter": jitter, "seed": seed,
+           "density": density, "material": (material or {}).get("name", "steel"), "jitter": jitter, "seed": seed,
            "velocity": list(velocity)}
     if div_velocity is not None:
         blk["divVelocity"] = list(div_velocity)

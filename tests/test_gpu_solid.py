@@ -4,7 +4,8 @@ Small bars at sizes the oracle finishes in seconds, driven hard enough that ever
 branch runs within a few steps (an initial velocity field past the yield strain rate,
 grips on a short ramp): plane strain and axisymmetry with the super-time-stepped
 hourglass damper, Colle shifting and ALE transport, Johnson-Cook; a 3-D block with the
-explicit damper, J2 hardening and the corrected grad v.  fp64: every field to 1e-9 of
+explicit damper, J2 hardening and the corrected grad v; a Mie-Grueneisen aluminium bar
+under compression with the VNR bulk viscosity and Monaghan's pair viscosity.  fp64: every field to 1e-9 of
 its scale after one step and 1e-7 after five, the step's scalars and the grips alike.
 Mixed mode (float pair sums and per-particle fields, f64 positions, velocities, EOS,
 return map and reductions): every field within 2e-4 of its scale over three steps.
@@ -33,6 +34,12 @@ SCENES = {
                                             rot_velocity=400.0, jitter=0.05),
     "plane_strain_noshift": lambda: solid_block(n=(20, 10, 1), pst=False, viscosity=0.0,
                                                 div_velocity=(-200.0, 250.0, 0.0), jitter=0.03),
+    # Mie-Grueneisen aluminium under compression with the VNR q and Monaghan's Pi
+    "axisymmetric_shock": lambda: solid_block(
+        n=(24, 8, 1), two_d="axisymmetric", material={"name": "projectile", "library": "al2017_t4"},
+        density=2.79e-06, div_velocity=(-400.0, -200.0, 0.0), jitter=0.02,
+        extra={"Solver": {"bulkViscosity": {"quadratic": 1.5},
+                          "artificialViscosity": {"alpha": 1.0, "beta": 2.0}}}),
 }
 
 SCALARS = (N.SPH_S_DT, N.SPH_S_PLASTIC_WORK, N.SPH_S_IE_PAIR, N.SPH_S_IE_ALE, N.SPH_S_HG_WORK,
@@ -79,7 +86,7 @@ def compare(ps, st, s, tol, eps=0.0):
         "sigma_dev": 1e-6 * M.G, "pressure": max(1e-6 * M.K, 64.0 * M.K * eps),
         "acceleration": 1e-6 * M.K / (M.rho0 * h), "divergence": U / h,
         "e_int": 1e-6 * M.K / M.rho0, "temperature": 1.0, "eps_plastic": 1e-6,
-        "grad_v": U / h,
+        "grad_v": U / h, "q_visc": 1e-6 * M.K,
     }
     pairs = [("x", s.by_pid("x")), ("v", s.by_pid("v")), ("u", s.by_pid("u")), ("V", s.by_pid("V")),
              ("pressure", s.by_pid("p")), ("acceleration", s.by_pid("acc")),
@@ -87,6 +94,8 @@ def compare(ps, st, s, tol, eps=0.0):
              ("sigma_dev", sym(s.by_pid("sig"))), ("eps_plastic", s.by_pid("eps_p")),
              ("e_int", s.by_pid("e_int")), ("temperature", s.by_pid("T")),
              ("grad_v", s.by_pid("grad_v").reshape(-1, 3, 3))]
+    if st.q_l2 > 0.0:
+        pairs.append(("q_visc", s.by_pid("q")))
     order = np.argsort(ps.pid.to_numpy(), kind="stable")
     errs = {}
     for name, ref in pairs:
@@ -127,7 +136,8 @@ def test_five_steps_fp64(gpu, name):
         assert s.scal[N.SPH_S_STS_STAGES] >= 1
 
 
-@pytest.mark.parametrize("name", ["plane_strain_grips", "axisymmetric_grips", "block3d_explicit"])
+@pytest.mark.parametrize("name", ["plane_strain_grips", "axisymmetric_grips", "block3d_explicit",
+                                  "axisymmetric_shock"])
 def test_three_steps_mixed(gpu, name):
     ps, solver, st, s = run_pair(SCENES[name](), 3, precision="mixed")
     compare(ps, st, s, 2e-4, eps=2.0 ** -24)

@@ -197,7 +197,7 @@ def test_update_output_is_trace_free_and_yields():
 
 
 def test_strengthless_material_carries_no_deviator():
-    mat = {"name": "steel", "density0": 7.9e-6, "eos": {"type": "linear", "bulkModulus": 160.0}}
+    mat = {"name": "strengthless", "density0": 7.9e-6, "eos": {"type": "linear", "bulkModulus": 160.0}}
     Lv = np.zeros((3, 3)); Lv[0, 1] = 1.0
     lay, st, s = _constitutive({"material": mat, "hourglass": 0.0}, Lv,
                                sig0=np.array([0.1, 0.0, -0.1, 0.2, 0.0, 0.0]))

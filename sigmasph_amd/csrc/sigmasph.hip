@@ -1252,6 +1252,10 @@ struct sph_ctx {
     double* cons_part = nullptr;   // [ncons * 3][nblk] grip force partials
     int cons_cap = 0;
     NList nl = {nullptr, nullptr, 0, 0};   // the solid step's neighbour list
+    // the hourglass sub-steps' stage-invariant pair operator, filled by the first stage
+    void* hg_pair = nullptr;       // [ncomp][nmax][npad] of R: w_j W_ij e e^T
+    double4* hg_src = nullptr;     // [npad] sum_j w_j W_ij (e . <G> r) e
+    bool hg_pairs = true;          // later stages read them instead of recomputing
 };
 
 static int set_err(sph_ctx* c, int code, const char* fmt, const char* a = "", long long b = 0) {
@@ -1380,6 +1384,7 @@ extern "C" int sph_ctx_create(sph_ctx** out, const sph_dims* d) {
             }
             c->lpp[k] = v;
         }
+        if (const char* e = getenv("SIGMASPH_HG_PAIRS")) c->hg_pairs = atoi(e) != 0;   // A/B switch
         c->tiles.ntx = (d->box_n[0] + B - 1) / B;
         int64_t nt = (int64_t)c->tiles.ntx * d->box_n[1] * d->box_n[2];
         if (nt > (int64_t)1 << 30) { delete c; return SPH_EINVAL; }
@@ -1430,6 +1435,8 @@ extern "C" int sph_ctx_destroy(sph_ctx* c) {
     if (c->cons_part) (void)hipFree(c->cons_part);
     if (c->nl.idx) (void)hipFree(c->nl.idx);
     if (c->nl.cnt) (void)hipFree(c->nl.cnt);
+    if (c->hg_pair) (void)hipFree(c->hg_pair);
+    if (c->hg_src) (void)hipFree(c->hg_src);
     for (auto& t : c->tk) for (auto ev : t.ev) (void)hipEventDestroy(ev);
     delete c;
     return SPH_OK;

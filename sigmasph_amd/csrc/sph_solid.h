@@ -63,6 +63,7 @@ template <typename R> struct SolidArgs {
     int pstride;
     // hourglass sub-step pass only
     const double4* ysrc; double4* ydst; int stage;
+    R* hgA; double4* hgb; int hg_mode;   // 0 recompute, 1 recompute and store, 2 read stored
     __device__ bool any_out() const { return st.scal[SPH_S_OUT_OF_BOX] > 0.5; }
     // a sub-step pass beyond this step's count has nothing to do
     __device__ bool skip() const { return ydst != nullptr && stage >= (int)st.scal[SPH_S_STS_STAGES]; }
@@ -918,7 +919,8 @@ template <typename R> struct HgBody {
     double4 yi;
     R Gi[3][3];
     R hx, hy, hz;
-    __device__ HgBody(const Args& a_) : a(a_) {}
+    R bx, by, bz;   // the stage-invariant part, sum_j w_j W (e . <G> r) e (mode 1)
+    __device__ HgBody(const Args& a_) : a(a_), bx(0), by(0), bz(0) {}
     __device__ void begin(int i, double& x, double& y, double& z) {
         double4 xi = a.st.b.x[i];
         x = xi.x; y = xi.y; z = xi.z;
@@ -950,9 +952,78 @@ template <typename R> struct HgBody {
         R k = wq<R>(F, b, b.V, j) * W * hg;
         hx += k * ex; hy += k * ey; hz += k * ez;
     }
+    // The first stage of a list slot: the same term split as A_ij (y_j - y_i) + c_ij e_ij
+    // with A_ij = w_j W_ij e e^T and c_ij = w_j W_ij e . <G> r, neither of which changes
+    // over the stages (grad v is refreshed once, before them); A goes to the slot, c e
+    // into the particle's stage-invariant source.
+    __device__ void pair_store(int j, int mir, double rxd, double ryd, double rzd, size_t slot, size_t cstride) {
+        const DFluid& F = a.F;
+        const DBuf<R>& b = a.st.b;
+        const bool two_d = (F.flags & SPH_TWO_D) != 0;
+        R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
+        R rn = fsqrt<R>(rx * rx + ry * ry + rz * rz);
+        R* A = a.hgA + slot;
+        if (!(rn > (R)1e-9)) {
+            A[0] = 0; A[cstride] = 0; A[2 * cstride] = 0;
+            if (!two_d) { A[3 * cstride] = 0; A[4 * cstride] = 0; A[5 * cstride] = 0; }
+            return;
+        }
+        double4 yj = a.ysrc[j];
+        if (mir & 1) yj.y = -yj.y;
+        if (mir & 2) yj.z = -yj.z;
+        R Gj[3][3];
+        load_gv(a.sd.gv, j, two_d, Gj);
+        mirror_gv(Gj, mir);
+        R inv_rn = frcp<R>(rn);
+        R ex = -rx * inv_rn, ey = -ry * inv_rn, ez = -rz * inv_rn;
+        R gx = (R)0.5 * ((Gi[0][0] + Gj[0][0]) * rx + (Gi[0][1] + Gj[0][1]) * ry + (Gi[0][2] + Gj[0][2]) * rz);
+        R gy = (R)0.5 * ((Gi[1][0] + Gj[1][0]) * rx + (Gi[1][1] + Gj[1][1]) * ry + (Gi[1][2] + Gj[1][2]) * rz);
+        R gz = (R)0.5 * ((Gi[2][0] + Gj[2][0]) * rx + (Gi[2][1] + Gj[2][1]) * ry + (Gi[2][2] + Gj[2][2]) * rz);
+        R q = rn * (R)F.inv_h, om = (R)1 - q;
+        R sw = wq<R>(F, b, b.V, j) * ((R)F.alpha * om * om * om * om * ((R)4 * q + (R)1));
+        R dyx = (R)(yj.x - yi.x), dyy = (R)(yj.y - yi.y), dyz = (R)(yj.z - yi.z);
+        R c = sw * (gx * ex + gy * ey + gz * ez);
+        R axx = sw * ex * ex, axy = sw * ex * ey, ayy = sw * ey * ey;
+        A[0] = axx; A[cstride] = axy; A[2 * cstride] = ayy;
+        hx += axx * dyx + axy * dyy; hy += axy * dyx + ayy * dyy;
+        if (!two_d) {
+            R azz = sw * ez * ez, axz = sw * ex * ez, ayz = sw * ey * ez;
+            A[3 * cstride] = azz; A[4 * cstride] = axz; A[5 * cstride] = ayz;
+            hx += axz * dyz; hy += ayz * dyz; hz += axz * dyx + ayz * dyy + azz * dyz;
+        }
+        bx += c * ex; by += c * ey; bz += c * ez;
+    }
+    // a later stage: A_ij (M y_j - y_i) from the slot
+    __device__ void pair_fast(int j, int mir, size_t slot, size_t cstride, bool two_d) {
+        double4 yj = a.ysrc[j];
+        if (mir & 1) yj.y = -yj.y;
+        if (mir & 2) yj.z = -yj.z;
+        const R* A = a.hgA + slot;
+        R dyx = (R)(yj.x - yi.x), dyy = (R)(yj.y - yi.y);
+        R axx = A[0], axy = A[cstride], ayy = A[2 * cstride];
+        hx += axx * dyx + axy * dyy; hy += axy * dyx + ayy * dyy;
+        if (!two_d) {
+            R dyz = (R)(yj.z - yi.z);
+            R azz = A[3 * cstride], axz = A[4 * cstride], ayz = A[5 * cstride];
+            hx += axz * dyz; hy += ayz * dyz; hz += axz * dyx + ayz * dyy + azz * dyz;
+        }
+    }
     template <int LPP> __device__ void lane_reduce() {
         if (LPP == 1) return;
         hx = lane_sum<LPP>(hx); hy = lane_sum<LPP>(hy); hz = lane_sum<LPP>(hz);
+        if (a.hg_mode == 1) { bx = lane_sum<LPP>(bx); by = lane_sum<LPP>(by); bz = lane_sum<LPP>(bz); }
+    }
+    // fold the stage-invariant source in (mode 2) or keep it for the later stages (mode 1)
+    __device__ void source(int i, bool listed) {
+        if (!listed) return;
+        if (a.hg_mode == 1) {
+            double4 v; v.x = (double)bx; v.y = (double)by; v.z = (double)bz; v.w = 0.0;
+            a.hgb[i] = v;
+            hx += bx; hy += by; hz += bz;
+        } else if (a.hg_mode == 2) {
+            double4 v = a.hgb[i];
+            hx += (R)v.x; hy += (R)v.y; hz += (R)v.z;
+        }
     }
     __device__ void end(int i) {
         const DFluid& F = a.F;
@@ -1334,6 +1405,53 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_list(DGrid g, int n, typename Bod
     body.reduce(red, blockIdx.x);
 }
 
+// One hourglass sub-step over the list.  Mode 1 (the first stage) evaluates each pair in
+// full and stores its stage-invariant operator; mode 2 (later stages) applies the stored
+// operator to the new stage field.  Particles whose list overflowed traverse the cells
+// and evaluate in full at every stage.
+template <typename R, int LPP>
+__global__ __launch_bounds__(SPH_BLOCK) void k_hg_list(DGrid g, int n, SolidArgs<R> a, NList nl) {
+    if (a.skip()) return;   // uniform over the grid
+    HgBody<R> body(a);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / LPP, lane = t % LPP;
+    const bool act = i < n;
+    const bool two_d = (a.F.flags & SPH_TWO_D) != 0;
+    const size_t cstride = (size_t)nl.nmax * nl.npad;
+    bool listed = false;
+    if (act) {
+        double xi, yi, zi;
+        body.begin(i, xi, yi, zi);
+        const int cnt = nl.cnt[i];
+        listed = cnt <= nl.nmax;
+        if (listed) {
+            for (int k = lane; k < cnt; k += LPP) {
+                const size_t slot = (size_t)k * nl.npad + i;
+                const int e = nl.idx[slot];
+                const int j = e & 0x3fffffff, mir = (e >> 30) & 3;
+                if (a.hg_mode == 2) {
+                    body.pair_fast(j, mir, slot, cstride, two_d);
+                } else {
+                    const double4 xj = a.X[j];
+                    double dx = min_image(g, 0, xi - xj.x);
+                    double dy = (mir & 1) ? -(-yi - xj.y) : min_image(g, 1, yi - xj.y);
+                    double dz = (mir & 2) ? -(-zi - xj.z) : min_image(g, 2, zi - xj.z);
+                    body.pair_store(j, mir, dx, dy, dz, slot, cstride);
+                }
+            }
+        } else if (lane == 0) {
+            auto f = [&](int j, int mir, double rx, double ry, double rz) { body.pair(j, mir, rx, ry, rz); };
+            Traverse tr{g, a.s.start, a.s.cellid, a.X, a.any_out()};
+            tr(i, xi, yi, zi, f);
+        }
+    }
+    body.template lane_reduce<LPP>();
+    if (act && lane == 0) {
+        body.source(i, listed);
+        body.end(i);
+    }
+}
+
 // ====================================================================================
 // host side of the solid step
 // ====================================================================================
@@ -1412,6 +1530,7 @@ static SolidArgs<R> solid_args(sph_ctx* c, sph_state* st, sph_solid* sd, const D
     a.X = (const double4*)st->buf[st->cur].x;
     a.pstride = c->pstride;
     a.ysrc = nullptr; a.ydst = nullptr; a.stage = 0;
+    a.hgA = nullptr; a.hgb = nullptr; a.hg_mode = 0;
     return a;
 }
 
@@ -1501,6 +1620,20 @@ static int launch_list_pass(sph_ctx* c, const typename Body::Args& a, hipStream_
 }
 
 template <typename R>
+static void launch_hg_pass(sph_ctx* c, const SolidArgs<R>& a, hipStream_t hs) {
+    const int n = (int)c->dims.n;
+    const int lpp = c->lpp[PASS_HG];
+    const int blocks = (int)(((int64_t)n * lpp + SPH_BLOCK - 1) / SPH_BLOCK);
+    dim3 gb(blocks), bb(SPH_BLOCK);
+    switch (lpp) {
+    case 8: hipLaunchKernelGGL((k_hg_list<R, 8>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
+    case 4: hipLaunchKernelGGL((k_hg_list<R, 4>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
+    case 2: hipLaunchKernelGGL((k_hg_list<R, 2>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
+    default: hipLaunchKernelGGL((k_hg_list<R, 1>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
+    }
+}
+
+template <typename R>
 static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F, const DSolidP& S,
                       hipStream_t hs) {
     const int nb = c->nblk;
@@ -1530,12 +1663,20 @@ static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F,
         // the count is a device number (k_dt_solid); launch the most any dt can need and
         // let the passes beyond it return at once, so the host never waits on the device
         SolidArgs<R> a = solid_args<R>(c, st, sd, F, S);
+        a.hgA = (R*)c->hg_pair;
+        a.hgb = c->hg_src;
         for (int k = 0; k < S.sts_bound; ++k) {
             a.ysrc = (k & 1) ? a.sd.y1 : a.sd.y0;
             a.ydst = (k & 1) ? a.sd.y0 : a.sd.y1;
             a.stage = k;
             timer_begin(c, 6, hs);
-            launch_list_pass<HgBody<R>>(c, a, hs, PASS_HG);
+            if (c->hg_pairs) {
+                a.hg_mode = k == 0 ? 1 : 2;
+                launch_hg_pass<R>(c, a, hs);
+            } else {
+                a.hg_mode = 0;
+                launch_list_pass<HgBody<R>>(c, a, hs, PASS_HG);
+            }
             timer_end(c, 6, hs);
         }
         hipLaunchKernelGGL((k_sts_kick2<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st),
@@ -1592,6 +1733,12 @@ extern "C" int sph_solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const sp
         HIPCHECK(c, hipMalloc((void**)&c->nl.cnt, sizeof(int32_t) * (size_t)npad));
         c->nl.nmax = nmax;
         c->nl.npad = npad;
+    }
+    if (S->sts && S->hourglass > 0.0 && c->hg_pairs && !c->hg_pair) {
+        const int ncomp = (c->dims.flags & SPH_TWO_D) ? 3 : 6;
+        const size_t rsz = c->dims.precision == SPH_PREC_MIXED ? sizeof(float) : sizeof(double);
+        HIPCHECK(c, hipMalloc(&c->hg_pair, rsz * ncomp * (size_t)c->nl.nmax * c->nl.npad));
+        HIPCHECK(c, hipMalloc((void**)&c->hg_src, sizeof(double4) * (size_t)c->nl.npad));
     }
     DFluid F = make_fluid(c, P);
     DSolidP D = make_solid_params(c, P, S);

@@ -1286,7 +1286,7 @@ static int check_launch(sph_ctx* c, const char* what) {
     return SPH_OK;
 }
 
-static const int TIMER_SLOTS = 64;
+static const int TIMER_SLOTS = 1024;
 
 static void timer_begin(sph_ctx* c, int k, hipStream_t st) {
     if (!c->timing) return;
@@ -1371,7 +1371,7 @@ extern "C" int sph_ctx_create(sph_ctx** out, const sph_dims* d) {
         c->tiles.pts = B * ppc;
         // per pass: the passes with a heavy per-particle epilogue keep fewer lanes (the
         // epilogue runs on one lane of each group)
-        const int cap[PASS_KINDS] = {2, 4, 8, 8, 2, 4, 8, 8};
+        const int cap[PASS_KINDS] = {2, 4, 8, 8, 4, 2, 8, 8};
         static const char* names[PASS_KINDS] = {"VOLUME", "FORCE", "SHIFT", "PAIRS", "STRESS",
                                                 "SFORCE", "GRADV", "HG"};
         for (int k = 0; k < PASS_KINDS; ++k) {

@@ -175,6 +175,96 @@ struct Traverse {
     }
 };
 
+// The same traversal shared by L consecutive lanes of a wave: each candidate range is
+// taken L at a time, lane l testing candidate b + l, and F(ok, j, mir, rx, ry, rz) is
+// called by every lane of the group the same number of times (so it may ballot), with
+// `ok` the neighbour predicate of that lane's candidate.  Visiting the accepted
+// candidates of each round in lane order reproduces the serial traversal's order.
+template <int L> struct TraverseLanes {
+    const DGrid& g;
+    const int* __restrict__ start;
+    const int4* __restrict__ cellid;
+    const double4* __restrict__ X;
+    bool any_out;
+    int lane;
+
+    template <class F>
+    __device__ __forceinline__ void range(int b, int e, bool check, int cx, int cy, int cz, int i,
+                                          int mir, double tx, double ty, double tz, F& f) const {
+        for (int jb = b; jb < e; jb += L) {
+            const int j = jb + lane;
+            bool ok = j < e;
+            double dx = 0.0, dy = 0.0, dz = 0.0;
+            if (ok && check) {
+                int4 cj = cellid[j];
+                if (cj.x != cx || cj.y != cy || cj.z != cz) ok = false;
+            }
+            if (ok && mir == 0 && j == i) ok = false;
+            if (ok) {
+                double4 xj = X[j];
+                dx = min_image(g, 0, tx - xj.x);
+                dy = min_image(g, 1, ty - xj.y);
+                dz = min_image(g, 2, tz - xj.z);
+                ok = dist2_exact(dx, dy, dz) < g.h2;
+                if (mir & 1) dy = -dy;
+                if (mir & 2) dz = -dz;
+            }
+            f(ok, j, mir, dx, dy, dz);
+        }
+    }
+
+    template <class F>
+    __device__ __forceinline__ void cell(int cx, int cy, int cz, int i, int mir, double tx, double ty,
+                                         double tz, F& f) const {
+        if (in_box(g, cx, cy, cz)) {
+            int k = dense_key(g, cx, cy, cz);
+            range(start[k], start[k + 1], false, cx, cy, cz, i, mir, tx, ty, tz, f);
+        } else if (any_out) {
+            int k = g.nbox + (int)hash_cell(cx, cy, cz, g.hash_bits);
+            range(start[k], start[k + 1], true, cx, cy, cz, i, mir, tx, ty, tz, f);
+        }
+    }
+
+    // the cell order of Traverse::operator()
+    template <class F>
+    __device__ __forceinline__ void operator()(int i, double xi, double yi, double zi, F& f) const {
+        if (!finite3(xi, yi, zi)) return;
+        int cx = cell1(g, 0, xi), cy = cell1(g, 1, yi), cz = cell1(g, 2, zi);
+        const bool two_d = (g.flags & SPH_TWO_D) != 0;
+        const bool px = (g.periodic & 1) != 0;
+        for (int oz = -1; oz <= 1; ++oz) {
+            if (two_d && oz != 0 && cz == 0 && g.nb[2] == 1 && g.lo[2] == 0 && !any_out) continue;
+            int zz = wrap_cell(g, 2, cz + oz);
+            for (int oy = -1; oy <= 1; ++oy) {
+                int yy = wrap_cell(g, 1, cy + oy);
+                bool yz_in = yy >= g.lo[1] && yy < g.lo[1] + g.nb[1] && zz >= g.lo[2] &&
+                             zz < g.lo[2] + g.nb[2];
+                if (yz_in && !px && cx - 1 >= g.lo[0] && cx + 1 < g.lo[0] + g.nb[0]) {
+                    int k = dense_key(g, cx - 1, yy, zz);
+                    range(start[k], start[k + 3], false, 0, 0, 0, i, 0, xi, yi, zi, f);
+                } else {
+                    for (int ox = -1; ox <= 1; ++ox)
+                        cell(wrap_cell(g, 0, cx + ox), yy, zz, i, 0, xi, yi, zi, f);
+                }
+            }
+        }
+        const bool my = (g.flags & SPH_MIRROR_Y) != 0, mz = (g.flags & SPH_MIRROR_Z) != 0;
+        if (my && yi < g.h) {
+            for (int oz = -1; oz <= 1; ++oz)
+                for (int ox = -1; ox <= 1; ++ox)
+                    cell(wrap_cell(g, 0, cx + ox), 0, wrap_cell(g, 2, cz + oz), i, 1, xi, -yi, zi, f);
+        }
+        if (mz && zi < g.h) {
+            for (int oy = -1; oy <= 1; ++oy)
+                for (int ox = -1; ox <= 1; ++ox)
+                    cell(wrap_cell(g, 0, cx + ox), wrap_cell(g, 1, cy + oy), 0, i, 2, xi, yi, -zi, f);
+        }
+        if (my && mz && yi < g.h && zi < g.h) {
+            for (int ox = -1; ox <= 1; ++ox) cell(wrap_cell(g, 0, cx + ox), 0, 0, i, 3, xi, -yi, -zi, f);
+        }
+    }
+};
+
 // ------------------------------------------------------------------------------------
 // symmetric 3x3 eigenproblem by cyclic Jacobi rotations (stands in for ti.sym_eig) and
 // the regularised inverse of spd_inverse.py
@@ -310,6 +400,26 @@ __device__ __forceinline__ double block_max(double v, double* lds) {
     }
     __syncthreads();
     return r;
+}
+
+// multi-value block sum: v[q] of every thread summed into thread 0's v[q]
+template <int K>
+__device__ __forceinline__ void block_sum_n(double* v, double* lds /* K * waves */) {
+    const int W = blockDim.x / SPH_WAVE, w = threadIdx.x / SPH_WAVE, l = threadIdx.x % SPH_WAVE;
+#pragma unroll
+    for (int q = 0; q < K; ++q) v[q] = wave_sum(v[q]);
+    if (l == 0)
+#pragma unroll
+        for (int q = 0; q < K; ++q) lds[q * W + w] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            double r = 0.0;
+            for (int k = 0; k < W; ++k) r += lds[q * W + k];
+            v[q] = r;
+        }
+    __syncthreads();
 }
 
 // atomic max of a non-negative double through its bit pattern

@@ -26,6 +26,7 @@
 #define SPH_S_PEND_HG   24
 #define SPH_S_KE_POST   25
 #define SPH_MAX_CONS    16
+#define SPH_LIST_LANES  4     // lanes per particle of the neighbour-list build
 
 // scene constants of the solid step, folded on the host
 struct DSolidP {
@@ -1068,18 +1069,17 @@ __global__ void k_solid_begin(double* scal, DSolidP S, double* cons, int ncons) 
 
 // compute_adaptive_dt (DSPH.py:1365-1430) with the solid's caps: clamp_dt_hourglass
 // (SOLID.py:4158-4161) or sts_stage_count (4510-4539); banks pass SA's works.
-__global__ __launch_bounds__(1024) void k_dt_solid(DFluid F, DSolidP S, double* scal, DScratch s, int nparts_a,
-                                                   int nparts_b, int pstride) {
-    __shared__ double red[1024 / SPH_WAVE];
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+__global__ __launch_bounds__(SPH_BLOCK) void k_dt_solid(DFluid F, DSolidP S, double* scal, DScratch s,
+                                                         int nparts_a, int nparts_b, int pstride) {
+    __shared__ double red[8 * SPH_BLOCK / SPH_WAVE];
+    double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const int slots[8] = {PART_WDIFF, PART_PLASTIC, PART_IEPAIR, PART_IEADV, PART_IEDIFF, PART_IEALE,
                           PART_VISCS, PART_HGW};
     for (int k = threadIdx.x; k < nparts_a; k += blockDim.x)
-        for (int q = 0; q < 6; ++q) acc[q] += s.part[slots[q] * pstride + k];
+        for (int q = 0; q < 6; ++q) t[q] += s.part[slots[q] * pstride + k];
     for (int k = threadIdx.x; k < nparts_b; k += blockDim.x)
-        for (int q = 6; q < 8; ++q) acc[q] += s.part[slots[q] * pstride + k];
-    double t[8];
-    for (int q = 0; q < 8; ++q) t[q] = block_sum(acc[q], red);
+        for (int q = 6; q < 8; ++q) t[q] += s.part[slots[q] * pstride + k];
+    block_sum_n<8>(t, red);
     if (threadIdx.x == 0) {
         double dt_prev = scal[SPH_S_DT];
         scal[SPH_S_DIFF_WORK] += dt_prev * t[0];
@@ -1279,22 +1279,23 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_constraints(DFluid F, DState<R> s
 
 // the step's totals: wall work, the leftover (deferred one step), the damper's kinetic
 // energy and the split kick's cross term (SOLID.py:4590-4606), the grip forces, the clock
-__global__ __launch_bounds__(1024) void k_bank_solid(double* scal, DScratch s, DSolidP S, int nblk, int pstride,
-                                                     const double* cons_part, double* cons, int ncons,
-                                                     int ma_global) {
-    __shared__ double red[1024 / SPH_WAVE];
-    double a[5] = {0, 0, 0, 0, 0};
+__global__ __launch_bounds__(SPH_BLOCK) void k_bank_solid(double* scal, DScratch s, DSolidP S, int nblk,
+                                                           int pstride, const double* cons_part, double* cons,
+                                                           int ncons, int ma_global) {
+    __shared__ double red[5 * SPH_BLOCK / SPH_WAVE];
+    double t[5] = {0, 0, 0, 0, 0};
     const int slots[5] = {PART_WALL, PART_LEFT, PART_KE, PART_KE2, PART_CROSS};
     const int use = S.sts ? 5 : 2;
     for (int k = threadIdx.x; k < nblk; k += blockDim.x)
-        for (int q = 0; q < use; ++q) a[q] += s.part[slots[q] * pstride + k];
-    double t[5] = {0, 0, 0, 0, 0};
-    for (int q = 0; q < use; ++q) t[q] = block_sum(a[q], red);
-    for (int c = 0; c < 3 * ncons; ++c) {
-        double v = 0.0;
-        for (int k = threadIdx.x; k < nblk; k += blockDim.x) v += cons_part[c * nblk + k];
-        double tv = block_sum(v, red);
-        if (threadIdx.x == 0) cons[6 * (c / 3) + (c % 3)] = tv;
+        for (int q = 0; q < use; ++q) t[q] += s.part[slots[q] * pstride + k];
+    block_sum_n<5>(t, red);
+    for (int c = 0; c < ncons; ++c) {
+        double v[3] = {0.0, 0.0, 0.0};
+        for (int k = threadIdx.x; k < nblk; k += blockDim.x)
+            for (int q = 0; q < 3; ++q) v[q] += cons_part[(3 * c + q) * nblk + k];
+        block_sum_n<3>(v, red);
+        if (threadIdx.x == 0)
+            for (int q = 0; q < 3; ++q) cons[6 * c + q] = v[q];
     }
     if (threadIdx.x == 0) {
         scal[SPH_S_WALL_WORK] += t[0];
@@ -1352,21 +1353,32 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_solid_reduce(int n, DBuf<R> b, DS
 // order; the passes then read the list instead of filtering ~3x as many candidates.  A
 // particle with more than `nmax` neighbours keeps the generic traversal.
 // ------------------------------------------------------------------------------------
+// L lanes per particle: each round of L candidates is ranked by a ballot over the group,
+// so the list is the serial traversal's, in its order.
+template <int L>
 __global__ __launch_bounds__(SPH_BLOCK) void k_build_list(DGrid g, int n, const int* __restrict__ start,
                                                           const int4* __restrict__ cellid,
                                                           const double4* __restrict__ X, const double* scal,
                                                           NList nl) {
-    int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
-    if (i >= n) return;
+    const int t = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    const int i = t / L, lane = t % L;
+    if (i >= n) return;   // whole groups: SPH_BLOCK is a multiple of L
+    const int gbase = (threadIdx.x % SPH_WAVE) - lane;
+    const unsigned long long gmask = (L == 64) ? ~0ull : ((1ull << L) - 1ull);
+    const unsigned long long below = (1ull << lane) - 1ull;
     double4 xi = X[i];
     int c = 0;
-    auto f = [&](int j, int mir, double, double, double) {
-        if (c < nl.nmax) nl.idx[(size_t)c * nl.npad + i] = j | (mir << 30);
-        ++c;
+    auto f = [&](bool ok, int j, int mir, double, double, double) {
+        const unsigned long long m = (__ballot(ok) >> gbase) & gmask;
+        if (ok) {
+            const int slot = c + __popcll(m & below);
+            if (slot < nl.nmax) nl.idx[(size_t)slot * nl.npad + i] = j | (mir << 30);
+        }
+        c += __popcll(m);
     };
-    Traverse tr{g, start, cellid, X, scal[SPH_S_OUT_OF_BOX] > 0.5};
+    TraverseLanes<L> tr{g, start, cellid, X, scal[SPH_S_OUT_OF_BOX] > 0.5, lane};
     tr(i, xi.x, xi.y, xi.z, f);
-    nl.cnt[i] = c;
+    if (lane == 0) nl.cnt[i] = c;
 }
 
 // One pair pass over the list: LPP consecutive lanes share a particle and take every
@@ -1639,9 +1651,13 @@ static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F,
     const int nb = c->nblk;
     int rc;
     // the neighbour list of the step's positions
-    hipLaunchKernelGGL(k_build_list, dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, (int)c->dims.n,
-                       (const int*)c->s.start, (const int4*)c->s.cellid, (const double4*)st->buf[st->cur].x,
-                       (const double*)st->scal, c->nl);
+    {
+        const int64_t nt = (int64_t)c->dims.n * SPH_LIST_LANES;
+        hipLaunchKernelGGL((k_build_list<SPH_LIST_LANES>), dim3((int)((nt + SPH_BLOCK - 1) / SPH_BLOCK)),
+                           dim3(SPH_BLOCK), 0, hs, c->grid, (int)c->dims.n, (const int*)c->s.start,
+                           (const int4*)c->s.cellid, (const double4*)st->buf[st->cur].x,
+                           (const double*)st->scal, c->nl);
+    }
     // pass SA: reads history[hcur], writes history[1 - hcur]
     timer_begin(c, 4, hs);
     int slots_a = launch_list_pass<StressBody<R>>(c, solid_args<R>(c, st, sd, F, S), hs, PASS_STRESS);
@@ -1652,7 +1668,7 @@ static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F,
     timer_begin(c, 5, hs);
     int slots_b = launch_list_pass<SolidForceBody<R>>(c, solid_args<R>(c, st, sd, F, S), hs, PASS_SFORCE);
     timer_end(c, 5, hs);
-    hipLaunchKernelGGL(k_dt_solid, dim3(1), dim3(1024), 0, hs, F, S, st->scal, c->s, slots_a, slots_b,
+    hipLaunchKernelGGL(k_dt_solid, dim3(1), dim3(SPH_BLOCK), 0, hs, F, S, st->scal, c->s, slots_a, slots_b,
                        c->pstride);
     if ((rc = check_launch(c, "solid force pass"))) return rc;
     if (S.sts) {
@@ -1697,7 +1713,7 @@ static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F,
     }
     hipLaunchKernelGGL((k_finish<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st), c->s,
                        c->pstride, (R*)sd->sig[sd->hcur]);
-    hipLaunchKernelGGL(k_bank_solid, dim3(1), dim3(1024), 0, hs, st->scal, c->s, S, nb, c->pstride,
+    hipLaunchKernelGGL(k_bank_solid, dim3(1), dim3(SPH_BLOCK), 0, hs, st->scal, c->s, S, nb, c->pstride,
                        (const double*)c->cons_part, sd->cons, S.ncons, F.ma_mode == 1 ? 1 : 0);
     return check_launch(c, "solid finish");
 }

@@ -84,6 +84,9 @@ extern "C" {
 #define SPH_R_KINETIC_OFFGRID 6 /* kinetic energy of non-finite particles   -> out[0]   */
 #define SPH_R_INTERNAL       7  /* solid: Sum m e_int                       -> out[0]   */
 #define SPH_R_ELASTIC        8  /* solid: Sum V (K theta^2/2 + s:s/(4G))    -> out[0]   */
+#define SPH_R_COMPRESSION    9  /* solid: Sum m e_cold(rho), barotropic rows -> out[0]   */
+#define SPH_R_COMPRESSION_CLAMPED 10 /* the same, zero below rho0         -> out[0]   */
+#define SPH_R_NEG_INTERNAL  11  /* solid: Sum m min(e_int, 0)               -> out[0]   */
 
 /* ---- materials of the solid step (materials.py:96-97, 506-548) -------------------- */
 #define SPH_EOS_LINEAR       0  /* p = K (rho/rho0 - 1): Tait at gamma = 1, S = K      */
@@ -310,8 +313,12 @@ int  sph_solid_sort(sph_ctx* ctx, sph_state* st, sph_solid* sd, void* stream);
 int  sph_solid_step(sph_ctx* ctx, sph_state* st, sph_solid* sd, const sph_fluid_params* prm,
                     const sph_solid_params* sprm, void* stream);
 
-/* f64 solid reductions (SPH_R_INTERNAL: compute_internal_energy; SPH_R_ELASTIC:
- * compute_elastic_energy, SOLID.py:2295-2322) into device double out[0]. */
+/* f64 solid reductions into device double out[0]: SPH_R_INTERNAL
+ * (compute_internal_energy), SPH_R_ELASTIC (compute_elastic_energy, SOLID.py:2295-2322),
+ * SPH_R_COMPRESSION / _CLAMPED (compute_compression_energy over the linear, Tait and
+ * Gamma0 = 0 Mie-Grueneisen rows, without / with the clamp below rho0 that applies when
+ * negative pressure is not allowed, SOLID.py:2474-2567), SPH_R_NEG_INTERNAL
+ * (compute_negative_internal_energy, SOLID.py:2435-2472). */
 int  sph_solid_reduce(sph_ctx* ctx, const sph_state* st, const sph_solid* sd, int32_t which,
                       double* out, void* stream);
 

@@ -1791,6 +1791,11 @@ double oracle_solid_reduce(const or_state* s, int which) {
     for (int64_t i = 0; i < s->n; ++i) {
         const sph_material* M = or_mat(s, i);
         if (which == SPH_R_INTERNAL) out += s->m[i] * s->e_int[i];
+        else if (which == SPH_R_NEG_INTERNAL) { if (s->e_int[i] < 0.0) out += s->m[i] * s->e_int[i]; }
+        else if (which == SPH_R_COMPRESSION || which == SPH_R_COMPRESSION_CLAMPED) {
+            if (M->eos_kind != SPH_EOS_MG || M->mg_C4 == 0.0)
+                out += s->m[i] * oracle_cold_energy(M, s->m[i] / s->V[i], which == SPH_R_COMPRESSION);
+        }
         else if (which == SPH_R_ELASTIC && M->G > 0.0) {
             double V = s->V[i], rho = s->m[i] / V, th = (rho - M->rho0) / M->rho0;
             const double* q = s->sig + 6 * i;

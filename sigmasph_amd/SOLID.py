@@ -87,6 +87,7 @@ class SolidSettings:
         self.default_material = ps.default_material
         self.c_p = float(self.default_material.c_p)
         self.fs.c_signal = self.c_p
+        self.c_signal = self.c_p
         self.CFL = self.fs.CFL
         self.dt_initial = self.CFL * ps.h_min / self.c_p                 # SOLID.py:317
         if c("timeStepSize") is not None:
@@ -131,6 +132,13 @@ class SolidSettings:
         self.solid_visc_work_on = ((True if track is None else bool(track))
                                    and (self.fs.viscosity > 0.0 or self.av_on))
         self.thermal_on = any(m.cv > 0.0 for m in mats)
+        # the rows with a cold energy of the density alone (SOLID.py:175-190), and the
+        # guard on e for the EOS that reads it (SOLID.py:1452)
+        self.compression_energy_complete = all(
+            m.eos_kind != N.EOS_MG or m.mg["C4"] == 0.0 for m in mats)
+        self.mg_on = any(m.eos_kind == N.EOS_MG for m in mats)
+        self.e_guard_on = self.mg_on
+        self.allow_negative_pressure = bool(c("allowNegativePressure"))
         self.plastic = any(m.plastic_kind != N.PLASTIC_NONE for m in mats)
         self.ncons = len(getattr(ps, "constraints", []) or [])
         self.mat_table = material_table(self.materials)
@@ -261,6 +269,18 @@ class HypoElasticSolver(SPHBase):
     def compute_diffusion_work(self):
         return float(self.diff_work[None])
 
+    def compute_integrated_gap(self):
+        return self.compute_diffusion_work()
+
+    def compute_compression_energy(self):
+        """Sum m e_cold(rho) over the barotropic rows (SOLID.py:2474-2509)."""
+        return self._solid_reduce(N.SPH_R_COMPRESSION if self.allow_negative_pressure
+                                  else N.SPH_R_COMPRESSION_CLAMPED)
+
+    def compute_negative_internal_energy(self):
+        """Sum m min(e_int, 0) (SOLID.py:2435-2472)."""
+        return self._solid_reduce(N.SPH_R_NEG_INTERNAL)
+
     def energy_budget(self):
         ke = self.compute_kinetic_energy()
         ie = self.compute_internal_energy()
@@ -272,5 +292,9 @@ class HypoElasticSolver(SPHBase):
                 "ie_ale": float(self.ie_ale[None]), "ie_ale_adv": float(self.ie_ale_adv[None]),
                 "ie_ale_diff": float(self.ie_ale_diff[None]),
                 "wall": self.compute_wall_energy(), "visc": self.compute_viscous_energy(),
+                "comp": self.compute_compression_energy(),
+                "comp_complete": self.compression_energy_complete,
+                "e_neg": self.compute_negative_internal_energy(),
                 "leftover": lw, "leftover_raw": lw, "wd": self.compute_diffusion_work(),
+                "gap_integrated": self.compute_integrated_gap(),
                 "total": ke + ie + hg + pe - lw}

@@ -56,6 +56,9 @@ SPH_R_TAIT_ENERGY = 5
 SPH_R_KINETIC_OFFGRID = 6
 SPH_R_INTERNAL = 7
 SPH_R_ELASTIC = 8
+SPH_R_COMPRESSION = 9
+SPH_R_COMPRESSION_CLAMPED = 10
+SPH_R_NEG_INTERNAL = 11
 # material kinds
 EOS_LINEAR, EOS_TAIT, EOS_MG = 0, 1, 3
 PLASTIC_NONE, PLASTIC_LINEAR, PLASTIC_JC = 0, 1, 2

@@ -129,13 +129,30 @@ PARAMS = [
     P("HOURGLASS_STS_STAGES", "Hourglass.superTimeStepping", "stages", "int", 0),
     P("HOURGLASS_STS_REFRESH", "Hourglass.superTimeStepping", "refreshGradV", "bool", True),
     P("HOURGLASS_STS_SAFETY", "Hourglass.superTimeStepping", "safety", "float", 0.8),
+    # Output (params.py:1226-1316): read by the driver, not by the step
+    P("exportTxt", "Output", "txt", "bool", False),
+    P("exportNetcdf", "Output", "netcdf", "bool", False),
+    P("exportRelaxationNetcdf", "Output", "relaxationNetcdf", "bool", False),
+    P("exportRelaxationInterval", "Output", "relaxationInterval", "int", 1),
+    P("trackDeformationGradient", "Output", "deformationGradient", "bool", False),
+    P("exportPly", "Output", "ply", "bool", False),
+    P("exportObj", "Output", "obj", "bool", False),
+    P("exportFrame", "Output", "frame", "bool", False),
+    P("exportConstraints", "Output", "constraints", "bool", None),
+    P("exportConstraintsInterval", "Output", "constraintsInterval", "int", 1),
+    P("exportInterval", "Output", "interval", "int", None),
+    P("colorField", "Output", "colorField", "str", None,
+      choices=("pressure", "divergence", "vonMises", "epsPlastic", "burnFraction", "internalEnergy",
+               "damage", "porosity", "damageTension", "jcOmega")),
+    P("cameraMode", "Output", "cameraMode", "str", "iso", choices=("iso", "front")),
+    P("invisibleObjects", "Output", "invisibleObjects", "list", None),
     # legacy flat material constants (derived from Materials[] when it is given)
     P("density0", "Materials", "density0", "float", 1000.0),
     P("c0", "Materials", "c0", "float", None),
     P("exponent", "Materials", "exponent", "float", 7.0),
 ]
 BY_NAME = {p.name: p for p in PARAMS}
-CARDS = ("Domain", "Time", "Solver", "Load", "Diffusion", "Shifting", "Hourglass")
+CARDS = ("Domain", "Time", "Solver", "Load", "Diffusion", "Shifting", "Hourglass", "Output")
 #: sub-cards; those whose presence switches a feature on name their flag
 #: (params.py:1326-1340)
 SUBCARDS = ("Hourglass.superTimeStepping", "Solver.bulkViscosity", "Solver.artificialViscosity")
@@ -196,6 +213,10 @@ def _coerce(p, value, where, errors):
             if not isinstance(value, (list, tuple)) or len(value) != 3:
                 raise TypeError
             return [float(v) for v in value]
+        if t == "list":
+            if not isinstance(value, (list, tuple)):
+                raise TypeError
+            return list(value)
         if t == "str":
             if not isinstance(value, str):
                 raise TypeError

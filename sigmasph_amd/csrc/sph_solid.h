@@ -1323,6 +1323,15 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_solid_reduce(int n, DBuf<R> b, DS
         double m = (double)b.m[i];
         if (which == SPH_R_INTERNAL) {
             r0 = m * (double)sd.e_in[i];
+        } else if (which == SPH_R_NEG_INTERNAL) {
+            r0 = fmin(m * (double)sd.e_in[i], 0.0);
+        } else if (which == SPH_R_COMPRESSION || which == SPH_R_COMPRESSION_CLAMPED) {
+            // the barotropic rows: linear, Tait, and Mie-Grueneisen at Gamma0 = 0
+            const sph_material& M = sd.mat[sd.eos[i]];
+            if (M.eos_kind != SPH_EOS_MG || M.mg_C4 == 0.0) {
+                double V = (double)b.V[i] + (mixed ? (double)b.V_lo[i] : 0.0);
+                r0 = m * cold_energy(M, m / V, which == SPH_R_COMPRESSION ? 1 : 0);
+            }
         } else {
             const sph_material& M = sd.mat[sd.eos[i]];
             if (M.G > 0.0) {
@@ -1769,7 +1778,7 @@ extern "C" int sph_solid_reduce(sph_ctx* c, const sph_state* st, const sph_solid
     if (!c || !st || !sd) return SPH_EINVAL;
     int rc = check_state(c, st, false);
     if (rc) return rc;
-    if (!out || (which != SPH_R_INTERNAL && which != SPH_R_ELASTIC))
+    if (!out || which < SPH_R_INTERNAL || which > SPH_R_NEG_INTERNAL)
         return set_err(c, SPH_EINVAL, "bad solid reduce%s", "", 0);
     if (!solid_ok(sd, false) || !sd->mat) return set_err(c, SPH_EINVAL, "null solid field%s", "", 0);
     hipStream_t hs = (hipStream_t)stream;

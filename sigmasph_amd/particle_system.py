@@ -493,6 +493,35 @@ class ParticleSystem(SceneLayout):
         self.q_visc = Field(lambda: self._sderived["q_visc"], 1, "q_visc")
 
     # ---------------------------------------------------------------------------------
+    def dump(self, obj_id=None):
+        """The per-particle state of one object, or of every valid particle, as numpy
+        arrays in storage order (dump, particle_system.py:1111-1200): position,
+        velocity, divergence, lam, pressure, object_id, density and, for a solid, the
+        total stress components, von Mises, eps_plastic, eroded (nothing is eroded
+        here), e_int and temperature."""
+        obj = self.object_id.to_numpy()
+        mask = (obj >= 0) if obj_id is None else (obj == obj_id)
+        V = self.V.to_numpy().astype(np.float64)
+        if self.precision == "mixed":
+            V = V + self.V_lo.to_numpy()
+        p = self.pressure.to_numpy()[mask]
+        out = {"position": self.x.to_numpy()[mask], "velocity": self.v.to_numpy()[mask],
+               "divergence": self.divergence.to_numpy()[mask], "lam": self.lam.to_numpy()[mask],
+               "pressure": p, "object_id": obj[mask],
+               "density": self.m.to_numpy()[mask] / V[mask]}
+        if self.is_solid:
+            sd = self.sigma_dev.to_numpy()[mask].astype(np.float64)
+            sig = -p[:, None, None] * np.eye(3)[None] + sd
+            for name, (a, b) in (("xx", (0, 0)), ("yy", (1, 1)), ("zz", (2, 2)), ("xy", (0, 1)),
+                                 ("xz", (0, 2)), ("yz", (1, 2))):
+                out["sigma_" + name] = sig[:, a, b]
+            out["von_mises"] = np.sqrt(1.5 * np.einsum("kij,kij->k", sd, sd))
+            out["eps_plastic"] = self.eps_plastic.to_numpy()[mask]
+            out["eroded"] = np.zeros(int(mask.sum()), dtype=np.int32)
+            out["e_int"] = self.e_int.to_numpy()[mask]
+            out["temperature"] = self.temperature.to_numpy()[mask]
+        return out
+
     def _live(self):
         return self._bufs[self.state.cur]
 

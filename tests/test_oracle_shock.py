@@ -190,3 +190,36 @@ def test_dt_limits_on_a_converging_field(mode):
         lim = cfl * h / (Q + math.sqrt(Q * Q + cp * cp))
     assert lim < min(acoustic, force)
     assert dt == pytest.approx(lim, rel=1e-12)
+
+
+@pytest.mark.parametrize("mat", ["linear", "mg_gamma0", "mg"])
+def test_compression_energy_reduction(mat):
+    """Sum m e_cold(rho) over a uniformly compressed block: the Tait integral at gamma = 1
+    (S/rho0 (ln x + 1/x - 1)) for the linear row, the Mie-Grueneisen cold integral at
+    Gamma0 = 0, nothing for a row whose pressure reads e (Gamma0 > 0); the clamped form
+    is zero in tension; the negative internal energy is Sum m min(e, 0)."""
+    material = None
+    if mat != "linear":
+        g = 0.0 if mat == "mg_gamma0" else 2.0
+        material = {"name": "p", "density0": 7.9e-6,
+                    "eos": {"type": "mie_gruneisen", "c0": 4500.0, "s": 1.4, "gamma0": g},
+                    "strength": {"type": "hypoElastic", "youngsModulus": 200.0, "poissonRatio": 0.3}}
+    lay, st, P, S = solid_layout(solid_block(n=(6, 5, 1), material=material))
+    s = solid_oracle_state(lay, st)
+    row = st.mat_table[0]
+    for x in (1.02, 0.97):
+        s.V[:] = s.m / (row.rho0 * x)
+        got = O.solid_reduce(s, N.SPH_R_COMPRESSION)
+        clamped = O.solid_reduce(s, N.SPH_R_COMPRESSION_CLAMPED)
+        if mat == "linear":
+            want = s.m.sum() * row.eos_S / row.rho0 * (math.log(x) + 1.0 / x - 1.0)
+        elif mat == "mg_gamma0":
+            want = s.m.sum() * O.cold_energy(row, row.rho0 * x)
+        else:
+            want = 0.0
+        assert got == pytest.approx(want, rel=1e-12, abs=1e-300)
+        assert clamped == (pytest.approx(want, rel=1e-12, abs=1e-300) if x > 1.0 else 0.0)
+    rng = np.random.default_rng(3)
+    s.e_int[:] = rng.normal(size=s.n)
+    assert O.solid_reduce(s, N.SPH_R_NEG_INTERNAL) == pytest.approx(
+        np.sum(s.m * np.minimum(s.e_int, 0.0)), rel=1e-12)

@@ -24,6 +24,7 @@ import torch
 
 from . import _native as N
 from .fields import Field, Scalar, sym_field, grad_field
+from .geometry import region_from_spec
 
 
 def lattice_points(lower, size, diameter, dim=3):
@@ -35,16 +36,17 @@ def lattice_points(lower, size, diameter, dim=3):
 
 
 def carve(x, shape):
-    """Keep the lattice points inside a block shape.  One shape is built:
-    `gaugeDogbone`, a round bar with a parallel gauge in the (x, r) half-plane, centred
-    on x = 0: gauge of diameter `gaugeDiameter` and length `gaugeLength`, circular
-    fillets of radius `filletRadius` tangent to the gauge, heads of diameter
-    `headDiameter` and length `headLength`.  (The reference's own dogbone shapes,
-    geometry.py:17-371, have no parallel gauge.)"""
+    """Keep the lattice points inside a block shape: the reference's shapes
+    (`region_from_spec`, centred by default on the centroid of the lattice laid,
+    particle_system.py:299-318), or `gaugeDogbone`, a round bar with a parallel gauge in
+    the (x, r) half-plane, centred on x = 0: gauge of diameter `gaugeDiameter` and length
+    `gaugeLength`, circular fillets of radius `filletRadius` tangent to the gauge, heads
+    of diameter `headDiameter` and length `headLength` (the reference's dogbones have no
+    parallel gauge)."""
     kind = str(shape.get("type", "")).lower()
     if kind != "gaugedogbone":
-        raise NotImplementedError("block shape %r is not part of this build (gaugeDogbone is)"
-                                  % shape.get("type"))
+        region = region_from_spec(shape, x.mean(axis=0))
+        return x[region.contains(x)]
     Rh = 0.5 * float(shape["headDiameter"])
     Rg = 0.5 * float(shape["gaugeDiameter"])
     Lg = float(shape["gaugeLength"])

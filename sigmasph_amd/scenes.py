@@ -209,3 +209,46 @@ def dogbone_c2(dx0=0.1, elongation=3.0, rate=1.0, cfl=0.3):
              "components": "xy", "displacement": [0.5 * elongation, 0.0, 0.0]},
         ],
     }
+
+
+def tensile_axi_dogbone_nofail(dx0=0.1, duration=0.5, steps_per_cycle=50, output=None):
+    """The axisymmetric round-bar tension deck of the reference's dogbone resolution
+    study (data/scenes/tensile_axi_dogbone_nofail_dx0.1.json, every level differing only
+    in particleRadius and the axis offset start[1] = particleRadius): the `dogboneaxi`
+    shape with 10 mm heads 3.6 mm long, 6 mm gauge diameter, fillet radius 20.36 mm,
+    carved from a 40 x 40 mm block centred on x = 0; the library's stainless_steel
+    (Johnson-Cook); grips on |x| >= 10 mm pulled +-10 mm over a 0.9 ms smoothstep ramp;
+    CFL 0.3, Morris viscosity 0.02, delta 0.1, Colle shifting with ALE history, the rate
+    hourglass damper (alpha 4) super-time-stepped.  640 / 2508 / 10026 / 40114 particles
+    at dx0 = 0.4 / 0.2 / 0.1 / 0.05 mm."""
+    pr = 0.5 * dx0
+    sc = {
+        "Domain": {"particleRadius": pr, "supportRadiusFactor": 6.0, "axisymmetric": True,
+                   "boundary": "open"},
+        "Time": {"CFL": 0.3, "duration": duration, "maxSteps": 400000,
+                 "stepsPerRenderUpdate": steps_per_cycle},
+        "Solver": {"type": "hypoElastic", "viscosity": 0.02, "allowNegativePressure": True,
+                   "dampingCoefficient": 0.0, "kernelCorrection": False},
+        "Materials": [{"name": "specimen", "library": "stainless_steel"}],
+        "Load": {"gravitation": [0.0, 0.0, 0.0], "rampTime": 0.9},
+        "Diffusion": {"delta": 0.1},
+        "Shifting": {"enabled": True, "mode": "colle", "maMode": "fixed", "maFixed": 0.05, "R": 0.2,
+                     "n": 4, "maxShift": 0.05, "aleMomentum": True, "aleHistory": True},
+        "Hourglass": {"alpha": 4.0, "superTimeStepping": {"mode": "subcycle"}},
+        "Output": dict({"netcdf": False, "constraints": True, "constraintsInterval": 1, "txt": False,
+                        "interval": 20, "colorField": "epsPlastic", "cameraMode": "front"},
+                       **(output or {})),
+        "SolidBlocks": [{
+            "objectId": 0, "start": [0.0, pr, 0.0], "end": [40.0, 40.0, pr],
+            "translation": [-20.0, 0.0, 0.0], "scale": [1, 1, 1], "velocity": [0.0, 0.0, 0.0],
+            "density": 7.9e-06,
+            "shape": {"type": "dogboneaxi", "tabWidth": 10.0, "tabHeight": 3.6, "gaugeWidth": 6,
+                      "radius": 20.36},
+            "material": "specimen"}],
+        "Constraints": [
+            {"name": "left", "region": [[-20, 0, -1.0], [-10, 20, 1.0]], "components": "xy",
+             "displacement": [-10.0, 0.0, 0.0]},
+            {"name": "right", "region": [[10.0, 0.0, -1.0], [20, 20.0, 1.0]], "components": "xy",
+             "displacement": [10.0, 0.0, 0.0]}],
+    }
+    return sc

@@ -1216,7 +1216,7 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_sts_kick2(DGrid g, DFluid F, DSta
 template <typename R>
 __global__ __launch_bounds__(SPH_BLOCK) void k_constraints(DFluid F, DState<R> st, DSolid<R> sd, DScratch s,
                                                            double* cons_part, int ncons, int nblk) {
-    __shared__ double red[SPH_BLOCK / SPH_WAVE];
+    __shared__ double red[3 * SPH_BLOCK / SPH_WAVE];
     int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
     double fx = 0.0, fy = 0.0, fz = 0.0, vmax = 0.0;
     int cid = -1;
@@ -1261,15 +1261,11 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_constraints(DFluid F, DState<R> s
         }
     }
     for (int c = 0; c < ncons; ++c) {
-        bool mine = cid == c;
-        double a0 = block_sum(mine ? fx : 0.0, red);
-        double a1 = block_sum(mine ? fy : 0.0, red);
-        double a2 = block_sum(mine ? fz : 0.0, red);
-        if (threadIdx.x == 0) {
-            cons_part[(3 * c + 0) * nblk + blockIdx.x] = a0;
-            cons_part[(3 * c + 1) * nblk + blockIdx.x] = a1;
-            cons_part[(3 * c + 2) * nblk + blockIdx.x] = a2;
-        }
+        const bool mine = cid == c;
+        double a[3] = {mine ? fx : 0.0, mine ? fy : 0.0, mine ? fz : 0.0};
+        block_sum_n<3>(a, red);
+        if (threadIdx.x == 0)
+            for (int q = 0; q < 3; ++q) cons_part[(3 * c + q) * nblk + blockIdx.x] = a[q];
     }
     if (F.ma_mode == 1) {
         double bm = block_max(vmax, red);

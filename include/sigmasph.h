@@ -335,6 +335,15 @@ int  sph_reduce(sph_ctx* ctx, const sph_state* st, int32_t which, const double* 
  * 1: force pass, 2: shift pass, 3: sort, 4: solid stress pass, 5: solid force pass,
  * 6: hourglass sub-step pass).  Enabled by sph_timing(ctx, 1). */
 int  sph_timing(sph_ctx* ctx, int32_t enable);
+
+/* Whole-step graphs (no counterpart in the reference): with enable != 0, sph_solid_step
+ * on a non-null stream records its launches into a HIP graph the first time it sees a
+ * set of arguments (the state and solid structs with their live indices, the parameter
+ * blocks, the stream) and replays that graph when the same set returns -- a steady run
+ * alternates between two.  The step never waits on the device, so the replayed work is
+ * the eager step's, kernel for kernel.  Timing (sph_timing) takes the eager path.
+ * Disabling frees the captured graphs. */
+int  sph_graphs(sph_ctx* ctx, int32_t enable);
 int  sph_timing_read(sph_ctx* ctx, int32_t which, double* avg_ms, int64_t* launches);
 
 #ifdef __cplusplus

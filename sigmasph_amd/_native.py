@@ -183,7 +183,7 @@ EXPORTS = (
     "sph_fluid_step", "sph_fluid_pass_volume", "sph_fluid_pass_force",
     "sph_fluid_advect", "sph_fluid_pass_shift", "sph_fluid_finish",
     "sph_solid_sort", "sph_solid_step", "sph_solid_reduce",
-    "sph_reduce", "sph_timing", "sph_timing_read",
+    "sph_reduce", "sph_timing", "sph_timing_read", "sph_graphs",
 )
 
 _lib = None
@@ -229,6 +229,8 @@ def lib():
     L.sph_reduce.argtypes = [vp, P(SphState), ctypes.c_int32, vp, P(SphFluidParams), vp, vp]
     L.sph_timing.restype = ctypes.c_int
     L.sph_timing.argtypes = [vp, ctypes.c_int32]
+    L.sph_graphs.restype = ctypes.c_int
+    L.sph_graphs.argtypes = [vp, ctypes.c_int32]
     L.sph_timing_read.restype = ctypes.c_int
     L.sph_timing_read.argtypes = [vp, ctypes.c_int32, P(ctypes.c_double), P(ctypes.c_int64)]
     _lib = L

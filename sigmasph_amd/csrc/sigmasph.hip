@@ -1233,6 +1233,32 @@ struct TimerKind {
     int used = 0;
 };
 
+// what fixes a captured step: the state and solid structs (buffer pointers and live
+// indices), the parameter blocks and the stream
+struct GraphKey {
+    std::vector<char> b;
+    void set(const sph_state* st, const sph_solid* sd, const sph_fluid_params* P, const sph_solid_params* S,
+             hipStream_t hs) {
+        b.clear();
+        auto add = [&](const void* p, size_t n) {
+            const char* q = (const char*)p;
+            b.insert(b.end(), q, q + n);
+        };
+        add(st, sizeof *st);
+        if (sd) add(sd, sizeof *sd);
+        add(P, sizeof *P);
+        if (S) add(S, sizeof *S);
+        add(&hs, sizeof hs);
+    }
+    bool operator==(const GraphKey& o) const { return b == o.b; }
+};
+
+struct GraphEntry {
+    GraphKey key;
+    hipGraphExec_t exec = nullptr;
+    int cur_after = 0, hcur_after = 0;
+};
+
 struct sph_ctx {
     sph_dims dims;
     DGrid grid;
@@ -1256,6 +1282,30 @@ struct sph_ctx {
     void* hg_pair = nullptr;       // [ncomp][nmax][npad] of R: w_j W_ij e e^T
     double4* hg_src = nullptr;     // [npad] sum_j w_j W_ij (e . <G> r) e
     bool hg_pairs = true;          // later stages read them instead of recomputing
+    // captured steps (sph_graphs): replayed when the same arguments come back
+    bool graphs = false;
+    GraphEntry graph[4];
+    int graph_next = 0;
+    GraphEntry* find_graph(const GraphKey& k) {
+        for (auto& e : graph)
+            if (e.exec && e.key == k) return &e;
+        return nullptr;
+    }
+    void add_graph(const GraphKey& k, hipGraphExec_t ex, int cur_after, int hcur_after) {
+        GraphEntry& e = graph[graph_next];
+        graph_next = (graph_next + 1) % 4;
+        if (e.exec) (void)hipGraphExecDestroy(e.exec);
+        e.key = k;
+        e.exec = ex;
+        e.cur_after = cur_after;
+        e.hcur_after = hcur_after;
+    }
+    void drop_graphs() {
+        for (auto& e : graph) {
+            if (e.exec) (void)hipGraphExecDestroy(e.exec);
+            e.exec = nullptr;
+        }
+    }
 };
 
 static int set_err(sph_ctx* c, int code, const char* fmt, const char* a = "", long long b = 0) {
@@ -1435,6 +1485,7 @@ extern "C" int sph_ctx_destroy(sph_ctx* c) {
     if (c->cons_part) (void)hipFree(c->cons_part);
     if (c->nl.idx) (void)hipFree(c->nl.idx);
     if (c->nl.cnt) (void)hipFree(c->nl.cnt);
+    c->drop_graphs();
     if (c->hg_pair) (void)hipFree(c->hg_pair);
     if (c->hg_src) (void)hipFree(c->hg_src);
     for (auto& t : c->tk) for (auto ev : t.ev) (void)hipEventDestroy(ev);
@@ -1452,6 +1503,13 @@ extern "C" int sph_timing(sph_ctx* c, int32_t enable) {
     }
     for (auto& t : c->tk) t.used = 0;
     c->timing = enable != 0;
+    return SPH_OK;
+}
+
+extern "C" int sph_graphs(sph_ctx* c, int32_t enable) {
+    if (!c) return SPH_EINVAL;
+    c->graphs = enable != 0;
+    if (!c->graphs) c->drop_graphs();
     return SPH_OK;
 }
 

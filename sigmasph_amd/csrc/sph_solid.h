@@ -1723,6 +1723,18 @@ static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F,
     return check_launch(c, "solid finish");
 }
 
+// the launches of one solid step on stream hs (recorded when the stream is capturing)
+static int solid_step_launch(sph_ctx* c, sph_state* st, sph_solid* sd, const sph_fluid_params* P,
+                             const sph_solid_params* S, hipStream_t hs) {
+    int rc;
+    DFluid F = make_fluid(c, P);
+    DSolidP D = make_solid_params(c, P, S);
+    hipLaunchKernelGGL(k_solid_begin, dim3(1), dim3(64), 0, hs, st->scal, D, sd->cons, S->ncons);
+    if ((rc = sph_solid_sort(c, st, sd, (void*)hs))) return rc;
+    if (c->dims.precision == SPH_PREC_MIXED) return solid_step<float>(c, st, sd, F, D, hs);
+    return solid_step<double>(c, st, sd, F, D, hs);
+}
+
 extern "C" int sph_solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const sph_fluid_params* P,
                               const sph_solid_params* S, void* stream) {
     if (!c || !st || !sd || !P || !S) return SPH_EINVAL;
@@ -1761,12 +1773,44 @@ extern "C" int sph_solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const sp
         HIPCHECK(c, hipMalloc(&c->hg_pair, rsz * ncomp * (size_t)c->nl.nmax * c->nl.npad));
         HIPCHECK(c, hipMalloc((void**)&c->hg_src, sizeof(double4) * (size_t)c->nl.npad));
     }
-    DFluid F = make_fluid(c, P);
-    DSolidP D = make_solid_params(c, P, S);
-    hipLaunchKernelGGL(k_solid_begin, dim3(1), dim3(64), 0, hs, st->scal, D, sd->cons, S->ncons);
-    if ((rc = sph_solid_sort(c, st, sd, stream))) return rc;
-    if (c->dims.precision == SPH_PREC_MIXED) return solid_step<float>(c, st, sd, F, D, hs);
-    return solid_step<double>(c, st, sd, F, D, hs);
+    // replay: a step is fixed by its arguments and the host-side buffer indices, which
+    // alternate with period two, so two captured graphs serve a steady run
+    const bool graph = c->graphs && !c->timing && hs != nullptr;
+    GraphKey key;
+    if (graph) {
+        key.set(st, sd, P, S, hs);
+        if (GraphEntry* e = c->find_graph(key)) {
+            HIPCHECK(c, hipGraphLaunch(e->exec, hs));
+            st->cur = e->cur_after;
+            sd->hcur = e->hcur_after;
+            return check_launch(c, "solid step graph");
+        }
+        HIPCHECK(c, hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal));
+    }
+    const int cur0 = st->cur, hcur0 = sd->hcur;
+    rc = solid_step_launch(c, st, sd, P, S, hs);
+    if (!graph) return rc;
+    hipGraph_t g = nullptr;
+    hipError_t ec = hipStreamEndCapture(hs, &g);
+    if (rc || ec != hipSuccess || !g) {
+        // nothing ran: the launches were only recorded
+        st->cur = cur0;
+        sd->hcur = hcur0;
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return rc ? rc : set_err(c, SPH_EHIP, "solid step capture failed: %s", hipGetErrorString(ec), 0);
+    }
+    hipGraphExec_t ex = nullptr;
+    ec = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ec != hipSuccess) {
+        st->cur = cur0;
+        sd->hcur = hcur0;
+        return set_err(c, SPH_EHIP, "solid step graph instantiation failed: %s", hipGetErrorString(ec), 0);
+    }
+    c->add_graph(key, ex, st->cur, sd->hcur);
+    HIPCHECK(c, hipGraphLaunch(ex, hs));
+    return check_launch(c, "solid step graph");
 }
 
 extern "C" int sph_solid_reduce(sph_ctx* c, const sph_state* st, const sph_solid* sd, int32_t which, double* out,

@@ -114,6 +114,9 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
     ap.add_argument("--workload", default="dogbone", choices=("dogbone", "tgv160", "dambreak"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graphs", default="auto", choices=("auto", "on", "off"),
+                    help="time the solid step replayed from captured HIP graphs (auto: on for "
+                         "the solid workload)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has "
@@ -137,14 +140,34 @@ def main():
         solver.step()
     torch.cuda.synchronize()
     lib = ps._lib
+    solid = ps.is_solid
+    use_graphs = args.graphs == "on" or (args.graphs == "auto" and solid)
+    el_graph = None
+    if use_graphs:
+        # the timed window of the value: the step replayed from its two captured graphs
+        # on a side stream (the first two steps capture them)
+        N.check(lib.sph_graphs(ps.ctx, 1), ps.ctx, "sph_graphs")
+        side = torch.cuda.Stream()
+        with torch.cuda.stream(side):
+            for _ in range(2):
+                solver.step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                solver.step()
+            torch.cuda.synchronize()
+            el_graph = time.perf_counter() - t0
+        N.check(lib.sph_graphs(ps.ctx, 0), ps.ctx, "sph_graphs")
+    # the kernel-timing window: the same number of steps taken eagerly with HIP events
+    # around every pair pass (events are not recorded inside a graph)
     N.check(lib.sph_timing(ps.ctx, 1), ps.ctx, "sph_timing")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         solver.step()
     torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    solid = ps.is_solid
+    el_eager = time.perf_counter() - t0
+    el = el_graph if el_graph is not None else el_eager
     kinds = ["pass_volume", "pass_force", "pass_shift", "sort", "solid_stress", "solid_force",
              "hourglass_stage"]
     avg = {}
@@ -198,6 +221,8 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_particle": bm[dom], "avg_ms": avg[dom][0]},
         "kernel_ms": {k: v[0] for k, v in avg.items()},
+        "step_graphs": bool(use_graphs),
+        "ms_per_step_eager": 1e3 * el_eager / args.steps,
         "kernel_ms_per_step": kernel_ms_step,
         "cpu_baseline": None,
     }

@@ -185,7 +185,9 @@ def main():
     if solid:
         stages = int(ps.scal[N.SPH_S_STS_STAGES].item())
         bm = solid_kernel_bytes(args.precision, two_d=ps.two_d, pst=bool(solver.settings.fs.pst_enabled),
-                                sts=bool(solver.hg_sts), stages=max(stages, 1))
+                                sts=bool(solver.hg_sts), stages=max(stages, 1),
+                                list_pairs=ps.neighbor_count() / n,
+                                hg_pairs=os.environ.get("SIGMASPH_HG_PAIRS", "1") != "0")
         pair_kinds = ("solid_stress", "solid_force", "hourglass_stage", "pass_shift")
     else:
         bm = fluid_kernel_bytes(args.precision, pst=bool(solver.pst_enabled),

@@ -40,11 +40,18 @@ def fluid_kernel_bytes(precision="fp64", pst=True, track_visc=False, cells_per_p
 
 
 def solid_kernel_bytes(precision="fp64", two_d=True, pst=True, sts=True, stages=1,
-                       cells_per_particle=1.0 / 36.0):
+                       cells_per_particle=1.0 / 36.0, list_pairs=0.0, hg_pairs=True):
     """The same counting rule for the solid step.  A single-layer deck stores the full
     3x3 grad v row but only its in-plane block and G22 are useful (5 values); the
     deviator is 6 values (xx, yy, xy, zz, xz, yz) and 4 are useful in a single layer.
-    `stages` is the hourglass sub-step count of the step (the stage pass runs that often)."""
+    `stages` is the hourglass sub-step count of the step (the stage pass runs that often).
+
+    The step's neighbour list is a per-pair array, not a neighbour re-read: the build
+    writes 4 B per entry (`list_pairs` entries per particle, mirror images included) and
+    every list pass reads them.  With the stored hourglass operator (`hg_pairs`) the
+    first stage of a step writes `ncomp` values per entry and later stages read them in
+    place of x_j and grad v_j; `hourglass_stage` is then the average over the step's
+    `stages` launches."""
     r = 8 if precision == "fp64" else 4
     v3 = 24
     s3 = 3 * r
@@ -77,5 +84,19 @@ def solid_kernel_bytes(precision="fp64", two_d=True, pst=True, sts=True, stages=
     out["finish"] = (4 * v3 + (s3 if pst else 0) + r + s3) + 3 * v3
     fields = 4 + 4 + 4 * v3 + s3 + 3 * r + (r if mixed else 0) + (sgn + 2) * r + 2 * r + 3 * 4 + s3
     out["sort"] = (v3 + 4) + 3 * 2 * cell + (4 + 4) + (4 + 4 + 4) + 4 + 2 * fields
+    lst = 4.0 * list_pairs + 4.0
+    out["build_list"] = v3 + 2 * cell + lst
+    out["solid_stress"] += lst
+    out["solid_force"] += lst
+    if sts:
+        out["grad_v"] += lst
+        full = out["hourglass_stage"] + lst
+        if hg_pairs:
+            A = (3 if two_d else 6) * r * list_pairs
+            first = full + A + 32
+            later = v3 + 32 + 2 * r + 4 + lst + A + v3
+            out["hourglass_stage"] = (first + (max(stages, 1) - 1) * later) / max(stages, 1)
+        else:
+            out["hourglass_stage"] = full
     out["step"] = sum(v for k, v in out.items() if k != "hourglass_stage") + stages * out["hourglass_stage"]
     return out

@@ -42,6 +42,9 @@ def make_scene(name):
         return scenes.tgv_block(n=160), "taylor-green block 160^3, periodic, dx=1mm"
     if name == "dambreak":
         return scenes.dam_break_3d(), "3-D dam break 0.4x0.4x0.2 m column, dx=0.01 m, reflect box"
+    if name == "ensemble":
+        return (scenes.tensile_ensemble(), "256 plane-strain tensile specimens in one deck, W~U[2,8] "
+                "L~U[10,40] mm at dx=0.1 mm, per-specimen JC rows, 512 grips, hourglass STS")
     raise SystemExit("unknown workload %r" % name)
 
 
@@ -56,9 +59,13 @@ def cpu_baseline(workload, precision, seconds=15.0):
     from sigmasph_amd import scenes
     threads = min(os.cpu_count() or 1, 16)
     O.set_threads(threads)
-    if workload == "dogbone":
+    if workload in ("dogbone", "ensemble"):
         from sigmasph_amd.SOLID import SolidSettings
-        sc, sample = scenes.dogbone_c2(), "the full dogbone (26 880 particles), same numerics"
+        if workload == "dogbone":
+            sc, sample = scenes.dogbone_c2(), "the full dogbone (26 880 particles), same numerics"
+        else:
+            sc = scenes.ensemble_member(scenes.tensile_ensemble(), 0)
+            sample = "specimen 0 of the ensemble, same numerics"
         lay = SceneLayout(SimConfig(sc))
         st = SolidSettings(lay)
         P, S = st.params(), st.solid_params()
@@ -93,6 +100,43 @@ def cpu_baseline(workload, precision, seconds=15.0):
             "sample": "%s, %d steps in %.1f s" % (sample, steps, el)}
 
 
+def looped_baseline(deck, args, n_total):
+    """The ensemble stepped one specimen system at a time (each its own context, its
+    launches after the previous system's) over a sample of specimens: particle-steps/s of
+    that loop, against which the batched deck's rate is compared."""
+    import torch
+    from sigmasph_amd import scenes
+    from sigmasph_amd.config import SimConfig
+    from sigmasph_amd.particle_system import ParticleSystem
+    count = len(deck["SolidBlocks"])
+    ks = list(range(0, count, max(1, count // args.loop_sample)))[:args.loop_sample]
+    systems = []
+    for k in ks:
+        ps = ParticleSystem(SimConfig(scenes.ensemble_member(deck, k)), precision=args.precision)
+        sv = ps.build_solver()
+        sv.initialize()
+        systems.append((ps, sv))
+    for _ in range(args.warmup):
+        for ps, sv in systems:
+            sv.step()
+    torch.cuda.synchronize()
+    steps = max(5, args.steps // 3)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for ps, sv in systems:
+            sv.step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    m = sum(ps.particle_max_num for ps, _ in systems)
+    for ps, _ in systems:
+        ps.close()
+    rate = m * steps / el
+    return {"value": rate, "unit": "particle-steps/s", "specimens": len(ks), "particles": m,
+            "ms_per_loop": 1e3 * el / steps,
+            "note": "each specimen its own context, stepped in turn; the batched deck holds %d particles"
+                    % n_total}
+
+
 def measured_traffic(workload, precision, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter runs
     (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration), or None
@@ -112,7 +156,11 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
-    ap.add_argument("--workload", default="dogbone", choices=("dogbone", "tgv160", "dambreak"))
+    ap.add_argument("--workload", default="dogbone", choices=("dogbone", "tgv160", "dambreak", "ensemble"))
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the lanes per pair pass and the hourglass operator before timing")
+    ap.add_argument("--loop-sample", type=int, default=16,
+                    help="ensemble: specimens stepped one system at a time for the looped baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graphs", default="auto", choices=("auto", "on", "off"),
                     help="time the solid step replayed from captured HIP graphs (auto: on for "
@@ -141,6 +189,11 @@ def main():
     torch.cuda.synchronize()
     lib = ps._lib
     solid = ps.is_solid
+    tuned = None
+    if args.autotune:
+        from sigmasph_amd.autotune import autotune
+        tuned = autotune(ps, solver, steps=4)
+        log("autotune: %s" % tuned)
     use_graphs = args.graphs == "on" or (args.graphs == "auto" and solid)
     el_graph = None
     if use_graphs:
@@ -223,12 +276,15 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_particle": bm[dom], "avg_ms": avg[dom][0]},
         "kernel_ms": {k: v[0] for k, v in avg.items()},
+        "autotune": tuned,
         "step_graphs": bool(use_graphs),
         "ms_per_step_eager": 1e3 * el_eager / args.steps,
         "kernel_ms_per_step": kernel_ms_step,
         "cpu_baseline": None,
     }
     ps.close()
+    if args.workload == "ensemble":
+        out["looped"] = looped_baseline(scene, args, n)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.workload, args.precision)

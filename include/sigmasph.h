@@ -217,7 +217,9 @@ typedef struct {
     int32_t thermal;         /* some material has a specific heat (SOLID.py:494)            */
     int32_t plastic;         /* some material can yield (SOLID.py:407)                      */
     int32_t nmat;            /* rows of the material table                                  */
-    int32_t ncons;           /* displacement constraints (grips)                            */
+    int32_t ncons;           /* displacement constraints (grips), at most 4096; up to 16 the
+                              * grip forces are fixed-order sums (bitwise reproducible),
+                              * beyond that atomic adds                                     */
     int32_t pad_;
 } sph_solid_params;
 
@@ -344,6 +346,18 @@ int  sph_timing(sph_ctx* ctx, int32_t enable);
  * the eager step's, kernel for kernel.  Timing (sph_timing) takes the eager path.
  * Disabling frees the captured graphs. */
 int  sph_graphs(sph_ctx* ctx, int32_t enable);
+
+/* Launch-shape knobs (no counterpart in the reference), for the autotuner
+ * (sigmasph_amd/autotune.py): SPH_TUNE_LANES + k sets the lanes per particle (1, 2, 4, 8)
+ * of pair pass k (0 volume, 1 force, 2 shift, 3 pair count: cell-tiled, at most the lanes
+ * their tiles were sized for; 4 stress, 5 stress divergence, 6 grad v, 7 hourglass
+ * stage: neighbour-list passes, any); SPH_TUNE_HG_PAIRS (0/1) switches the stored
+ * hourglass operator.  Results do not depend on the knobs beyond summation order.
+ * Setting a knob frees captured graphs. */
+#define SPH_TUNE_LANES     0
+#define SPH_TUNE_HG_PAIRS  8
+int  sph_tune(sph_ctx* ctx, int32_t knob, int32_t value);
+int  sph_tune_get(sph_ctx* ctx, int32_t knob, int32_t* value);
 int  sph_timing_read(sph_ctx* ctx, int32_t which, double* avg_ms, int64_t* launches);
 
 #ifdef __cplusplus

@@ -59,6 +59,8 @@ SPH_R_ELASTIC = 8
 SPH_R_COMPRESSION = 9
 SPH_R_COMPRESSION_CLAMPED = 10
 SPH_R_NEG_INTERNAL = 11
+SPH_TUNE_LANES = 0
+SPH_TUNE_HG_PAIRS = 8
 # material kinds
 EOS_LINEAR, EOS_TAIT, EOS_MG = 0, 1, 3
 PLASTIC_NONE, PLASTIC_LINEAR, PLASTIC_JC = 0, 1, 2
@@ -183,7 +185,7 @@ EXPORTS = (
     "sph_fluid_step", "sph_fluid_pass_volume", "sph_fluid_pass_force",
     "sph_fluid_advect", "sph_fluid_pass_shift", "sph_fluid_finish",
     "sph_solid_sort", "sph_solid_step", "sph_solid_reduce",
-    "sph_reduce", "sph_timing", "sph_timing_read", "sph_graphs",
+    "sph_reduce", "sph_timing", "sph_timing_read", "sph_graphs", "sph_tune", "sph_tune_get",
 )
 
 _lib = None
@@ -229,6 +231,10 @@ def lib():
     L.sph_reduce.argtypes = [vp, P(SphState), ctypes.c_int32, vp, P(SphFluidParams), vp, vp]
     L.sph_timing.restype = ctypes.c_int
     L.sph_timing.argtypes = [vp, ctypes.c_int32]
+    L.sph_tune.restype = ctypes.c_int
+    L.sph_tune.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
+    L.sph_tune_get.restype = ctypes.c_int
+    L.sph_tune_get.argtypes = [vp, ctypes.c_int32, P(ctypes.c_int32)]
     L.sph_graphs.restype = ctypes.c_int
     L.sph_graphs.argtypes = [vp, ctypes.c_int32]
     L.sph_timing_read.restype = ctypes.c_int

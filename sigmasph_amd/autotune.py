@@ -1,0 +1,88 @@
+"""Launch-shape autotuner: picks the lanes per particle of every pair pass and the
+stored hourglass operator for a given particle system by timing them on the GPU.
+
+The knobs (`sph_tune`) change only how the work is spread over lanes, never what is
+computed beyond the order of a pair sum, so tuning is safe mid-run: it takes ordinary
+steps of the simulation.  Each round sets one value for every tunable pass at once and
+reads the per-pass HIP-event averages (`sph_timing`), so a pass is tuned from the rounds'
+own timings -- one round per candidate value, not one per pass and value; the stored
+hourglass operator is then tried on and off with the chosen lanes, judged on the
+hourglass stages' time per step.  Passes the step does not run keep their values.
+
+    from sigmasph_amd.autotune import autotune
+    choice = autotune(ps, solver, steps=4)      # {"lanes": {...}, "hg_pairs": bool, ...}
+"""
+import ctypes
+
+import torch
+
+from . import _native as N
+
+PASSES = ("volume", "force", "shift", "pairs", "stress", "stress_divergence", "grad_v", "hourglass_stage")
+# the timing kinds of sph_timing_read that measure each pass (grad v has none of its own)
+TIMER_OF = {"volume": 0, "force": 1, "shift": 2, "stress": 4, "stress_divergence": 5, "hourglass_stage": 6}
+TILED = ("volume", "force", "shift", "pairs")
+
+
+def _get(ps, knob):
+    v = ctypes.c_int32()
+    N.check(ps._lib.sph_tune_get(ps.ctx, knob, ctypes.byref(v)), ps.ctx, "sph_tune_get")
+    return v.value
+
+
+def _set(ps, knob, value):
+    return ps._lib.sph_tune(ps.ctx, knob, int(value)) == N.SPH_OK
+
+
+def _timed(ps, solver, steps):
+    """Per-pass average launch time (ms) and launches over `steps` steps."""
+    lib = ps._lib
+    torch.cuda.synchronize()
+    N.check(lib.sph_timing(ps.ctx, 1), ps.ctx, "sph_timing")
+    for _ in range(steps):
+        solver.step()
+    torch.cuda.synchronize()
+    out = {}
+    for name, k in TIMER_OF.items():
+        ms, cnt = ctypes.c_double(), ctypes.c_int64()
+        N.check(lib.sph_timing_read(ps.ctx, k, ctypes.byref(ms), ctypes.byref(cnt)), ps.ctx, "timing")
+        if cnt.value:
+            out[name] = (ms.value, cnt.value)
+    lib.sph_timing(ps.ctx, 0)
+    return out
+
+
+def autotune(ps, solver, steps=4, candidates=(1, 2, 4, 8), verbose=False):
+    """Tune `ps`'s context in place; returns the choice and the timings it rests on."""
+    knobs = {name: N.SPH_TUNE_LANES + k for k, name in enumerate(PASSES)}
+    start = {name: _get(ps, knobs[name]) for name in PASSES}
+    solver.step()                                   # settle: lists, scratch, first captures
+    table = {}
+    for lpp in candidates:
+        ok = {name: _set(ps, knobs[name], lpp) for name in TIMER_OF}
+        t = _timed(ps, solver, steps)
+        for name, (ms, cnt) in t.items():
+            if ok.get(name):
+                table.setdefault(name, {})[lpp] = ms * cnt / steps
+        if verbose:
+            print("lanes %d: %s" % (lpp, {k: round(v[0] * 1e3, 1) for k, v in t.items()}))
+    lanes = {}
+    for name in PASSES:
+        if name in table and table[name]:
+            lanes[name] = min(table[name], key=table[name].get)
+        else:
+            lanes[name] = start[name]
+        _set(ps, knobs[name], lanes[name])
+    hg = None
+    if "hourglass_stage" in table:
+        per = {}
+        for flag in (1, 0):
+            _set(ps, N.SPH_TUNE_HG_PAIRS, flag)
+            t = _timed(ps, solver, steps)
+            if "hourglass_stage" in t:
+                ms, cnt = t["hourglass_stage"]
+                per[flag] = ms * cnt / steps
+        hg = bool(min(per, key=per.get)) if per else True
+        _set(ps, N.SPH_TUNE_HG_PAIRS, int(hg))
+        table["hg_pairs"] = per
+    return {"lanes": lanes, "hg_pairs": hg, "ms_per_step": table}

@@ -1506,6 +1506,35 @@ extern "C" int sph_timing(sph_ctx* c, int32_t enable) {
     return SPH_OK;
 }
 
+extern "C" int sph_tune(sph_ctx* c, int32_t knob, int32_t value) {
+    if (!c) return SPH_EINVAL;
+    if (knob >= SPH_TUNE_LANES && knob < SPH_TUNE_LANES + PASS_KINDS) {
+        const int kind = knob - SPH_TUNE_LANES;
+        if (value != 1 && value != 2 && value != 4 && value != 8)
+            return set_err(c, SPH_EINVAL, "lanes per particle must be 1, 2, 4 or 8%s", "", 0);
+        // a cell-tiled pass fits one tile's particles in a workgroup at the lanes its tiles
+        // were sized for; the list passes take any count
+        if (kind < PASS_STRESS && value > c->tiles.lpp)
+            return set_err(c, SPH_EINVAL, "a tiled pass takes at most %s%lld lanes per particle", "",
+                           c->tiles.lpp);
+        c->lpp[kind] = value;
+    } else if (knob == SPH_TUNE_HG_PAIRS) {
+        c->hg_pairs = value != 0;
+    } else {
+        return set_err(c, SPH_EINVAL, "unknown tuning knob%s %lld", "", knob);
+    }
+    c->drop_graphs();   // captured launches carry the old shapes
+    return SPH_OK;
+}
+
+extern "C" int sph_tune_get(sph_ctx* c, int32_t knob, int32_t* value) {
+    if (!c || !value) return SPH_EINVAL;
+    if (knob >= SPH_TUNE_LANES && knob < SPH_TUNE_LANES + PASS_KINDS) *value = c->lpp[knob - SPH_TUNE_LANES];
+    else if (knob == SPH_TUNE_HG_PAIRS) *value = c->hg_pairs ? 1 : 0;
+    else return set_err(c, SPH_EINVAL, "unknown tuning knob%s %lld", "", knob);
+    return SPH_OK;
+}
+
 extern "C" int sph_graphs(sph_ctx* c, int32_t enable) {
     if (!c) return SPH_EINVAL;
     c->graphs = enable != 0;

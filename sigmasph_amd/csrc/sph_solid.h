@@ -25,7 +25,8 @@
 #define SPH_S_PEND_VISC 23
 #define SPH_S_PEND_HG   24
 #define SPH_S_KE_POST   25
-#define SPH_MAX_CONS    16
+#define SPH_MAX_CONS    4096  // grips a step takes
+#define SPH_CONS_BLOCK  16    // up to this many, grip forces are fixed-order block sums
 #define SPH_LIST_LANES  4     // lanes per particle of the neighbour-list build
 
 // scene constants of the solid step, folded on the host
@@ -1049,6 +1050,7 @@ template <typename R> struct HgBody {
 // update_load_factor (SOLID.py:5001-5017) from the clock; banks what the previous step
 // left for this one (SOLID.py:2146-2236 read them one step late); clears the grips.
 __global__ void k_solid_begin(double* scal, DSolidP S, double* cons, int ncons) {
+    for (int k = threadIdx.x; k < 6 * ncons; k += blockDim.x) cons[k] = 0.0;
     if (threadIdx.x != 0) return;
     double dt = scal[SPH_S_DT];
     scal[SPH_S_VISC_WORK] += dt * scal[SPH_S_PEND_VISC];
@@ -1064,7 +1066,6 @@ __global__ void k_solid_begin(double* scal, DSolidP S, double* cons, int ncons) 
         scal[SPH_S_LOAD_S] = u * u * (3.0 - 2.0 * u);
         scal[SPH_S_LOAD_SDOT] = 6.0 * u * (1.0 - u) / S.ramp_time;
     }
-    for (int k = 0; k < 6 * ncons; ++k) cons[k] = 0.0;
 }
 
 // compute_adaptive_dt (DSPH.py:1365-1430) with the solid's caps: clamp_dt_hourglass
@@ -1260,12 +1261,21 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_constraints(DFluid F, DState<R> s
             if (finite3(x.x, x.y, x.z)) vmax = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
         }
     }
-    for (int c = 0; c < ncons; ++c) {
-        const bool mine = cid == c;
-        double a[3] = {mine ? fx : 0.0, mine ? fy : 0.0, mine ? fz : 0.0};
-        block_sum_n<3>(a, red);
-        if (threadIdx.x == 0)
-            for (int q = 0; q < 3; ++q) cons_part[(3 * c + q) * nblk + blockIdx.x] = a[q];
+    if (ncons > SPH_CONS_BLOCK) {
+        // many grips (an ensemble): each grip particle adds its reaction in place
+        if (cid >= 0) {
+            if (fx != 0.0) atomicAdd(&sd.cons[6 * cid + 0], fx);
+            if (fy != 0.0) atomicAdd(&sd.cons[6 * cid + 1], fy);
+            if (fz != 0.0) atomicAdd(&sd.cons[6 * cid + 2], fz);
+        }
+    } else {
+        for (int c = 0; c < ncons; ++c) {
+            const bool mine = cid == c;
+            double a[3] = {mine ? fx : 0.0, mine ? fy : 0.0, mine ? fz : 0.0};
+            block_sum_n<3>(a, red);
+            if (threadIdx.x == 0)
+                for (int q = 0; q < 3; ++q) cons_part[(3 * c + q) * nblk + blockIdx.x] = a[q];
+        }
     }
     if (F.ma_mode == 1) {
         double bm = block_max(vmax, red);
@@ -1285,7 +1295,7 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_bank_solid(double* scal, DScratch
     for (int k = threadIdx.x; k < nblk; k += blockDim.x)
         for (int q = 0; q < use; ++q) t[q] += s.part[slots[q] * pstride + k];
     block_sum_n<5>(t, red);
-    for (int c = 0; c < ncons; ++c) {
+    for (int c = 0; c < (ncons > SPH_CONS_BLOCK ? 0 : ncons); ++c) {
         double v[3] = {0.0, 0.0, 0.0};
         for (int k = threadIdx.x; k < nblk; k += blockDim.x)
             for (int q = 0; q < 3; ++q) v[q] += cons_part[(3 * c + q) * nblk + k];
@@ -1729,7 +1739,7 @@ static int solid_step_launch(sph_ctx* c, sph_state* st, sph_solid* sd, const sph
     int rc;
     DFluid F = make_fluid(c, P);
     DSolidP D = make_solid_params(c, P, S);
-    hipLaunchKernelGGL(k_solid_begin, dim3(1), dim3(64), 0, hs, st->scal, D, sd->cons, S->ncons);
+    hipLaunchKernelGGL(k_solid_begin, dim3(1), dim3(256), 0, hs, st->scal, D, sd->cons, S->ncons);
     if ((rc = sph_solid_sort(c, st, sd, (void*)hs))) return rc;
     if (c->dims.precision == SPH_PREC_MIXED) return solid_step<float>(c, st, sd, F, D, hs);
     return solid_step<double>(c, st, sd, F, D, hs);
@@ -1751,8 +1761,8 @@ extern "C" int sph_solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const sp
         return set_err(c, SPH_EINVAL, "hourglass sub-step limits out of range%s", "", 0);
     hipStream_t hs = (hipStream_t)stream;
     if (!c->cons_part) {
-        HIPCHECK(c, hipMalloc((void**)&c->cons_part, sizeof(double) * 3 * SPH_MAX_CONS * (size_t)c->nblk));
-        c->cons_cap = SPH_MAX_CONS;
+        HIPCHECK(c, hipMalloc((void**)&c->cons_part, sizeof(double) * 3 * SPH_CONS_BLOCK * (size_t)c->nblk));
+        c->cons_cap = SPH_CONS_BLOCK;
     }
     if (!c->nl.idx) {
         // room for twice the lattice's neighbour count (plus the images near a plane)

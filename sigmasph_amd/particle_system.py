@@ -459,7 +459,7 @@ class ParticleSystem(SceneLayout):
         self._sderived = {"grad_v": z(n, 10), "hg_y0": z(n, 4, dt=torch.float64),
                           "hg_y1": z(n, 4, dt=torch.float64), "v_sync": z(n, 4, dt=torch.float64),
                           "q_visc": z(n)}
-        self.cons = z(16, 6, dt=torch.float64)
+        self.cons = z(max(16, len(getattr(self, "constraints", []) or [])), 6, dt=torch.float64)
         h0, b0 = self._hist[0], self._sbufs[0]
         h0["e_int"].copy_(torch.from_numpy(self.e_init))
         b0["T"].copy_(torch.from_numpy(self.T_init))

@@ -252,3 +252,82 @@ def tensile_axi_dogbone_nofail(dx0=0.1, duration=0.5, steps_per_cycle=50, output
              "displacement": [10.0, 0.0, 0.0]}],
     }
     return sc
+
+
+def tensile_ensemble(count=256, seed=0, dx=0.1, width=(2.0, 8.0), length=(10.0, 40.0),
+                     A=(0.2, 0.6), B=(0.3, 1.2), n=(0.1, 0.6), C=(0.01, 0.05), strain=0.1,
+                     rate=1.0, grip=1.0, gap=1.0, cfl=0.3):
+    """`count` plane-strain tensile specimens in one deck, the batched ensemble: each a
+    W x L rectangle (W ~ U`width`, L ~ U`length` mm, lattice dx) with its own
+    Johnson-Cook row (A, B, n, C ~ U[...] GPa / - / - / -, eps0_dot = 1/ms), drawn per
+    specimen in the order W, L, A, B, n, C from numpy's default_rng(seed); K = 160,
+    G = 77 GPa, rho 7.9e-6 for all.  The outer `grip` mm of each end is a grip; the two
+    grips move apart by `strain` x L in total on one smoothstep ramp sized so that a
+    specimen of mean length elongates at `rate` mm/ms on average.  Specimens are packed
+    in rows `gap` mm apart (more than the 0.3 mm support), so none interacts with
+    another.  Numerics as the C2 dogbone (support 3 dx, CFL 0.3, viscosity 0.02,
+    delta 0.1, Colle shifting, rate hourglass alpha 4 super-time-stepped)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    K, G = 160.0, 77.0
+    E = 9.0 * K * G / (3.0 * K + G)
+    nu = (3.0 * K - 2.0 * G) / (2.0 * (3.0 * K + G))
+    specs = []
+    for k in range(count):
+        w = rng.uniform(*width)
+        l = rng.uniform(*length)
+        a, b, nn, c = rng.uniform(*A), rng.uniform(*B), rng.uniform(*n), rng.uniform(*C)
+        specs.append((w, l, a, b, nn, c))
+    area = sum((s[0] + gap) * (s[1] + gap) for s in specs)
+    row_len = max(max(s[1] for s in specs) + gap, math.sqrt(area))
+    mean_len = sum(s[1] for s in specs) / count
+    ramp = strain * mean_len / rate
+    mats, blocks, cons = [], [], []
+    x, y, row_h = 0.0, 0.0, 0.0
+    for k, (w, l, a, b, nn, c) in enumerate(specs):
+        if x > 0.0 and x + l > row_len:
+            x, y, row_h = 0.0, y + row_h + gap, 0.0
+        name = "spec%03d" % k
+        mats.append({"name": name, "library": "stainless_steel",
+                     "strength": {"youngsModulus": E, "poissonRatio": nu,
+                                  "plasticity": {"A": a, "B": b, "n": nn, "C": c, "eps0_dot": 1.0}}})
+        lo = [x + 0.5 * dx, y + 0.5 * dx, 0.0]
+        blocks.append({"objectId": k, "start": lo, "end": [x + l, y + w, 0.25 * dx], "density": 7.9e-06,
+                       "material": name})
+        d = 0.5 * strain * l
+        cons.append({"name": "%s_left" % name, "region": [[x - 0.4 * gap, y - 0.4 * gap, -1.0],
+                                                           [x + grip, y + w + 0.4 * gap, 1.0]],
+                     "components": "xy", "displacement": [-d, 0.0, 0.0]})
+        cons.append({"name": "%s_right" % name, "region": [[x + l - grip, y - 0.4 * gap, -1.0],
+                                                            [x + l + 0.4 * gap, y + w + 0.4 * gap, 1.0]],
+                     "components": "xy", "displacement": [d, 0.0, 0.0]})
+        x += l + gap
+        row_h = max(row_h, w)
+    return {
+        "Domain": {"start": [-2.0, -2.0, 0.0], "end": [row_len + 2.0, y + row_h + 2.0, 1.0],
+                   "particleRadius": 0.5 * dx, "supportRadiusFactor": 6.0, "planeStrain": True},
+        "Time": {"CFL": cfl},
+        "Solver": {"type": "hypoElastic", "viscosity": 0.02, "allowNegativePressure": True},
+        "Materials": mats,
+        "Load": {"gravitation": [0.0, 0.0, 0.0], "rampTime": ramp},
+        "Diffusion": {"delta": 0.1},
+        "Shifting": {"enabled": True, "mode": "colle", "maMode": "fixed", "maFixed": 0.05,
+                     "R": 0.2, "n": 4, "maxShift": 0.05, "aleMomentum": True, "aleHistory": True},
+        "Hourglass": {"alpha": 4.0, "superTimeStepping": {"mode": "subcycle"}},
+        "SolidBlocks": blocks,
+        "Constraints": cons,
+    }
+
+
+def ensemble_member(deck, k):
+    """Specimen k of a `tensile_ensemble` deck as a deck of its own: the same numerics,
+    load ramp and domain, one block, its material row and its two grips."""
+    import copy
+    one = copy.deepcopy({key: v for key, v in deck.items()
+                         if key not in ("SolidBlocks", "Materials", "Constraints")})
+    blk = copy.deepcopy(deck["SolidBlocks"][k])
+    blk["objectId"] = 0
+    one["SolidBlocks"] = [blk]
+    one["Materials"] = [copy.deepcopy(m) for m in deck["Materials"] if m["name"] == blk["material"]]
+    one["Constraints"] = copy.deepcopy(deck["Constraints"][2 * k: 2 * k + 2])
+    return one

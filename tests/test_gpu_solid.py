@@ -19,10 +19,17 @@ from sigmasph_amd import _native as N
 from sigmasph_amd.config import SimConfig
 from sigmasph_amd.particle_system import ParticleSystem
 from sigmasph_amd.SOLID import SolidSettings
-from sigmasph_amd.scenes import solid_block
+from sigmasph_amd.scenes import solid_block, tensile_ensemble
 from helpers import solid_oracle_state
 
 pytestmark = pytest.mark.gpu
+
+def _stretched(deck):
+    """Each block starts with an extension rate past the yield strain rate."""
+    for b in deck["SolidBlocks"]:
+        b["divVelocity"] = [300.0, -150.0, 0.0]
+    return deck
+
 
 SCENES = {
     "plane_strain_grips": lambda: solid_block(n=(24, 8, 1), grips=0.05, ramp_time=1e-4,
@@ -34,6 +41,10 @@ SCENES = {
                                             rot_velocity=400.0, jitter=0.05),
     "plane_strain_noshift": lambda: solid_block(n=(20, 10, 1), pst=False, viscosity=0.0,
                                                 div_velocity=(-200.0, 250.0, 0.0), jitter=0.03),
+    # nine specimens with their own Johnson-Cook rows and 18 grips (past 16 the grip
+    # forces are summed by atomic adds)
+    "ensemble9": lambda: _stretched(tensile_ensemble(count=9, width=(0.8, 1.2), length=(3.0, 4.0),
+                                                     grip=0.5, rate=30.0)),
     # Mie-Grueneisen aluminium under compression with the VNR q and Monaghan's Pi
     "axisymmetric_shock": lambda: solid_block(
         n=(24, 8, 1), two_d="axisymmetric", material={"name": "projectile", "library": "al2017_t4"},

@@ -358,6 +358,35 @@ int  sph_graphs(sph_ctx* ctx, int32_t enable);
 #define SPH_TUNE_HG_PAIRS  8
 int  sph_tune(sph_ctx* ctx, int32_t knob, int32_t value);
 int  sph_tune_get(sph_ctx* ctx, int32_t knob, int32_t* value);
+
+/* ---- the constitutive surrogate's network (no counterpart in the reference) ---------
+ * An MLP 12 -> 64 -> 64 -> 7 with ReLU, evaluated and trained on the bf16 matrix cores
+ * with f32 sums.  Parameters are f32, packed: W1[64][16] (inputs padded to 16), b1[64],
+ * W2[64][64], b2[64], W3[16][64] (outputs padded to 16), b3[16] = SPH_MLP_NPARAM floats;
+ * `wb` holds bf16 copies of W1, W2, W3 (SPH_MLP_NBF halves), refreshed by sph_mlp_sync
+ * and by every training step.  Inputs are rows of 16 floats (12 used), outputs rows of
+ * 8 (7 used); norm = input mean [16], input 1/std [16], output mean [16], output std [16].
+ * The loss is the mean square error in the normalised output space. */
+#define SPH_MLP_NPARAM 6288
+#define SPH_MLP_NBF    6144
+typedef struct {
+    float* w;        /* device [SPH_MLP_NPARAM] parameters                                */
+    void*  wb;       /* device [SPH_MLP_NBF] bf16 matrices                                 */
+    float* grad;     /* device [SPH_MLP_NPARAM] gradient sum (training)                   */
+    float* m1;       /* device [SPH_MLP_NPARAM] Adam first moment (training)              */
+    float* m2;       /* device [SPH_MLP_NPARAM] Adam second moment (training)             */
+    float* norm;     /* device [64] normalisation                                         */
+} sph_mlp;
+int  sph_mlp_sync(const sph_mlp* net, void* stream);
+/* y[n][8] = net(x[n][16]) */
+int  sph_mlp_forward(const sph_mlp* net, const float* x, float* y, int64_t n, void* stream);
+/* grad += d(loss)/d(parameters) over the n samples, *loss (device) += their loss */
+int  sph_mlp_gradient(const sph_mlp* net, const float* x, const float* t, int64_t n, float* loss,
+                      void* stream);
+/* one Adam step (beta 0.9, 0.999, eps 1e-8; `step` >= 1 counts the steps) on the
+ * gradient of the n samples; clears the gradient */
+int  sph_mlp_train_step(const sph_mlp* net, const float* x, const float* t, int64_t n, float lr,
+                        int32_t step, float* loss, void* stream);
 int  sph_timing_read(sph_ctx* ctx, int32_t which, double* avg_ms, int64_t* launches);
 
 #ifdef __cplusplus

@@ -178,6 +178,15 @@ class SphSolid(ctypes.Structure):
     ]
 
 
+SPH_MLP_NPARAM = 6288
+SPH_MLP_NBF = 6144
+
+
+class SphMlp(ctypes.Structure):
+    _fields_ = [("w", ctypes.c_void_p), ("wb", ctypes.c_void_p), ("grad", ctypes.c_void_p),
+                ("m1", ctypes.c_void_p), ("m2", ctypes.c_void_p), ("norm", ctypes.c_void_p)]
+
+
 #: every symbol include/sigmasph.h declares
 EXPORTS = (
     "sph_ctx_create", "sph_ctx_destroy", "sph_last_error", "sph_version",
@@ -186,6 +195,7 @@ EXPORTS = (
     "sph_fluid_advect", "sph_fluid_pass_shift", "sph_fluid_finish",
     "sph_solid_sort", "sph_solid_step", "sph_solid_reduce",
     "sph_reduce", "sph_timing", "sph_timing_read", "sph_graphs", "sph_tune", "sph_tune_get",
+    "sph_mlp_sync", "sph_mlp_forward", "sph_mlp_gradient", "sph_mlp_train_step",
 )
 
 _lib = None
@@ -237,6 +247,15 @@ def lib():
     L.sph_tune_get.argtypes = [vp, ctypes.c_int32, P(ctypes.c_int32)]
     L.sph_graphs.restype = ctypes.c_int
     L.sph_graphs.argtypes = [vp, ctypes.c_int32]
+    L.sph_mlp_sync.restype = ctypes.c_int
+    L.sph_mlp_sync.argtypes = [P(SphMlp), vp]
+    L.sph_mlp_forward.restype = ctypes.c_int
+    L.sph_mlp_forward.argtypes = [P(SphMlp), vp, vp, ctypes.c_int64, vp]
+    L.sph_mlp_gradient.restype = ctypes.c_int
+    L.sph_mlp_gradient.argtypes = [P(SphMlp), vp, vp, ctypes.c_int64, vp, vp]
+    L.sph_mlp_train_step.restype = ctypes.c_int
+    L.sph_mlp_train_step.argtypes = [P(SphMlp), vp, vp, ctypes.c_int64, ctypes.c_float, ctypes.c_int32,
+                                     vp, vp]
     L.sph_timing_read.restype = ctypes.c_int
     L.sph_timing_read.argtypes = [vp, ctypes.c_int32, P(ctypes.c_double), P(ctypes.c_int64)]
     _lib = L

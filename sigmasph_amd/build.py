@@ -11,6 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc", "sigmasph.hip")
 DEPS = [os.path.join(HERE, "csrc", "sph_device.h"), os.path.join(HERE, "csrc", "sph_solid.h"),
+        os.path.join(HERE, "csrc", "sph_mlp.h"),
         os.path.join(ROOT, "include", "sigmasph.h")]
 LIB = os.path.join(HERE, "libsigmasph.so")
 

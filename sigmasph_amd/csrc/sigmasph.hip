@@ -1839,4 +1839,5 @@ extern "C" int sph_reduce(sph_ctx* c, const sph_state* st, int32_t which, const 
     return check_launch(c, "reduce");
 }
 
+#include "sph_mlp.h"
 #include "sph_solid.h"

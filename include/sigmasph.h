@@ -72,6 +72,8 @@ extern "C" {
 #define SPH_S_KE_PRE       19   /* solid: kinetic energy before the damper (scratch)    */
 #define SPH_S_IE_ALE_ADV   20   /* solid: the advection share of SPH_S_IE_ALE           */
 #define SPH_S_IE_ALE_DIFF  21   /* solid: the diffusion share of SPH_S_IE_ALE           */
+#define SPH_S_SUR_YIELDING 26   /* solid: particles the surrogate saw in the last step  */
+#define SPH_S_SUR_FALLBACK 27   /* solid: of those, answered by the plain return map    */
 #define SPH_NSCAL          32
 
 /* ---- reductions (sph_reduce `which`) ---------------------------------------------- */
@@ -221,7 +223,10 @@ typedef struct {
                               * grip forces are fixed-order sums (bitwise reproducible),
                               * beyond that atomic adds                                     */
     int32_t pad_;
+    double  sur_tol;         /* the surrogate's acceptance tolerance (fraction of sigma_y)   */
 } sph_solid_params;
+
+typedef struct sph_mlp sph_mlp;   /* the surrogate's network, defined below */
 
 /* The solid step's per-particle fields (device pointers, Real = double or float as
  * sph_dims.precision says).
@@ -231,7 +236,7 @@ typedef struct {
  *  - sorted fields T, de_int, eos_id, bc_*: two buffers that follow sph_state.cur.
  *  - grad_v, q_visc, hg_y, v_sync: derived within a step. */
 typedef struct {
-    void*    sig[2];         /* [n][6] deviatoric stress xx,yy,zz,xy,xz,yz (SOLID.py:421)   */
+    void*    sig[2];         /* [n][6] deviatoric stress xx,yy,xy,zz,xz,yz (SOLID.py:421)   */
     void*    eps_p[2];       /* [n] equivalent plastic strain                               */
     void*    e_int[2];       /* [n] specific internal energy (SOLID.py:488)                 */
     int32_t  hcur;
@@ -248,6 +253,27 @@ typedef struct {
     double*  v_sync;         /* [n][4] velocity after the first half kick                   */
     const sph_material* mat; /* device [nmat]                                               */
     double*  cons;           /* device [ncons][6]: grip force xyz, grip displacement xyz    */
+    /* The constitutive surrogate and its training data (no counterpart in the reference;
+     * all NULL: the plain return map).  With `surrogate` set, a Johnson-Cook particle
+     * whose trial deviator lies outside the yield surface writes its network inputs to
+     * sur_x, the network answers in sur_y, and the answer is kept where the yield-surface
+     * residual, the radial-return magnitude and the direction are all within
+     * sph_solid_params.sur_tol of the yield stress; the plain return map answers the
+     * others.  With `harvest_x` set, every plain return with plastic flow appends its
+     * inputs and answer (rows past harvest_cap are counted but not written).  Input row:
+     * trial deviator / its von Mises value (xx, yy, xy, zz, xz, yz), log(vm / A),
+     * eps_p, homologous temperature, log(1 + overshoot / (3 G dt eps0_dot)),
+     * rho / rho0 - 1, log(dt eps0_dot), then (not network inputs) the temperature as a
+     * float pair [12] + [15] and vm [14]; answer row: returned deviator / vm (6),
+     * d_eps_p 3G / vm, d_eps_p. */
+    const sph_mlp* surrogate;/* host pointer to the network, or NULL                         */
+    float*   sur_x;          /* [n][16] inputs of the particles the surrogate answers          */
+    float*   sur_y;          /* [n][8] its answers                                             */
+    int32_t* sur_flag;       /* [n] 1 where the surrogate is asked this step                   */
+    float*   harvest_x;      /* [harvest_cap][16] or NULL                                      */
+    float*   harvest_y;      /* [harvest_cap][8]                                               */
+    int64_t  harvest_cap;
+    int64_t* harvest_count;  /* device counter                                                 */
 } sph_solid;
 
 /* ---- context ---------------------------------------------------------------------- */
@@ -369,14 +395,14 @@ int  sph_tune_get(sph_ctx* ctx, int32_t knob, int32_t* value);
  * The loss is the mean square error in the normalised output space. */
 #define SPH_MLP_NPARAM 6288
 #define SPH_MLP_NBF    6144
-typedef struct {
+struct sph_mlp {
     float* w;        /* device [SPH_MLP_NPARAM] parameters                                */
     void*  wb;       /* device [SPH_MLP_NBF] bf16 matrices                                 */
     float* grad;     /* device [SPH_MLP_NPARAM] gradient sum (training)                   */
     float* m1;       /* device [SPH_MLP_NPARAM] Adam first moment (training)              */
     float* m2;       /* device [SPH_MLP_NPARAM] Adam second moment (training)             */
     float* norm;     /* device [64] normalisation                                         */
-} sph_mlp;
+};
 int  sph_mlp_sync(const sph_mlp* net, void* stream);
 /* y[n][8] = net(x[n][16]) */
 int  sph_mlp_forward(const sph_mlp* net, const float* x, float* y, int64_t n, void* stream);

@@ -157,6 +157,7 @@ class SolidSettings:
         S.l_eig_tol = self.l_eig_tol
         S.q_l2, S.q_cl = self.q_l2, self.q_bulk * self.q_length
         S.av_alpha, S.av_beta, S.av_eps = self.av_alpha, self.av_beta, self.av_eps
+        S.sur_tol = 0.01
         S.kernel_correction = int(self.kernel_correction)
         S.jaumann = int(self.jaumann)
         S.ale_history = int(self.pst_ale_history)
@@ -227,6 +228,56 @@ class HypoElasticSolver(SPHBase):
         N.check(ps._lib.sph_solid_step(ps.ctx, ctypes.byref(ps.state), ctypes.byref(ps.solid),
                                        ctypes.byref(self.params), ctypes.byref(self.sparams),
                                        ps.stream()), ps.ctx, "sph_solid_step")
+
+    # ---- the constitutive surrogate (no counterpart in the reference) -----------------
+    def harvest(self, cap):
+        """Record every plain Johnson-Cook return with plastic flow from now on: inputs
+        [cap][16] and answers [cap][8] on the device (see sph_solid in the header)."""
+        ps = self.ps
+        dev = ps.device
+        self._hx = torch.zeros((int(cap), 16), dtype=torch.float32, device=dev)
+        self._hy = torch.zeros((int(cap), 8), dtype=torch.float32, device=dev)
+        self._hn = torch.zeros(1, dtype=torch.int64, device=dev)
+        sd = ps.solid
+        sd.harvest_x, sd.harvest_y = self._hx.data_ptr(), self._hy.data_ptr()
+        sd.harvest_cap, sd.harvest_count = int(cap), self._hn.data_ptr()
+
+    def harvested(self):
+        """(inputs, answers) recorded so far, device tensors, and the rows seen in all
+        (rows past the capacity are counted, not kept)."""
+        seen = int(self._hn.item())
+        k = min(seen, self._hx.shape[0])
+        return self._hx[:k], self._hy[:k], seen
+
+    def stop_harvest(self):
+        sd = self.ps.solid
+        sd.harvest_x = sd.harvest_y = sd.harvest_count = None
+        sd.harvest_cap = 0
+
+    def use_surrogate(self, network, tol=0.01):
+        """Let `network` (surrogate.Network) answer the yielding Johnson-Cook particles,
+        kept where within `tol` of the yield stress, the plain return map elsewhere."""
+        ps = self.ps
+        n, dev = ps.particle_max_num, ps.device
+        self._sx = torch.zeros((n, 16), dtype=torch.float32, device=dev)
+        self._sy = torch.zeros((n, 8), dtype=torch.float32, device=dev)
+        self._sf = torch.zeros(n, dtype=torch.int32, device=dev)
+        self._net = network
+        sd = ps.solid
+        sd.surrogate = ctypes.cast(ctypes.pointer(network.s), ctypes.c_void_p).value
+        sd.sur_x, sd.sur_y, sd.sur_flag = self._sx.data_ptr(), self._sy.data_ptr(), self._sf.data_ptr()
+        self.sparams.sur_tol = float(tol)
+
+    def drop_surrogate(self):
+        sd = self.ps.solid
+        sd.surrogate = sd.sur_x = sd.sur_y = sd.sur_flag = None
+        self._net = None
+
+    def surrogate_stats(self):
+        """(particles the surrogate was asked, of them answered by the plain map) in the
+        last step."""
+        sc = self.ps.scal
+        return int(sc[N.SPH_S_SUR_YIELDING].item()), int(sc[N.SPH_S_SUR_FALLBACK].item())
 
     def get_constraint_data(self):
         """The grips' latest force and displacement (SOLID.py:5198-5225)."""

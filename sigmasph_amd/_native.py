@@ -45,6 +45,8 @@ SPH_S_STS_STAGES = 18
 SPH_S_KE_PRE = 19
 SPH_S_IE_ALE_ADV = 20
 SPH_S_IE_ALE_DIFF = 21
+SPH_S_SUR_YIELDING = 26
+SPH_S_SUR_FALLBACK = 27
 SPH_NSCAL = 32
 # reductions
 SPH_R_KINETIC = 0
@@ -157,7 +159,7 @@ class SphSolidParams(ctypes.Structure):
         "l_eig_tol", "q_l2", "q_cl", "av_alpha", "av_beta", "av_eps")] + [(n, ctypes.c_int32) for n in (
         "kernel_correction", "jaumann", "ale_history", "sts", "sts_stages", "sts_max_stages",
         "sts_refresh", "dt_pair_mode", "visc_work", "thermal", "plastic", "nmat", "ncons",
-        "pad_")]
+        "pad_")] + [("sur_tol", ctypes.c_double)]
 
 
 _vp2 = ctypes.c_void_p * 2
@@ -175,6 +177,10 @@ class SphSolid(ctypes.Structure):
         ("v_sync", ctypes.c_void_p),
         ("mat", ctypes.c_void_p),
         ("cons", ctypes.c_void_p),
+        ("surrogate", ctypes.c_void_p),
+        ("sur_x", ctypes.c_void_p), ("sur_y", ctypes.c_void_p), ("sur_flag", ctypes.c_void_p),
+        ("harvest_x", ctypes.c_void_p), ("harvest_y", ctypes.c_void_p),
+        ("harvest_cap", ctypes.c_int64), ("harvest_count", ctypes.c_void_p),
     ]
 
 

@@ -75,7 +75,7 @@ enum { PART_WDIFF = 0, PART_LEFT = 1, PART_VISC = 2, PART_WALL = 3, PART_R0 = 4,
        // the solid step's partials
        PART_PLASTIC = 7, PART_IEPAIR = 8, PART_IEADV = 9, PART_IEDIFF = 10, PART_IEALE = 11,
        PART_HGW = 12, PART_KE = 13, PART_KE2 = 14, PART_CROSS = 15, PART_VISCS = 16,
-       SPH_NPART = 17 };
+       PART_SURW = 17, PART_SURN = 18, PART_SURF = 19, SPH_NPART = 20 };
 enum { MAX_C2 = 0, MAX_ACC = 1, MAX_V = 2, MAX_MG = 3, MAX_QC = 4, MAX_AV = 5 };
 
 // a neighbour list: entry k of particle i at idx[k * npad + i], j | mirror << 30

@@ -39,6 +39,8 @@ struct DSolidP {
     double q_l2, q_cl;                   // VNR: C_Q l^2, C_Q l
     double av_alpha, av_beta, av_epsh2;  // Monaghan: alpha, beta, eps h^2
     int q_on, av_on;
+    int sur_on, harvest_on;              // the surrogate answers / plain returns are harvested
+    double sur_tol;
 };
 
 template <typename R> struct DSolid {
@@ -51,6 +53,8 @@ template <typename R> struct DSolid {
     R* gv;                            // [n][10]
     double4* y0; double4* y1; double4* vs;
     R* q;                             // [n] VNR q (when q_on)
+    float* sur_x; float* sur_y; int32_t* sur_flag;
+    float* hx; float* hy; long long hcap; unsigned long long* hcount;
     const sph_material* mat;
     double* cons;
 };
@@ -227,6 +231,43 @@ __device__ double return_map(const sph_material& M, double q[6], double eps_p, d
     if (d > 0.0)
         for (int k = 0; k < 6; ++k) q[k] *= sc;
     return d;
+}
+
+// The Johnson-Cook yield stress at zero increment (rate factor 1) and the thermal factor:
+// the elastic test of jc_radial_return (MAT002_johnson_cook.py:280-300)
+__device__ __forceinline__ double jc_static_yield(const sph_material& M, double eps_p, double temp, double* m_temp) {
+    double mt = 1.0;
+    if (M.jc_Tm > M.jc_T0 && temp > M.jc_T0) {
+        double ts = fmin(1.0, fmax(0.0, (temp - M.jc_T0) / (M.jc_Tm - M.jc_T0)));
+        mt = fmax(0.0, 1.0 - pow(ts, M.jc_m));
+    }
+    double s0 = M.jc_A;
+    if (eps_p > 0.0 && M.jc_B > 0.0 && M.jc_n > 0.0) s0 += M.jc_B * pow(eps_p, M.jc_n);
+    *m_temp = mt;
+    return s0 * mt;
+}
+
+// the surrogate's input row for a trial deviator q (xx, yy, xy, zz, xz, yz) of von Mises
+// value vm: the network's 12 inputs, then the temperature as a float pair ([12] + [15])
+// and vm ([14]) for the particle's finishing kernel
+__device__ void sur_features(const sph_material& M, const double q[6], double vm, double eps_p, double temp,
+                             double dt, double rho_ratio, float* x) {
+    const double inv = 1.0 / vm;
+    for (int k = 0; k < 6; ++k) x[k] = (float)(q[k] * inv);
+    double mt;
+    const double sy0 = jc_static_yield(M, eps_p, temp, &mt);
+    const double de0 = dt * M.jc_eps0_dot;
+    x[6] = (float)log(vm / M.jc_A);
+    x[7] = (float)eps_p;
+    x[8] = (float)((M.jc_Tm > M.jc_T0) ? fmin(1.0, fmax(0.0, (temp - M.jc_T0) / (M.jc_Tm - M.jc_T0))) : 0.0);
+    x[9] = (float)(de0 > 0.0 ? log1p(fmax(vm - sy0, 0.0) / (3.0 * M.G) / de0) : 0.0);
+    x[10] = (float)(rho_ratio - 1.0);
+    x[11] = (float)(de0 > 0.0 ? log(de0) : 0.0);
+    const float th = (float)temp;
+    x[12] = th;
+    x[13] = (float)eps_p;
+    x[14] = (float)vm;
+    x[15] = (float)(temp - (double)th);
 }
 
 // ------------------------------------------------------------------------------------
@@ -481,12 +522,48 @@ template <typename R> struct StressBody {
         if (S.transport_ep) ep = fmax(0.0, ep + dt * (double)a_ep);
         double Ti = (double)sd.T[i];
         if (S.plastic) {
-            double dep = return_map(M, q6, ep, Ti, dt);
-            ep += dep;
-            if (dep > 0.0) {
-                double ss = q6[0] * q6[0] + q6[1] * q6[1] + q6[3] * q6[3] +
-                            2.0 * (q6[2] * q6[2] + q6[4] * q6[4] + q6[5] * q6[5]);
-                w_plastic += (double)b.V[i] * sqrt(1.5 * ss) * dep;
+            bool deferred = false;
+            if (S.sur_on) {
+                // outside the yield surface at zero increment: the surrogate answers and
+                // k_sur_apply finishes the particle from the trial state kept here
+                int flag = 0;
+                if (M.plastic_kind == SPH_PLASTIC_JC && M.G > 0.0 && M.jc_A > 0.0) {
+                    double ss = q6[0] * q6[0] + q6[1] * q6[1] + q6[3] * q6[3] +
+                                2.0 * (q6[2] * q6[2] + q6[4] * q6[4] + q6[5] * q6[5]);
+                    double vm = sqrt(1.5 * ss), mt;
+                    double sy0 = jc_static_yield(M, ep, Ti, &mt);
+                    if (vm > sy0 && sy0 > 0.0) {
+                        sur_features(M, q6, vm, ep, Ti, dt, (double)rho_i / M.rho0, sd.sur_x + 16 * (size_t)i);
+                        flag = 1;
+                        deferred = true;
+                    }
+                }
+                sd.sur_flag[i] = flag;
+            }
+            if (!deferred) {
+                double q0[6];
+                if (S.harvest_on)
+                    for (int k = 0; k < 6; ++k) q0[k] = q6[k];
+                double dep = return_map(M, q6, ep, Ti, dt);
+                if (dep > 0.0) {
+                    double ss = q6[0] * q6[0] + q6[1] * q6[1] + q6[3] * q6[3] +
+                                2.0 * (q6[2] * q6[2] + q6[4] * q6[4] + q6[5] * q6[5]);
+                    w_plastic += (double)b.V[i] * sqrt(1.5 * ss) * dep;
+                    if (S.harvest_on && M.plastic_kind == SPH_PLASTIC_JC) {
+                        const unsigned long long row = atomicAdd(sd.hcount, 1ull);
+                        if ((long long)row < sd.hcap) {
+                            double s0 = q0[0] * q0[0] + q0[1] * q0[1] + q0[3] * q0[3] +
+                                        2.0 * (q0[2] * q0[2] + q0[4] * q0[4] + q0[5] * q0[5]);
+                            double vm = sqrt(1.5 * s0);
+                            sur_features(M, q0, vm, ep, Ti, dt, (double)rho_i / M.rho0, sd.hx + 16 * (size_t)row);
+                            float* y = sd.hy + 8 * (size_t)row;
+                            for (int k = 0; k < 6; ++k) y[k] = (float)(q6[k] / vm);
+                            y[6] = (float)(dep * 3.0 * M.G / vm);
+                            y[7] = (float)dep;
+                        }
+                    }
+                }
+                ep += dep;
             }
         }
         {
@@ -1068,20 +1145,95 @@ __global__ void k_solid_begin(double* scal, DSolidP S, double* cons, int ncons) 
     }
 }
 
+// Finishes the particles the stress pass handed to the surrogate: the network's answer
+// (returned deviator / vm, d_eps_p 3G / vm) is kept when d_eps_p >= 0 and, with the
+// Johnson-Cook yield stress sigma_y(d) at the answer, |vm(s) - sigma_y|, the radial-return
+// magnitude |vm(s) - (vm_trial - 3 G d)| and the direction |s/vm(s) - q/vm| (von Mises
+// norm, times sigma_y) are all within tol sigma_y; otherwise the plain return map answers.
+// Writes the deviator and eps_p, and the plastic work and the counts per block.
+template <typename R>
+__global__ __launch_bounds__(SPH_BLOCK) void k_sur_apply(int n, DSolid<R> sd, const R* __restrict__ V,
+                                                         const double* __restrict__ scal, double tol, DScratch s,
+                                                         int pstride) {
+    __shared__ double red[3 * SPH_BLOCK / SPH_WAVE];
+    const int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    const double dt = scal[SPH_S_DT];   // the step's dt, as the stress pass used it
+    double acc[3] = {0.0, 0.0, 0.0};   // plastic work, particles asked, fallbacks
+    if (i < n && sd.sur_flag[i]) {
+        const sph_material& M = sd.mat[sd.eos[i]];
+        const float* x = sd.sur_x + 16 * (size_t)i;
+        const float* y = sd.sur_y + 8 * (size_t)i;
+        R* so = sd.sig_out + 6 * (size_t)i;
+        double q[6];
+        for (int k = 0; k < 6; ++k) q[k] = (double)so[k];
+        const double ss = q[0] * q[0] + q[1] * q[1] + q[3] * q[3] + 2.0 * (q[2] * q[2] + q[4] * q[4] + q[5] * q[5]);
+        const double vm = sqrt(1.5 * ss);
+        const double ep = (double)sd.ep_out[i];
+        const double temp = (double)x[12] + (double)x[15];
+        double snew[6], d = (double)y[6] * vm / (3.0 * M.G);
+        bool ok = d >= 0.0 && vm > 0.0;
+        for (int k = 0; k < 6; ++k) {
+            snew[k] = (double)y[k] * vm;
+            ok = ok && isfinite(snew[k]);
+        }
+        if (ok) {
+            double mt, dg;
+            jc_static_yield(M, ep, temp, &mt);
+            const double g = jc_residual(d, vm, M.G, M, mt, ep, dt, &dg);
+            const double sy = vm - 3.0 * M.G * d - g;     // sigma_y at the answer
+            const double s2 = snew[0] * snew[0] + snew[1] * snew[1] + snew[3] * snew[3] +
+                              2.0 * (snew[2] * snew[2] + snew[4] * snew[4] + snew[5] * snew[5]);
+            const double vs = sqrt(1.5 * s2);
+            double dd = 0.0;
+            for (int k = 0; k < 6; ++k) {
+                const double e = snew[k] / fmax(vs, 1e-300) - q[k] / vm;
+                dd += (k == 2 || k >= 4 ? 2.0 : 1.0) * e * e;
+            }
+            const double dir = sqrt(1.5 * dd);
+            ok = sy > 0.0 && fabs(vs - sy) <= tol * sy && fabs(vs - (vm - 3.0 * M.G * d)) <= tol * sy &&
+                 dir <= tol;
+        }
+        acc[1] = 1.0;
+        if (!ok) {
+            for (int k = 0; k < 6; ++k) snew[k] = q[k];
+            d = return_map(M, snew, ep, temp, dt);
+            acc[2] = 1.0;
+        }
+        for (int k = 0; k < 6; ++k) so[k] = (R)snew[k];
+        sd.ep_out[i] = (R)(ep + d);
+        if (d > 0.0) {
+            const double s2 = snew[0] * snew[0] + snew[1] * snew[1] + snew[3] * snew[3] +
+                              2.0 * (snew[2] * snew[2] + snew[4] * snew[4] + snew[5] * snew[5]);
+            acc[0] = (double)V[i] * sqrt(1.5 * s2) * d;
+        }
+    }
+    block_sum_n<3>(acc, red);
+    if (threadIdx.x == 0) {
+        s.part[PART_SURW * pstride + blockIdx.x] = acc[0];
+        s.part[PART_SURN * pstride + blockIdx.x] = acc[1];
+        s.part[PART_SURF * pstride + blockIdx.x] = acc[2];
+    }
+}
+
 // compute_adaptive_dt (DSPH.py:1365-1430) with the solid's caps: clamp_dt_hourglass
 // (SOLID.py:4158-4161) or sts_stage_count (4510-4539); banks pass SA's works.
 __global__ __launch_bounds__(SPH_BLOCK) void k_dt_solid(DFluid F, DSolidP S, double* scal, DScratch s,
-                                                         int nparts_a, int nparts_b, int pstride) {
-    __shared__ double red[8 * SPH_BLOCK / SPH_WAVE];
-    double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const int slots[8] = {PART_WDIFF, PART_PLASTIC, PART_IEPAIR, PART_IEADV, PART_IEDIFF, PART_IEALE,
-                          PART_VISCS, PART_HGW};
+                                                         int nparts_a, int nparts_b, int nparts_s, int pstride) {
+    __shared__ double red[11 * SPH_BLOCK / SPH_WAVE];
+    double t[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int slots[11] = {PART_WDIFF, PART_PLASTIC, PART_IEPAIR, PART_IEADV, PART_IEDIFF, PART_IEALE,
+                           PART_VISCS, PART_HGW, PART_SURW, PART_SURN, PART_SURF};
     for (int k = threadIdx.x; k < nparts_a; k += blockDim.x)
         for (int q = 0; q < 6; ++q) t[q] += s.part[slots[q] * pstride + k];
     for (int k = threadIdx.x; k < nparts_b; k += blockDim.x)
         for (int q = 6; q < 8; ++q) t[q] += s.part[slots[q] * pstride + k];
-    block_sum_n<8>(t, red);
+    for (int k = threadIdx.x; k < nparts_s; k += blockDim.x)
+        for (int q = 8; q < 11; ++q) t[q] += s.part[slots[q] * pstride + k];
+    block_sum_n<11>(t, red);
     if (threadIdx.x == 0) {
+        t[1] += t[8];
+        scal[SPH_S_SUR_YIELDING] = t[9];
+        scal[SPH_S_SUR_FALLBACK] = t[10];
         double dt_prev = scal[SPH_S_DT];
         scal[SPH_S_DIFF_WORK] += dt_prev * t[0];
         scal[SPH_S_PLASTIC_WORK] += t[1];
@@ -1542,6 +1694,9 @@ template <typename R> static DSolid<R> dsolid(const sph_state* st, const sph_sol
     d.gv = (R*)sd->grad_v;
     d.y0 = (double4*)sd->hg_y[0]; d.y1 = (double4*)sd->hg_y[1]; d.vs = (double4*)sd->v_sync;
     d.q = (R*)sd->q_visc;
+    d.sur_x = sd->sur_x; d.sur_y = sd->sur_y; d.sur_flag = sd->sur_flag;
+    d.hx = sd->harvest_x; d.hy = sd->harvest_y; d.hcap = (long long)sd->harvest_cap;
+    d.hcount = (unsigned long long*)sd->harvest_count;
     d.mat = sd->mat;
     d.cons = sd->cons;
     return d;
@@ -1678,13 +1833,28 @@ static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F,
     int slots_a = launch_list_pass<StressBody<R>>(c, solid_args<R>(c, st, sd, F, S), hs, PASS_STRESS);
     timer_end(c, 4, hs);
     if ((rc = check_launch(c, "solid stress pass"))) return rc;
+    int slots_s = 0;
+    if (S.sur_on) {
+        // the surrogate's answers for the particles the stress pass handed over (every row is
+        // evaluated; the finishing kernel reads the flagged ones), then their finish
+        const sph_mlp* net = sd->surrogate;
+        MlpDev m{net->w, (const unsigned short*)net->wb, net->norm};
+        const int64_t n = c->dims.n;
+        hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, hs, m,
+                           (const float*)sd->sur_x, sd->sur_y, n);
+        const DSolid<R> dsd = dsolid<R>(st, sd);
+        hipLaunchKernelGGL((k_sur_apply<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, (int)n, dsd,
+                           (const R*)st->buf[st->cur].V, (const double*)st->scal, S.sur_tol, c->s, c->pstride);
+        slots_s = nb;
+        if ((rc = check_launch(c, "surrogate"))) return rc;
+    }
     sd->hcur = 1 - sd->hcur;
     // pass SB on the updated stress, volume and pressure
     timer_begin(c, 5, hs);
     int slots_b = launch_list_pass<SolidForceBody<R>>(c, solid_args<R>(c, st, sd, F, S), hs, PASS_SFORCE);
     timer_end(c, 5, hs);
     hipLaunchKernelGGL(k_dt_solid, dim3(1), dim3(SPH_BLOCK), 0, hs, F, S, st->scal, c->s, slots_a, slots_b,
-                       c->pstride);
+                       slots_s, c->pstride);
     if ((rc = check_launch(c, "solid force pass"))) return rc;
     if (S.sts) {
         hipLaunchKernelGGL((k_sts_kick1<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, dstate<R>(c, st), dsolid<R>(st, sd),
@@ -1739,6 +1909,9 @@ static int solid_step_launch(sph_ctx* c, sph_state* st, sph_solid* sd, const sph
     int rc;
     DFluid F = make_fluid(c, P);
     DSolidP D = make_solid_params(c, P, S);
+    D.sur_on = sd->surrogate && sd->sur_x && sd->sur_y && sd->sur_flag && S->plastic;
+    D.harvest_on = sd->harvest_x && sd->harvest_y && sd->harvest_count && sd->harvest_cap > 0 && S->plastic;
+    D.sur_tol = S->sur_tol > 0.0 ? S->sur_tol : 0.01;
     hipLaunchKernelGGL(k_solid_begin, dim3(1), dim3(256), 0, hs, st->scal, D, sd->cons, S->ncons);
     if ((rc = sph_solid_sort(c, st, sd, (void*)hs))) return rc;
     if (c->dims.precision == SPH_PREC_MIXED) return solid_step<float>(c, st, sd, F, D, hs);
@@ -1759,6 +1932,8 @@ extern "C" int sph_solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const sp
     if (S->sts && (S->sts_max_stages < 1 || S->sts_max_stages > 1024 || S->sts_stages < 0 ||
                    S->sts_stages > S->sts_max_stages))
         return set_err(c, SPH_EINVAL, "hourglass sub-step limits out of range%s", "", 0);
+    if (sd->surrogate && (!mlp_ok(sd->surrogate) || !sd->sur_x || !sd->sur_y || !sd->sur_flag))
+        return set_err(c, SPH_EINVAL, "surrogate without its network or buffers%s", "", 0);
     hipStream_t hs = (hipStream_t)stream;
     if (!c->cons_part) {
         HIPCHECK(c, hipMalloc((void**)&c->cons_part, sizeof(double) * 3 * SPH_CONS_BLOCK * (size_t)c->nblk));

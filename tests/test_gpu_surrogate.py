@@ -1,0 +1,137 @@
+"""The constitutive surrogate in the solid step.
+
+- Harvest: every recorded row is a plain Johnson-Cook radial return: the returned
+  deviator is the trial direction scaled by 1 - 3 G d / vm, and d satisfies the
+  consistency condition vm - 3 G d = sigma_y(eps_p + d, d / dt, T) (recomputed here from
+  the row, to 1e-5 of vm).
+- A network whose answers never pass the check leaves every yielding particle to the
+  plain return map: the run equals the plain run to round-off, and the step reports
+  every asked particle as a fallback.
+- A network trained on the GPU on rows harvested from plain steps answers most yielding
+  particles of the same loading within the tolerance, and the stress field stays within
+  2 % of the plain run's (the acceptance test bounds each accepted answer to 1 % of the
+  yield stress): what the network buys is the share of return maps it replaces.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from sigmasph_amd import _native as N
+from sigmasph_amd.config import SimConfig
+from sigmasph_amd.particle_system import ParticleSystem
+from sigmasph_amd.scenes import solid_block
+from sigmasph_amd.surrogate import Network, init_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _deck():
+    return solid_block(n=(24, 8, 1), grips=0.05, ramp_time=1e-4, div_velocity=(300.0, -150.0, 0.0),
+                       jitter=0.02)
+
+
+def _system(gpu):
+    ps = ParticleSystem(SimConfig(_deck()))
+    sv = ps.build_solver()
+    sv.initialize()
+    return ps, sv
+
+
+def _jc_sigma_y(M, ep, d, dt, T):
+    et = ep + d
+    ss = M.jc_A + (M.jc_B * et ** M.jc_n if et > 0 else 0.0)
+    rate = (d / dt) / M.jc_eps0_dot
+    mr = 1.0 + M.jc_C * math.log(rate) if rate > 1.0 else 1.0
+    mt = 1.0
+    if M.jc_Tm > M.jc_T0 and T > M.jc_T0:
+        mt = max(0.0, 1.0 - min(1.0, (T - M.jc_T0) / (M.jc_Tm - M.jc_T0)) ** M.jc_m)
+    return ss * mr * mt
+
+
+def test_harvest_rows_are_radial_returns(gpu):
+    ps, sv = _system(gpu)
+    sv.harvest(100000)
+    for _ in range(3):
+        sv.step()
+    x, y, seen = sv.harvested()
+    assert seen > 50 and x.shape[0] == seen
+    x, y = x.cpu().numpy().astype(np.float64), y.cpu().numpy().astype(np.float64)
+    M = sv.mat_table[0]
+    np.testing.assert_allclose(y[:, :6], x[:, :6] * (1.0 - y[:, 6:7]), atol=2e-6)
+    for row in range(0, len(x), max(1, len(x) // 200)):
+        vm, ep, T, d = x[row, 14], x[row, 13], x[row, 12] + x[row, 15], y[row, 7]
+        dt = math.exp(x[row, 11]) / M.jc_eps0_dot
+        g = vm - 3.0 * M.G * d - _jc_sigma_y(M, ep, d, dt, T)
+        assert abs(g) <= 1e-5 * vm, (row, g, vm)
+    sv.stop_harvest()
+    ps.close()
+
+
+def _fields(ps):
+    o = np.argsort(ps.pid.to_numpy())
+    return {f: getattr(ps, f).to_numpy()[o].astype(np.float64)
+            for f in ("x", "v", "sigma_dev", "eps_plastic", "pressure", "e_int")}
+
+
+def test_rejecting_network_equals_the_plain_run(gpu):
+    a, sa = _system(gpu)
+    b, sb = _system(gpu)
+    p = init_params(0)
+    p[:] = 0.0
+    net = Network(device=gpu, params=p)
+    net.norm[32 + 6] = -1.0            # d_eps_p answer -1 for every row: always rejected
+    sb.use_surrogate(net)
+    asked = 0
+    for _ in range(4):
+        sa.step()
+        sb.step()
+        k, f = sb.surrogate_stats()
+        assert k == f
+        asked += k
+    assert asked > 0
+    fa, fb = _fields(a), _fields(b)
+    for name in fa:
+        scale = max(np.abs(fa[name]).max(), 1e-30)
+        assert np.abs(fa[name] - fb[name]).max() <= 1e-12 * scale, name
+    pa = float(a.scal[N.SPH_S_PLASTIC_WORK].item())
+    pb = float(b.scal[N.SPH_S_PLASTIC_WORK].item())
+    assert pb == pytest.approx(pa, rel=1e-12)
+    a.close()
+    b.close()
+
+
+def test_trained_network_replaces_most_return_maps(gpu):
+    ps, sv = _system(gpu)
+    sv.harvest(400000)
+    for _ in range(40):
+        sv.step()
+    x, y, seen = sv.harvested()
+    sv.stop_harvest()
+    ps.close()
+    assert seen > 1000
+    net = Network(device=gpu, seed=2)
+    net.set_normalisation(x, y)
+    hist = net.train(x.contiguous(), y.contiguous(), steps=3000, batch=4096, lr=3e-3, lr_end=1e-4,
+                     log_every=500)
+    print("training loss", hist)
+    a, sa = _system(gpu)
+    b, sb = _system(gpu)
+    sb.use_surrogate(net, tol=0.01)
+    asked = fell = 0
+    for _ in range(20):
+        sa.step()
+        sb.step()
+        k, f = sb.surrogate_stats()
+        asked += k
+        fell += f
+    share = 1.0 - fell / max(asked, 1)
+    print("asked %d, answered by the network %.1f %%" % (asked, 100.0 * share))
+    fa, fb = _fields(a), _fields(b)
+    s_err = np.abs(fa["sigma_dev"] - fb["sigma_dev"]).max() / np.abs(fa["sigma_dev"]).max()
+    print("deviator error %.3g" % s_err)
+    assert asked > 0 and share > 0.5
+    assert s_err <= 0.02
+    a.close()
+    b.close()

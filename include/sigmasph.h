@@ -261,11 +261,13 @@ typedef struct {
      * sph_solid_params.sur_tol of the yield stress; the plain return map answers the
      * others.  With `harvest_x` set, every plain return with plastic flow appends its
      * inputs and answer (rows past harvest_cap are counted but not written).  Input row:
-     * trial deviator / its von Mises value (xx, yy, xy, zz, xz, yz), log(vm / A),
-     * eps_p, homologous temperature, log(1 + overshoot / (3 G dt eps0_dot)),
-     * rho / rho0 - 1, log(dt eps0_dot), then (not network inputs) the temperature as a
-     * float pair [12] + [15] and vm [14]; answer row: returned deviator / vm (6),
-     * d_eps_p 3G / vm, d_eps_p. */
+     * trial deviator / its von Mises value vm (xx, yy, xy, zz, xz, yz), log(vm / A),
+     * eps_p, homologous temperature, log(1 + d_lin / (dt eps0_dot)) with
+     * d_lin = (vm - sigma_y0) / 3G the no-hardening rate-free increment,
+     * (vm - sigma_y0) / vm, log(dt eps0_dot), then (not network inputs) the temperature
+     * as a float pair [12] + [15] and vm [14]; answer row: the correction of the
+     * returned direction to the trial one (von Mises-unit vectors, 6), d_eps_p / d_lin,
+     * d_eps_p. */
     const sph_mlp* surrogate;/* host pointer to the network, or NULL                         */
     float*   sur_x;          /* [n][16] inputs of the particles the surrogate answers          */
     float*   sur_y;          /* [n][8] its answers                                             */

@@ -261,8 +261,9 @@ __device__ void sur_features(const sph_material& M, const double q[6], double vm
     x[7] = (float)eps_p;
     x[8] = (float)((M.jc_Tm > M.jc_T0) ? fmin(1.0, fmax(0.0, (temp - M.jc_T0) / (M.jc_Tm - M.jc_T0))) : 0.0);
     x[9] = (float)(de0 > 0.0 ? log1p(fmax(vm - sy0, 0.0) / (3.0 * M.G) / de0) : 0.0);
-    x[10] = (float)(rho_ratio - 1.0);
+    x[10] = (float)(fmax(vm - sy0, 0.0) / vm);
     x[11] = (float)(de0 > 0.0 ? log(de0) : 0.0);
+    (void)rho_ratio;
     const float th = (float)temp;
     x[12] = th;
     x[13] = (float)eps_p;
@@ -556,9 +557,16 @@ template <typename R> struct StressBody {
                                         2.0 * (q0[2] * q0[2] + q0[4] * q0[4] + q0[5] * q0[5]);
                             double vm = sqrt(1.5 * s0);
                             sur_features(M, q0, vm, ep, Ti, dt, (double)rho_i / M.rho0, sd.hx + 16 * (size_t)row);
+                            // the answer: the returned direction's correction to the trial
+                            // direction (von Mises-unit vectors), 3 G d / vm, d
                             float* y = sd.hy + 8 * (size_t)row;
-                            for (int k = 0; k < 6; ++k) y[k] = (float)(q6[k] / vm);
-                            y[6] = (float)(dep * 3.0 * M.G / vm);
+                            double s1 = q6[0] * q6[0] + q6[1] * q6[1] + q6[3] * q6[3] +
+                                        2.0 * (q6[2] * q6[2] + q6[4] * q6[4] + q6[5] * q6[5]);
+                            double vr = sqrt(1.5 * s1);
+                            for (int k = 0; k < 6; ++k) y[k] = (float)(vr > 0.0 ? q6[k] / vr - q0[k] / vm : 0.0);
+                            double mt;
+                            const double dlin = (vm - jc_static_yield(M, ep, Ti, &mt)) / (3.0 * M.G);
+                            y[6] = (float)(dlin > 0.0 ? dep / dlin : 0.0);
                             y[7] = (float)dep;
                         }
                     }
@@ -1145,12 +1153,14 @@ __global__ void k_solid_begin(double* scal, DSolidP S, double* cons, int ncons) 
     }
 }
 
-// Finishes the particles the stress pass handed to the surrogate: the network's answer
-// (returned deviator / vm, d_eps_p 3G / vm) is kept when d_eps_p >= 0 and, with the
-// Johnson-Cook yield stress sigma_y(d) at the answer, |vm(s) - sigma_y|, the radial-return
-// magnitude |vm(s) - (vm_trial - 3 G d)| and the direction |s/vm(s) - q/vm| (von Mises
-// norm, times sigma_y) are all within tol sigma_y; otherwise the plain return map answers.
-// Writes the deviator and eps_p, and the plastic work and the counts per block.
+// Finishes the particles the stress pass handed to the surrogate.  The network answers
+// the returned direction as a correction c to the trial direction n = q / vm (both of von
+// Mises norm 1) and the plastic strain increment as the fraction d / d_lin of the
+// no-hardening, rate-free increment d_lin = (vm - sigma_y0) / 3G; the deviator is
+// s = (vm - 3 G d) (n + c) / |n + c|.  The answer is kept when d >= 0, the yield-surface
+// residual |vm(s) - sigma_y(eps_p + d, d / dt, T)| is within tol sigma_y and the
+// direction moved by at most tol; otherwise the plain return map answers.  Writes the
+// deviator and eps_p, and the plastic work and the counts per block.
 template <typename R>
 __global__ __launch_bounds__(SPH_BLOCK) void k_sur_apply(int n, DSolid<R> sd, const R* __restrict__ V,
                                                          const double* __restrict__ scal, double tol, DScratch s,
@@ -1170,28 +1180,29 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_sur_apply(int n, DSolid<R> sd, co
         const double vm = sqrt(1.5 * ss);
         const double ep = (double)sd.ep_out[i];
         const double temp = (double)x[12] + (double)x[15];
-        double snew[6], d = (double)y[6] * vm / (3.0 * M.G);
-        bool ok = d >= 0.0 && vm > 0.0;
+        double mt0;
+        const double dlin = (vm - jc_static_yield(M, ep, temp, &mt0)) / (3.0 * M.G);
+        double snew[6], dirv[6], d = (double)y[6] * dlin;
+        bool ok = d >= 0.0 && vm > 0.0 && dlin > 0.0 && isfinite(d);
+        double c2 = 0.0, u2 = 0.0;
         for (int k = 0; k < 6; ++k) {
-            snew[k] = (double)y[k] * vm;
-            ok = ok && isfinite(snew[k]);
+            const double c = (double)y[k];
+            const double w = (k == 2 || k >= 4) ? 2.0 : 1.0;
+            dirv[k] = q[k] / vm + c;
+            c2 += w * c * c;
+            u2 += w * dirv[k] * dirv[k];
+            ok = ok && isfinite(c);
         }
+        const double un = sqrt(1.5 * u2);
+        ok = ok && un > 0.0 && sqrt(1.5 * c2) <= tol;
         if (ok) {
             double mt, dg;
             jc_static_yield(M, ep, temp, &mt);
             const double g = jc_residual(d, vm, M.G, M, mt, ep, dt, &dg);
             const double sy = vm - 3.0 * M.G * d - g;     // sigma_y at the answer
-            const double s2 = snew[0] * snew[0] + snew[1] * snew[1] + snew[3] * snew[3] +
-                              2.0 * (snew[2] * snew[2] + snew[4] * snew[4] + snew[5] * snew[5]);
-            const double vs = sqrt(1.5 * s2);
-            double dd = 0.0;
-            for (int k = 0; k < 6; ++k) {
-                const double e = snew[k] / fmax(vs, 1e-300) - q[k] / vm;
-                dd += (k == 2 || k >= 4 ? 2.0 : 1.0) * e * e;
-            }
-            const double dir = sqrt(1.5 * dd);
-            ok = sy > 0.0 && fabs(vs - sy) <= tol * sy && fabs(vs - (vm - 3.0 * M.G * d)) <= tol * sy &&
-                 dir <= tol;
+            ok = sy > 0.0 && vm - 3.0 * M.G * d > 0.0 && fabs(g) <= tol * sy;
+            const double mag = (vm - 3.0 * M.G * d) / un;
+            for (int k = 0; k < 6; ++k) snew[k] = mag * dirv[k];
         }
         acc[1] = 1.0;
         if (!ok) {

@@ -1,9 +1,10 @@
 """The constitutive surrogate in the solid step.
 
 - Harvest: every recorded row is a plain Johnson-Cook radial return: the returned
-  deviator is the trial direction scaled by 1 - 3 G d / vm, and d satisfies the
-  consistency condition vm - 3 G d = sigma_y(eps_p + d, d / dt, T) (recomputed here from
-  the row, to 1e-5 of vm).
+  direction equals the trial direction (the recorded correction is zero to float
+  round-off), and d satisfies the consistency condition
+  vm - 3 G d = sigma_y(eps_p + d, d / dt, T) (recomputed here from the row, to 1e-5 of
+  vm).
 - A network whose answers never pass the check leaves every yielding particle to the
   plain return map: the run equals the plain run to round-off, and the step reports
   every asked particle as a fallback.
@@ -59,7 +60,9 @@ def test_harvest_rows_are_radial_returns(gpu):
     assert seen > 50 and x.shape[0] == seen
     x, y = x.cpu().numpy().astype(np.float64), y.cpu().numpy().astype(np.float64)
     M = sv.mat_table[0]
-    np.testing.assert_allclose(y[:, :6], x[:, :6] * (1.0 - y[:, 6:7]), atol=2e-6)
+    assert np.abs(y[:, :6]).max() <= 1e-6
+    np.testing.assert_allclose(y[:, 6] * x[:, 10] * x[:, 14] / (3.0 * M.G), y[:, 7], rtol=1e-5)
+    assert np.all(y[:, 6] > 0.0) and np.all(y[:, 6] <= 1.0 + 1e-6)
     for row in range(0, len(x), max(1, len(x) // 200)):
         vm, ep, T, d = x[row, 14], x[row, 13], x[row, 12] + x[row, 15], y[row, 7]
         dt = math.exp(x[row, 11]) / M.jc_eps0_dot

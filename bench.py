@@ -42,6 +42,9 @@ def make_scene(name):
         return scenes.tgv_block(n=160), "taylor-green block 160^3, periodic, dx=1mm"
     if name == "dambreak":
         return scenes.dam_break_3d(), "3-D dam break 0.4x0.4x0.2 m column, dx=0.01 m, reflect box"
+    if name == "taylor":
+        return (scenes.taylor_bar(), "3-D Taylor bar, copper cylinder 7.6 x 25.4 mm at dx=0.1 mm, 190 m/s "
+                "onto a rigid frictionless wall (y=0 symmetry plane), MG EOS + JC")
     if name == "ensemble":
         return (scenes.tensile_ensemble(), "256 plane-strain tensile specimens in one deck, W~U[2,8] "
                 "L~U[10,40] mm at dx=0.1 mm, per-specimen JC rows, 512 grips, hourglass STS")
@@ -59,10 +62,12 @@ def cpu_baseline(workload, precision, seconds=15.0):
     from sigmasph_amd import scenes
     threads = min(os.cpu_count() or 1, 16)
     O.set_threads(threads)
-    if workload in ("dogbone", "ensemble"):
+    if workload in ("dogbone", "ensemble", "taylor"):
         from sigmasph_amd.SOLID import SolidSettings
         if workload == "dogbone":
             sc, sample = scenes.dogbone_c2(), "the full dogbone (26 880 particles), same numerics"
+        elif workload == "taylor":
+            sc, sample = scenes.taylor_bar(dx=0.4), "the Taylor bar at dx = 0.4 mm (18 459 particles), same numerics"
         else:
             sc = scenes.ensemble_member(scenes.tensile_ensemble(), 0)
             sample = "specimen 0 of the ensemble, same numerics"
@@ -137,6 +142,91 @@ def looped_baseline(deck, args, n_total):
                     % n_total}
 
 
+def surrogate_study(args, speeds=(100.0, 190.0, 250.0), harvest_steps=60, eval_steps=100, warm=20):
+    """The Taylor bar (dx = 0.1 mm): harvest plain fp64 Johnson-Cook returns at three impact
+    speeds, train the surrogate on the GPU, then run the 190 m/s impact from t = 0 four ways
+    -- plain fp64, plain mixed, surrogate + fp64, surrogate + mixed -- timing steps
+    warm..eval_steps of each and comparing the final deviator field with the plain fp64 run."""
+    import torch
+    import numpy as np
+    from sigmasph_amd import scenes
+    from sigmasph_amd.config import SimConfig
+    from sigmasph_amd.particle_system import ParticleSystem
+    from sigmasph_amd.surrogate import Network
+    xs, ys = [], []
+    t0 = time.perf_counter()
+    for v in speeds:
+        ps = ParticleSystem(SimConfig(scenes.taylor_bar(speed=v)), precision="fp64")
+        sv = ps.build_solver()
+        sv.initialize()
+        sv.harvest(1500000)
+        for _ in range(harvest_steps):
+            sv.step()
+        x, y, seen = sv.harvested()
+        xs.append(x.clone())
+        ys.append(y.clone())
+        log("harvest %g m/s: %d rows" % (v, seen))
+        ps.close()
+    X, Y = torch.cat(xs).contiguous(), torch.cat(ys).contiguous()
+    t_h = time.perf_counter() - t0
+    net = Network(seed=0)
+    net.set_normalisation(X, Y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = net.train(X, Y, steps=4000, batch=8192, lr=3e-3, lr_end=1e-4, log_every=1000)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    log("trained on %d rows in %.1f s: %s" % (X.shape[0], t_train, hist))
+
+    def run(precision, surrogate):
+        ps = ParticleSystem(SimConfig(scenes.taylor_bar(speed=190.0)), precision=precision)
+        sv = ps.build_solver()
+        sv.initialize()
+        if surrogate:
+            sv.use_surrogate(net, tol=0.01)
+        asked = fell = 0
+        for k in range(eval_steps):
+            if k == warm:
+                torch.cuda.synchronize()
+                tw = time.perf_counter()
+            sv.step()
+            if surrogate and k >= warm:
+                a, f = sv.surrogate_stats()
+                asked += a
+                fell += f
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - tw) / (eval_steps - warm)
+        o = np.argsort(ps.pid.to_numpy())
+        sig = ps.sigma_dev.to_numpy()[o].astype(np.float64)
+        ep = ps.eps_plastic.to_numpy()[o].astype(np.float64)
+        ps.close()
+        return ms, sig, ep, asked, fell
+
+    res = {}
+    base = None
+    for prec, sur in (("fp64", False), ("mixed", False), ("fp64", True), ("mixed", True)):
+        ms, sig, ep, asked, fell = run(prec, sur)
+        key = "%s%s" % (prec, "+surrogate" if sur else "")
+        r = {"ms_per_step": ms}
+        if base is None:
+            base = (sig, ep)
+        else:
+            r["deviator_err_max"] = float(np.abs(sig - base[0]).max() / np.abs(base[0]).max())
+            r["deviator_err_rms"] = float(np.sqrt(np.mean((sig - base[0]) ** 2)) / np.sqrt(np.mean(base[0] ** 2)))
+            r["eps_p_err_max"] = float(np.abs(ep - base[1]).max() / max(np.abs(base[1]).max(), 1e-30))
+        if sur:
+            r["answered_by_network"] = 1.0 - fell / max(asked, 1)
+            r["asked_per_step"] = asked / (eval_steps - warm)
+        res[key] = r
+        log("%s: %s" % (key, r))
+    plain = res["fp64"]["ms_per_step"]
+    return {"rows": int(X.shape[0]), "harvest_s": t_h, "train_s": t_train, "train_loss": hist,
+            "runs": res, "speedup_surrogate_mixed_vs_plain_fp64": plain / res["mixed+surrogate"]["ms_per_step"],
+            "speedup_mixed_vs_plain_fp64": plain / res["mixed"]["ms_per_step"],
+            "note": "190 m/s impact from t = 0, %d steps, timed over steps %d-%d; errors of the final "
+                    "deviator against the plain fp64 run" % (eval_steps, warm, eval_steps)}
+
+
 def measured_traffic(workload, precision, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter runs
     (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration), or None
@@ -156,7 +246,11 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
-    ap.add_argument("--workload", default="dogbone", choices=("dogbone", "tgv160", "dambreak", "ensemble"))
+    ap.add_argument("--workload", default="dogbone",
+                    choices=("dogbone", "tgv160", "dambreak", "ensemble", "taylor"))
+    ap.add_argument("--surrogate", action="store_true",
+                    help="taylor: harvest, train the constitutive surrogate on the GPU, and compare "
+                         "surrogate + mixed precision against the plain fp64 step")
     ap.add_argument("--autotune", action="store_true",
                     help="tune the lanes per pair pass and the hourglass operator before timing")
     ap.add_argument("--loop-sample", type=int, default=16,
@@ -285,6 +379,8 @@ def main():
     ps.close()
     if args.workload == "ensemble":
         out["looped"] = looped_baseline(scene, args, n)
+    if args.surrogate:
+        out["surrogate"] = surrogate_study(args)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.workload, args.precision)

@@ -331,3 +331,41 @@ def ensemble_member(deck, k):
     one["Materials"] = [copy.deepcopy(m) for m in deck["Materials"] if m["name"] == blk["material"]]
     one["Constraints"] = copy.deepcopy(deck["Constraints"][2 * k: 2 * k + 2])
     return one
+
+
+def taylor_bar(dx=0.1, speed=190.0, radius=3.8, length=25.4, cfl=0.3, hourglass=4.0, sts=True,
+               monaghan=(1.0, 2.0)):
+    """The Taylor-bar impact: a 3-D copper cylinder (diameter 2 `radius`, `length` mm long,
+    axis y) at `speed` mm/ms towards the y = 0 symmetry plane, which is a rigid
+    frictionless wall; lattice dx from y = dx/2.  Copper: Mie-Grueneisen (c0 = 3940 mm/ms,
+    s = 1.49, Gamma0 = 2.0), rho 8.96e-6, E = 124 GPa, nu = 0.34, Johnson-Cook A 0.09,
+    B 0.292, n 0.31, C 0.025, m 1.09 with the library copper's T0, Tm.  Numerics: support
+    3 dx, CFL 0.3, Morris viscosity 0.02, Monaghan's viscosity (alpha, beta) for the
+    impact front, delta 0.1, Colle shifting with ALE history, the rate hourglass damper."""
+    mat = {"name": "copper", "library": "copper",
+           "eos": {"type": "mie_gruneisen", "c0": 3940.0, "s": 1.49, "gamma0": 2.0},
+           "strength": {"youngsModulus": 124.0, "poissonRatio": 0.34,
+                        "plasticity": {"A": 0.09, "B": 0.292, "n": 0.31, "C": 0.025, "m": 1.09}}}
+    r = radius
+    solver = {"type": "hypoElastic", "viscosity": 0.02, "allowNegativePressure": True}
+    if monaghan:
+        solver["artificialViscosity"] = {"alpha": monaghan[0], "beta": monaghan[1]}
+    sc = {
+        "Domain": {"start": [-2 * r, 0.0, -2 * r], "end": [2 * r, length + 5.0, 2 * r],
+                   "particleRadius": 0.5 * dx, "supportRadiusFactor": 6.0, "symmetryPlanes": "y"},
+        "Time": {"CFL": cfl},
+        "Solver": solver,
+        "Materials": [mat],
+        "Load": {"gravitation": [0.0, 0.0, 0.0]},
+        "Diffusion": {"delta": 0.1},
+        "Shifting": {"enabled": True, "mode": "colle", "maMode": "fixed", "maFixed": 0.05,
+                     "R": 0.2, "n": 4, "maxShift": 0.05, "aleMomentum": True, "aleHistory": True},
+        "Hourglass": {"alpha": hourglass},
+        "SolidBlocks": [{"objectId": 0, "start": [-r + 0.5 * dx, 0.5 * dx, -r + 0.5 * dx],
+                         "end": [r, length, r], "density": 8.96e-06, "velocity": [0.0, -speed, 0.0],
+                         "material": "copper",
+                         "shape": {"type": "cylinder", "radius": r, "axis": "y", "center": [0.0, 0.0, 0.0]}}],
+    }
+    if sts:
+        sc["Hourglass"]["superTimeStepping"] = {"mode": "subcycle"}
+    return sc

@@ -19,7 +19,7 @@ from sigmasph_amd import _native as N
 from sigmasph_amd.config import SimConfig
 from sigmasph_amd.particle_system import ParticleSystem
 from sigmasph_amd.SOLID import SolidSettings
-from sigmasph_amd.scenes import solid_block, tensile_ensemble
+from sigmasph_amd.scenes import solid_block, tensile_ensemble, taylor_bar
 from helpers import solid_oracle_state
 
 pytestmark = pytest.mark.gpu
@@ -45,6 +45,9 @@ SCENES = {
     # forces are summed by atomic adds)
     "ensemble9": lambda: _stretched(tensile_ensemble(count=9, width=(0.8, 1.2), length=(3.0, 4.0),
                                                      grip=0.5, rate=30.0)),
+    # a small 3-D Taylor bar: Mie-Grueneisen copper on the y = 0 symmetry plane (mirror
+    # images in 3-D), Monaghan's viscosity, the super-time-stepped damper
+    "taylor3d": lambda: taylor_bar(dx=0.4, speed=190.0, radius=1.2, length=2.8),
     # Mie-Grueneisen aluminium under compression with the VNR q and Monaghan's Pi
     "axisymmetric_shock": lambda: solid_block(
         n=(24, 8, 1), two_d="axisymmetric", material={"name": "projectile", "library": "al2017_t4"},
@@ -148,7 +151,7 @@ def test_five_steps_fp64(gpu, name):
 
 
 @pytest.mark.parametrize("name", ["plane_strain_grips", "axisymmetric_grips", "block3d_explicit",
-                                  "axisymmetric_shock"])
+                                  "axisymmetric_shock", "taylor3d"])
 def test_three_steps_mixed(gpu, name):
     ps, solver, st, s = run_pair(SCENES[name](), 3, precision="mixed")
     compare(ps, st, s, 2e-4, eps=2.0 ** -24)

@@ -1041,8 +1041,9 @@ template <typename R> struct HgBody {
     }
     // The first stage of a list slot: the same term split as A_ij (y_j - y_i) + c_ij e_ij
     // with A_ij = w_j W_ij e e^T and c_ij = w_j W_ij e . <G> r, neither of which changes
-    // over the stages (grad v is refreshed once, before them); A goes to the slot, c e
-    // into the particle's stage-invariant source.
+    // over the stages (grad v is refreshed once, before them); A goes to the slot as its
+    // factors (w_j W_ij, e: 3 values in 2-D, 4 in 3-D), c e into the particle's
+    // stage-invariant source.
     __device__ void pair_store(int j, int mir, double rxd, double ryd, double rzd, size_t slot, size_t cstride) {
         const DFluid& F = a.F;
         const DBuf<R>& b = a.st.b;
@@ -1052,7 +1053,7 @@ template <typename R> struct HgBody {
         R* A = a.hgA + slot;
         if (!(rn > (R)1e-9)) {
             A[0] = 0; A[cstride] = 0; A[2 * cstride] = 0;
-            if (!two_d) { A[3 * cstride] = 0; A[4 * cstride] = 0; A[5 * cstride] = 0; }
+            if (!two_d) A[3 * cstride] = 0;
             return;
         }
         double4 yj = a.ysrc[j];
@@ -1070,14 +1071,15 @@ template <typename R> struct HgBody {
         R sw = wq<R>(F, b, b.V, j) * ((R)F.alpha * om * om * om * om * ((R)4 * q + (R)1));
         R dyx = (R)(yj.x - yi.x), dyy = (R)(yj.y - yi.y), dyz = (R)(yj.z - yi.z);
         R c = sw * (gx * ex + gy * ey + gz * ez);
-        R axx = sw * ex * ex, axy = sw * ex * ey, ayy = sw * ey * ey;
-        A[0] = axx; A[cstride] = axy; A[2 * cstride] = ayy;
-        hx += axx * dyx + axy * dyy; hy += axy * dyx + ayy * dyy;
+        A[0] = sw; A[cstride] = ex; A[2 * cstride] = ey;
+        R e_d = ex * dyx + ey * dyy;
         if (!two_d) {
-            R azz = sw * ez * ez, axz = sw * ex * ez, ayz = sw * ey * ez;
-            A[3 * cstride] = azz; A[4 * cstride] = axz; A[5 * cstride] = ayz;
-            hx += axz * dyz; hy += ayz * dyz; hz += axz * dyx + ayz * dyy + azz * dyz;
+            A[3 * cstride] = ez;
+            e_d += ez * dyz;
         }
+        R f = sw * e_d;
+        hx += f * ex; hy += f * ey;
+        if (!two_d) hz += f * ez;
         bx += c * ex; by += c * ey; bz += c * ez;
     }
     // a later stage: A_ij (M y_j - y_i) from the slot
@@ -1087,13 +1089,16 @@ template <typename R> struct HgBody {
         if (mir & 2) yj.z = -yj.z;
         const R* A = a.hgA + slot;
         R dyx = (R)(yj.x - yi.x), dyy = (R)(yj.y - yi.y);
-        R axx = A[0], axy = A[cstride], ayy = A[2 * cstride];
-        hx += axx * dyx + axy * dyy; hy += axy * dyx + ayy * dyy;
+        R sw = A[0], ex = A[cstride], ey = A[2 * cstride];
+        R e_d = ex * dyx + ey * dyy;
+        R ez = 0;
         if (!two_d) {
-            R dyz = (R)(yj.z - yi.z);
-            R azz = A[3 * cstride], axz = A[4 * cstride], ayz = A[5 * cstride];
-            hx += axz * dyz; hy += ayz * dyz; hz += axz * dyx + ayz * dyy + azz * dyz;
+            ez = A[3 * cstride];
+            e_d += ez * (R)(yj.z - yi.z);
         }
+        R f = sw * e_d;
+        hx += f * ex; hy += f * ey;
+        if (!two_d) hz += f * ez;
     }
     template <int LPP> __device__ void lane_reduce() {
         if (LPP == 1) return;
@@ -1964,7 +1969,7 @@ extern "C" int sph_solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const sp
         c->nl.npad = npad;
     }
     if (S->sts && S->hourglass > 0.0 && c->hg_pairs && !c->hg_pair) {
-        const int ncomp = (c->dims.flags & SPH_TWO_D) ? 3 : 6;
+        const int ncomp = (c->dims.flags & SPH_TWO_D) ? 3 : 4;
         const size_t rsz = c->dims.precision == SPH_PREC_MIXED ? sizeof(float) : sizeof(double);
         HIPCHECK(c, hipMalloc(&c->hg_pair, rsz * ncomp * (size_t)c->nl.nmax * c->nl.npad));
         HIPCHECK(c, hipMalloc((void**)&c->hg_src, sizeof(double4) * (size_t)c->nl.npad));

@@ -49,8 +49,8 @@ def solid_kernel_bytes(precision="fp64", two_d=True, pst=True, sts=True, stages=
     The step's neighbour list is a per-pair array, not a neighbour re-read: the build
     writes 4 B per entry (`list_pairs` entries per particle, mirror images included) and
     every list pass reads them.  With the stored hourglass operator (`hg_pairs`) the
-    first stage of a step writes 3 (2-D) or 4 (3-D) values per entry -- the pair's
-    weight and unit vector -- and later stages read them in
+    first stage of a step writes 2 (2-D) or 3 (3-D) values per entry -- u = sqrt(w W) e,
+    the operator being u u^T -- and later stages read them in
     place of x_j and grad v_j; `hourglass_stage` is then the average over the step's
     `stages` launches."""
     r = 8 if precision == "fp64" else 4
@@ -93,7 +93,7 @@ def solid_kernel_bytes(precision="fp64", two_d=True, pst=True, sts=True, stages=
         out["grad_v"] += lst
         full = out["hourglass_stage"] + lst
         if hg_pairs:
-            A = (3 if two_d else 4) * r * list_pairs
+            A = (2 if two_d else 3) * r * list_pairs
             first = full + A + 32
             later = v3 + 32 + 2 * r + 4 + lst + A + v3
             out["hourglass_stage"] = (first + (max(stages, 1) - 1) * later) / max(stages, 1)

@@ -1279,7 +1279,7 @@ struct sph_ctx {
     int cons_cap = 0;
     NList nl = {nullptr, nullptr, 0, 0};   // the solid step's neighbour list
     // the hourglass sub-steps' stage-invariant pair operator, filled by the first stage
-    void* hg_pair = nullptr;       // [ncomp][nmax][npad] of R: w_j W_ij e e^T
+    void* hg_pair = nullptr;       // [ncomp][nmax][npad] of R: u = sqrt(w_j W_ij) e
     double4* hg_src = nullptr;     // [npad] sum_j w_j W_ij (e . <G> r) e
     bool hg_pairs = true;          // later stages read them instead of recomputing
     // captured steps (sph_graphs): replayed when the same arguments come back
@@ -1424,8 +1424,17 @@ extern "C" int sph_ctx_create(sph_ctx** out, const sph_dims* d) {
         const int cap[PASS_KINDS] = {2, 4, 8, 8, 4, 2, 8, 8};
         static const char* names[PASS_KINDS] = {"VOLUME", "FORCE", "SHIFT", "PAIRS", "STRESS",
                                                 "SFORCE", "GRADV", "HG"};
+        // the list passes also split a long neighbour list over lanes on a full chip: at
+        // least one lane per ~14 expected neighbours, at most 4 (measured on one MI355X:
+        // 2 lanes at 28 neighbours in 2-D, 4 at 113 in 3-D, each about 1.35x and 2x the
+        // one-lane pass at 10^6 particles)
+        const double q = d->dx0 > 0.0 ? d->h / d->dx0 : 2.6;
+        const double expect = two_d ? M_PI * q * q : 4.0 / 3.0 * M_PI * q * q * q;
+        int lanes_list = 1;
+        while (lanes_list < 4 && 14.0 * (2 * lanes_list) <= expect) lanes_list *= 2;
         for (int k = 0; k < PASS_KINDS; ++k) {
             int v = std::min(lpp, cap[k]);
+            if (k >= PASS_STRESS) v = std::max(v, lanes_list);
             char key[64];
             snprintf(key, sizeof key, "SIGMASPH_LPP_%s", names[k]);
             if (const char* e = getenv(key)) {

@@ -25,6 +25,7 @@
 #define SPH_S_PEND_VISC 23
 #define SPH_S_PEND_HG   24
 #define SPH_S_KE_POST   25
+#define SPH_S_LIST_OVER 28   // 1 when some particle's list overflowed this step (else 0)
 #define SPH_MAX_CONS    4096  // grips a step takes
 #define SPH_CONS_BLOCK  16    // up to this many, grip forces are fixed-order block sums
 #define SPH_LIST_LANES  4     // lanes per particle of the neighbour-list build
@@ -69,7 +70,8 @@ template <typename R> struct SolidArgs {
     int pstride;
     // hourglass sub-step pass only
     const double4* ysrc; double4* ydst; int stage;
-    R* hgA; double4* hgb; int hg_mode;   // 0 recompute, 1 recompute and store, 2 read stored
+    // 0 recompute, 1 recompute and store, 2 read stored, 3 recompute overflowed particles only
+    R* hgA; double4* hgb; int hg_mode;
     __device__ bool any_out() const { return st.scal[SPH_S_OUT_OF_BOX] > 0.5; }
     // a sub-step pass beyond this step's count has nothing to do
     __device__ bool skip() const { return ydst != nullptr && stage >= (int)st.scal[SPH_S_STS_STAGES]; }
@@ -670,7 +672,9 @@ template <typename R> struct StressBody {
 // pass SB: compute_lam second pass + compute_stress_accel_task (SOLID.py:3544-3852) +
 //          compute_dt_task (4116-4156) per pair; the stress accelerations per particle
 // ------------------------------------------------------------------------------------
-template <typename R> struct SolidForceBody {
+// HGX false compiles the explicit hourglass damper out (a step that sub-cycles it, or
+// runs without it), which frees its registers: the two agree whenever hg_explicit is 0.
+template <typename R, bool HGX = true> struct SolidForceBody {
     typedef SolidArgs<R> Args;
     const Args& a;
     double c2loc, amag, w_visc, w_hg, av_c;
@@ -707,7 +711,7 @@ template <typename R> struct SolidForceBody {
         eos_i = a.sd.eos[i];
         cbi = (R)a.sd.mat[eos_i].c_p;
         si = load_sig(a.sd.sig_in, i, two_d);
-        if (a.S.hg_explicit) load_gv(a.sd.gv, i, two_d, Gi);
+        if (HGX && a.S.hg_explicit) load_gv(a.sd.gv, i, two_d, Gi);
         glx = gly = glz = swx = swy = swz = fjx = fjy = fjz = wsig = 0;
         avx = avy = avz = wmor = hx = hy = hz = whg = aax = aay = aaz = 0;
         pix = piy = piz = wpi = mu_max = 0;
@@ -767,7 +771,7 @@ template <typename R> struct SolidForceBody {
                 mu_max = fmax(mu_max, -mu);
             }
         }
-        if (S.hg_explicit && rn > (R)1e-9) {
+        if (HGX && S.hg_explicit && rn > (R)1e-9) {
             // the rate damper on v (SOLID.py:3778-3825): dv_hg = (v_j - v_i) - <G> (x_j - x_i)
             R Gj[3][3];
             load_gv(a.sd.gv, j, two_d, Gj);
@@ -809,8 +813,10 @@ template <typename R> struct SolidForceBody {
         wsig = lane_sum<LPP>(wsig);
         avx = lane_sum<LPP>(avx); avy = lane_sum<LPP>(avy); avz = lane_sum<LPP>(avz);
         wmor = lane_sum<LPP>(wmor);
-        hx = lane_sum<LPP>(hx); hy = lane_sum<LPP>(hy); hz = lane_sum<LPP>(hz);
-        whg = lane_sum<LPP>(whg);
+        if (HGX) {
+            hx = lane_sum<LPP>(hx); hy = lane_sum<LPP>(hy); hz = lane_sum<LPP>(hz);
+            whg = lane_sum<LPP>(whg);
+        }
         aax = lane_sum<LPP>(aax); aay = lane_sum<LPP>(aay); aaz = lane_sum<LPP>(aaz);
         c2max = lane_max<LPP>(c2max);
         if (a.S.av_on) {
@@ -862,7 +868,7 @@ template <typename R> struct SolidForceBody {
             }
         }
         double dhg = 0.0;
-        if (S.hg_explicit) {
+        if (HGX && S.hg_explicit) {
             // alpha (rho0 / rho_i) (c_p / h) (1 / V0) (1 / W(dx0))
             double k = S.alpha_hg * (M.rho0 / (double)rho_i) * (M.c_p * F.inv_h) * S.inv_V0 * S.inv_wdx0;
             ax += k * (double)hx; ay += k * (double)hy; az += k * (double)hz;
@@ -1041,9 +1047,9 @@ template <typename R> struct HgBody {
     }
     // The first stage of a list slot: the same term split as A_ij (y_j - y_i) + c_ij e_ij
     // with A_ij = w_j W_ij e e^T and c_ij = w_j W_ij e . <G> r, neither of which changes
-    // over the stages (grad v is refreshed once, before them); A goes to the slot as its
-    // factors (w_j W_ij, e: 3 values in 2-D, 4 in 3-D), c e into the particle's
-    // stage-invariant source.
+    // over the stages (grad v is refreshed once, before them); A goes to the slot as
+    // u = sqrt(w_j W_ij) e, A = u u^T (w_j W_ij >= 0: W = alpha (1-q)^4 (4q+1) on q >= 0;
+    // 2 values in 2-D, 3 in 3-D), c e into the particle's stage-invariant source.
     __device__ void pair_store(int j, int mir, double rxd, double ryd, double rzd, size_t slot, size_t cstride) {
         const DFluid& F = a.F;
         const DBuf<R>& b = a.st.b;
@@ -1052,8 +1058,8 @@ template <typename R> struct HgBody {
         R rn = fsqrt<R>(rx * rx + ry * ry + rz * rz);
         R* A = a.hgA + slot;
         if (!(rn > (R)1e-9)) {
-            A[0] = 0; A[cstride] = 0; A[2 * cstride] = 0;
-            if (!two_d) A[3 * cstride] = 0;
+            A[0] = 0; A[cstride] = 0;
+            if (!two_d) A[2 * cstride] = 0;
             return;
         }
         double4 yj = a.ysrc[j];
@@ -1071,34 +1077,37 @@ template <typename R> struct HgBody {
         R sw = wq<R>(F, b, b.V, j) * ((R)F.alpha * om * om * om * om * ((R)4 * q + (R)1));
         R dyx = (R)(yj.x - yi.x), dyy = (R)(yj.y - yi.y), dyz = (R)(yj.z - yi.z);
         R c = sw * (gx * ex + gy * ey + gz * ez);
-        A[0] = sw; A[cstride] = ex; A[2 * cstride] = ey;
-        R e_d = ex * dyx + ey * dyy;
+        R su = fsqrt<R>(fmax(sw, (R)0));
+        R ux = su * ex, uy = su * ey, uz = su * ez;
+        A[0] = ux; A[cstride] = uy;
+        R f = ux * dyx + uy * dyy;
         if (!two_d) {
-            A[3 * cstride] = ez;
-            e_d += ez * dyz;
+            A[2 * cstride] = uz;
+            f += uz * dyz;
+            hz += f * uz;
         }
-        R f = sw * e_d;
-        hx += f * ex; hy += f * ey;
-        if (!two_d) hz += f * ez;
+        hx += f * ux; hy += f * uy;
         bx += c * ex; by += c * ey; bz += c * ez;
     }
-    // a later stage: A_ij (M y_j - y_i) from the slot
+    // a later stage: A_ij (M y_j - y_i) = u (u . (M y_j - y_i)) from the slot
+    __device__ void begin_fast(int i) {
+        yi = a.ysrc[i];
+        hx = hy = hz = 0;
+    }
     __device__ void pair_fast(int j, int mir, size_t slot, size_t cstride, bool two_d) {
         double4 yj = a.ysrc[j];
         if (mir & 1) yj.y = -yj.y;
         if (mir & 2) yj.z = -yj.z;
         const R* A = a.hgA + slot;
-        R dyx = (R)(yj.x - yi.x), dyy = (R)(yj.y - yi.y);
-        R sw = A[0], ex = A[cstride], ey = A[2 * cstride];
-        R e_d = ex * dyx + ey * dyy;
-        R ez = 0;
-        if (!two_d) {
-            ez = A[3 * cstride];
-            e_d += ez * (R)(yj.z - yi.z);
+        R ux = A[0], uy = A[cstride];
+        R f = ux * (R)(yj.x - yi.x) + uy * (R)(yj.y - yi.y);
+        if (two_d) {
+            hx += f * ux; hy += f * uy;
+        } else {
+            R uz = A[2 * cstride];
+            f += uz * (R)(yj.z - yi.z);
+            hx += f * ux; hy += f * uy; hz += f * uz;
         }
-        R f = sw * e_d;
-        hx += f * ex; hy += f * ey;
-        if (!two_d) hz += f * ez;
     }
     template <int LPP> __device__ void lane_reduce() {
         if (LPP == 1) return;
@@ -1147,6 +1156,7 @@ __global__ void k_solid_begin(double* scal, DSolidP S, double* cons, int ncons) 
     scal[SPH_S_HG_WORK] += dt * scal[SPH_S_PEND_HG];
     scal[SPH_S_LEFTOVER] += scal[SPH_S_PEND_LEFT];
     scal[SPH_S_PEND_VISC] = scal[SPH_S_PEND_HG] = scal[SPH_S_PEND_LEFT] = 0.0;
+    scal[SPH_S_LIST_OVER] = 0.0;
     double t = scal[SPH_S_SIM_TIME];
     if (S.ramp_time <= 0.0) {
         scal[SPH_S_LOAD_S] = 1.0;
@@ -1541,7 +1551,7 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_solid_reduce(int n, DBuf<R> b, DS
 template <int L>
 __global__ __launch_bounds__(SPH_BLOCK) void k_build_list(DGrid g, int n, const int* __restrict__ start,
                                                           const int4* __restrict__ cellid,
-                                                          const double4* __restrict__ X, const double* scal,
+                                                          const double4* __restrict__ X, double* scal,
                                                           NList nl) {
     const int t = blockIdx.x * SPH_BLOCK + threadIdx.x;
     const int i = t / L, lane = t % L;
@@ -1561,7 +1571,10 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_build_list(DGrid g, int n, const 
     };
     TraverseLanes<L> tr{g, start, cellid, X, scal[SPH_S_OUT_OF_BOX] > 0.5, lane};
     tr(i, xi.x, xi.y, xi.z, f);
-    if (lane == 0) nl.cnt[i] = c;
+    if (lane == 0) {
+        nl.cnt[i] = c;
+        if (c > nl.nmax) scal[SPH_S_LIST_OVER] = 1.0;   // every writer stores the same value
+    }
 }
 
 // One pair pass over the list: LPP consecutive lanes share a particle and take every
@@ -1601,38 +1614,39 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_list(DGrid g, int n, typename Bod
 }
 
 // One hourglass sub-step over the list.  Mode 1 (the first stage) evaluates each pair in
-// full and stores its stage-invariant operator; mode 2 (later stages) applies the stored
-// operator to the new stage field.  Particles whose list overflowed traverse the cells
-// and evaluate in full at every stage.
+// full and stores its stage-invariant operator (the later stages apply it: k_hg_apply);
+// mode 3 (a later stage) runs only the particles whose list overflowed, and returns at
+// once when none did.  Overflowed particles traverse the cells and evaluate in full at
+// every stage.
 template <typename R, int LPP>
 __global__ __launch_bounds__(SPH_BLOCK) void k_hg_list(DGrid g, int n, SolidArgs<R> a, NList nl) {
     if (a.skip()) return;   // uniform over the grid
+    if (a.hg_mode == 3 && a.st.scal[SPH_S_LIST_OVER] == 0.0) return;   // uniform too
     HgBody<R> body(a);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = t / LPP, lane = t % LPP;
-    const bool act = i < n;
     const bool two_d = (a.F.flags & SPH_TWO_D) != 0;
     const size_t cstride = (size_t)nl.nmax * nl.npad;
-    bool listed = false;
+    bool listed = false, act = false;
+    int cnt = 0;
+    if (i < n) {
+        cnt = nl.cnt[i];
+        listed = cnt <= nl.nmax;
+        act = !(listed && a.hg_mode == 3);
+    }
     if (act) {
         double xi, yi, zi;
         body.begin(i, xi, yi, zi);
-        const int cnt = nl.cnt[i];
-        listed = cnt <= nl.nmax;
         if (listed) {
             for (int k = lane; k < cnt; k += LPP) {
                 const size_t slot = (size_t)k * nl.npad + i;
                 const int e = nl.idx[slot];
                 const int j = e & 0x3fffffff, mir = (e >> 30) & 3;
-                if (a.hg_mode == 2) {
-                    body.pair_fast(j, mir, slot, cstride, two_d);
-                } else {
-                    const double4 xj = a.X[j];
-                    double dx = min_image(g, 0, xi - xj.x);
-                    double dy = (mir & 1) ? -(-yi - xj.y) : min_image(g, 1, yi - xj.y);
-                    double dz = (mir & 2) ? -(-zi - xj.z) : min_image(g, 2, zi - xj.z);
-                    body.pair_store(j, mir, dx, dy, dz, slot, cstride);
-                }
+                const double4 xj = a.X[j];
+                double dx = min_image(g, 0, xi - xj.x);
+                double dy = (mir & 1) ? -(-yi - xj.y) : min_image(g, 1, yi - xj.y);
+                double dz = (mir & 2) ? -(-zi - xj.z) : min_image(g, 2, zi - xj.z);
+                body.pair_store(j, mir, dx, dy, dz, slot, cstride);
             }
         } else if (lane == 0) {
             auto f = [&](int j, int mir, double rx, double ry, double rz) { body.pair(j, mir, rx, ry, rz); };
@@ -1643,6 +1657,37 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_hg_list(DGrid g, int n, SolidArgs
     body.template lane_reduce<LPP>();
     if (act && lane == 0) {
         body.source(i, listed);
+        body.end(i);
+    }
+}
+
+// A later hourglass sub-step (mode 2) of the listed particles: per slot the stored u and
+// y_j, y_dst = y + sub k (sum_j u (u . (M y_j - y_i)) + source).  The overflowed
+// particles are k_hg_list's, in mode 3, launched after this.
+template <typename R, int LPP>
+__global__ __launch_bounds__(SPH_BLOCK) void k_hg_apply(int n, SolidArgs<R> a, NList nl) {
+    if (a.skip()) return;   // uniform over the grid
+    HgBody<R> body(a);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / LPP, lane = t % LPP;
+    const bool two_d = (a.F.flags & SPH_TWO_D) != 0;
+    const size_t cstride = (size_t)nl.nmax * nl.npad;
+    bool act = false;
+    if (i < n) {
+        const int cnt = nl.cnt[i];
+        act = cnt <= nl.nmax;
+        if (act) {
+            body.begin_fast(i);
+            for (int k = lane; k < cnt; k += LPP) {
+                const size_t slot = (size_t)k * nl.npad + i;
+                const int e = nl.idx[slot];
+                body.pair_fast(e & 0x3fffffff, (e >> 30) & 3, slot, cstride, two_d);
+            }
+        }
+    }
+    body.template lane_reduce<LPP>();
+    if (act && lane == 0) {
+        body.source(i, true);
         body.end(i);
     }
 }
@@ -1823,6 +1868,19 @@ static void launch_hg_pass(sph_ctx* c, const SolidArgs<R>& a, hipStream_t hs) {
     const int lpp = c->lpp[PASS_HG];
     const int blocks = (int)(((int64_t)n * lpp + SPH_BLOCK - 1) / SPH_BLOCK);
     dim3 gb(blocks), bb(SPH_BLOCK);
+    if (a.hg_mode == 2) {
+        switch (lpp) {
+        case 8: hipLaunchKernelGGL((k_hg_apply<R, 8>), gb, bb, 0, hs, n, a, c->nl); break;
+        case 4: hipLaunchKernelGGL((k_hg_apply<R, 4>), gb, bb, 0, hs, n, a, c->nl); break;
+        case 2: hipLaunchKernelGGL((k_hg_apply<R, 2>), gb, bb, 0, hs, n, a, c->nl); break;
+        default: hipLaunchKernelGGL((k_hg_apply<R, 1>), gb, bb, 0, hs, n, a, c->nl); break;
+        }
+        SolidArgs<R> b = a;
+        b.hg_mode = 3;
+        hipLaunchKernelGGL((k_hg_list<R, 1>), dim3((n + SPH_BLOCK - 1) / SPH_BLOCK), bb, 0, hs, c->grid, n, b,
+                           c->nl);
+        return;
+    }
     switch (lpp) {
     case 8: hipLaunchKernelGGL((k_hg_list<R, 8>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
     case 4: hipLaunchKernelGGL((k_hg_list<R, 4>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
@@ -1842,7 +1900,7 @@ static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F,
         hipLaunchKernelGGL((k_build_list<SPH_LIST_LANES>), dim3((int)((nt + SPH_BLOCK - 1) / SPH_BLOCK)),
                            dim3(SPH_BLOCK), 0, hs, c->grid, (int)c->dims.n, (const int*)c->s.start,
                            (const int4*)c->s.cellid, (const double4*)st->buf[st->cur].x,
-                           (const double*)st->scal, c->nl);
+                           st->scal, c->nl);
     }
     // pass SA: reads history[hcur], writes history[1 - hcur]
     timer_begin(c, 4, hs);
@@ -1867,7 +1925,9 @@ static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F,
     sd->hcur = 1 - sd->hcur;
     // pass SB on the updated stress, volume and pressure
     timer_begin(c, 5, hs);
-    int slots_b = launch_list_pass<SolidForceBody<R>>(c, solid_args<R>(c, st, sd, F, S), hs, PASS_SFORCE);
+    int slots_b = S.hg_explicit
+                      ? launch_list_pass<SolidForceBody<R, true>>(c, solid_args<R>(c, st, sd, F, S), hs, PASS_SFORCE)
+                      : launch_list_pass<SolidForceBody<R, false>>(c, solid_args<R>(c, st, sd, F, S), hs, PASS_SFORCE);
     timer_end(c, 5, hs);
     hipLaunchKernelGGL(k_dt_solid, dim3(1), dim3(SPH_BLOCK), 0, hs, F, S, st->scal, c->s, slots_a, slots_b,
                        slots_s, c->pstride);
@@ -1969,7 +2029,7 @@ extern "C" int sph_solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const sp
         c->nl.npad = npad;
     }
     if (S->sts && S->hourglass > 0.0 && c->hg_pairs && !c->hg_pair) {
-        const int ncomp = (c->dims.flags & SPH_TWO_D) ? 3 : 4;
+        const int ncomp = (c->dims.flags & SPH_TWO_D) ? 2 : 3;
         const size_t rsz = c->dims.precision == SPH_PREC_MIXED ? sizeof(float) : sizeof(double);
         HIPCHECK(c, hipMalloc(&c->hg_pair, rsz * ncomp * (size_t)c->nl.nmax * c->nl.npad));
         HIPCHECK(c, hipMalloc((void**)&c->hg_src, sizeof(double4) * (size_t)c->nl.npad));

@@ -356,6 +356,14 @@ class SimConfig:
             raise SceneError("%s: %d problem%s in the scene:\n  - %s" % (
                 os.path.basename(str(self.path)), len(errors), "" if len(errors) == 1 else "s",
                 "\n  - ".join(errors)))
+        if raw.get("ParticlesFile"):
+            # a path relative to the working directory (config_builder.py:255-257); one that
+            # is not there but sits beside the scene file is taken from there
+            pf = str(raw["ParticlesFile"])
+            if (not os.path.isabs(pf) and not os.path.exists(pf) and self.path
+                    and os.path.exists(os.path.join(os.path.dirname(str(self.path)), pf))):
+                pf = os.path.join(os.path.dirname(str(self.path)), pf)
+            resolved["ParticlesFile"] = pf
         self.config = {"Configuration": resolved}
         if verbose:
             print(self.summary())

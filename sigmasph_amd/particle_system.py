@@ -136,14 +136,16 @@ class SceneLayout:
         # ---- lay the particles (host, float64) ------------------------------------
         blocks = ([(b, self.material_fluid) for b in self.cfg.get_fluid_blocks()] +
                   [(b, self.material_solid) for b in self.cfg.get_solid_blocks()])
-        if c("ParticlesFile"):
-            raise NotImplementedError("ParticlesFile input is not part of this build")
         self.materials = self.cfg.get_materials() if self.is_solid else {}
         if self.is_solid and not self.materials:
             raise SystemExit("a hypoElastic deck needs Materials[]")
         self.default_material = (max(self.materials.values(), key=lambda m: m.c_p)
                                  if self.materials else None)
         xs, vs, rhos, objs, mids = [], [], [], [], []
+        if c("ParticlesFile"):
+            # the file replaces the blocks: its particles, grouped by object id
+            blocks = []
+            self._load_particles_file(c("ParticlesFile"), xs, vs, rhos, objs, mids)
         for blk, _mat in blocks:
             x, v, rho = self._lay_block(blk)
             if len(x) == 0:
@@ -184,6 +186,38 @@ class SceneLayout:
         self._init_arrays(x, v, rho, obj)
         if self.is_solid:
             self._init_solid(x, np.concatenate(mids))
+
+    def _load_particles_file(self, path, xs, vs, rhos, objs, mids):
+        """_load_particles_from_file (particle_system.py:731-854): the particles of a
+        ParticlesFile grouped by object id in ascending order, each group in file order,
+        with position, velocity and density rounded through float32 as the reference
+        loads them; a solid object takes the material of the SolidBlock with its
+        objectId, else the deck's default.  A file with per-particle sizes (`area` /
+        `volume`, which switch on variable h) is refused: one h for every particle here."""
+        from .netcdf_writer import read_particles_file
+        data = read_particles_file(path, self.two_d)
+        if "size" in data:
+            raise SystemExit("ParticlesFile '%s' carries per-particle sizes (variable h), which this "
+                             "build does not run: every particle has the one h of particleRadius" % path)
+        solid_mat = {int(b["objectId"]): b.get("material") for b in self.cfg.get_solid_blocks()}
+        obj = data["object_id"]
+        for oid in sorted(np.unique(obj).tolist()):
+            sel = obj == oid
+            x = data["position"][sel].astype(np.float32).astype(np.float64)
+            xs.append(x)
+            vs.append(data["velocity"][sel].astype(np.float32).astype(np.float64))
+            rhos.append(data["density"][sel].astype(np.float32).astype(np.float64))
+            objs.append(np.full(len(x), oid, dtype=np.int32))
+            entry = {"objectId": oid, "particleNum": len(x)}
+            if oid in solid_mat and solid_mat[oid] is not None:
+                entry["material"] = solid_mat[oid]
+            self.object_collection[oid] = entry
+            if self.is_solid:
+                name = entry.get("material")
+                if name is not None and name not in self.materials:
+                    raise SystemExit("object %d names unknown material %r" % (oid, name))
+                mat = self.materials[name] if name is not None else self.default_material
+                mids.append(np.full(len(x), mat.index, dtype=np.int32))
 
     def _init_solid(self, x, eos_id):
         """The solid's initial per-particle state: material row, e = e0 and T = T0 of

@@ -8,14 +8,15 @@ registry, `ParticleSystem` / `build_solver()` / `initialize()`, the render-cycle
 of `Time.stepsPerRenderUpdate` solver steps that sets `solver.dt[None]` before each
 step (clipping the last step to `Time.duration`) and follows the adaptive dt after it,
 `Time.maxSteps` as a cap on render cycles, the `ConstraintRecorder` CSVs, the
-extended-XYZ dumps of `Output.txt` every `Output.interval` cycles from cycle 0, the
+extended-XYZ dumps of `Output.txt` and the AMBER-NetCDF trajectory `<scene>.nc` of
+`Output.netcdf` every `Output.interval` cycles from cycle 0 (one dump feeding both), the
 status line once per cycle (KE / IE / SE / HE / PE / E, the path terms VW / WW / LW
 with the residual R, WD, NE, the drift against E0, the off-grid count), the NaN abort,
 and the closing wall-clock line, energy table, momentum table and budget terms.
 
 What it refuses, with a message: the window (run with --no-gui), `Output.frame`
-(screenshots of that window), PLY / OBJ / NetCDF output and `--hg-probe`, none of
-which this build provides.
+(screenshots of that window), PLY / OBJ output and `--hg-probe`, none of which this
+build provides.
 
 `main()` returns a dict (steps, render cycles, simulated time, initial and final
 budgets and momenta, the stop reason) so that a test can read what the run printed.
@@ -198,12 +199,13 @@ def main(argv=None, output_dir=None, quiet=False):
     for key, what in (("exportFrame", "Output.frame writes PNG screenshots of the window and cannot "
                                       "be used with --no-gui"),
                       ("exportPly", "Output.ply is not part of this build"),
-                      ("exportObj", "Output.obj is not part of this build"),
-                      ("exportNetcdf", "Output.netcdf is not part of this build (no netCDF4)")):
+                      ("exportObj", "Output.obj is not part of this build")):
         if c(key):
             sys.exit(what + ".")
     substeps = int(c("numberOfStepsPerRenderUpdate") or 1)
     output_txt = bool(c("exportTxt"))
+    output_netcdf = bool(c("exportNetcdf"))
+    nc_writer = None
     output_interval = int(c("exportInterval") or 1)
     sim_duration = c("sim_duration")
     max_steps = c("maxSteps")
@@ -251,8 +253,17 @@ def main(argv=None, output_dir=None, quiet=False):
                 stop = "reached duration"
                 log("\nSimulation finished at t = %.6f s" % sim_time)
                 break
-        if output_txt and cnt % output_interval == 0:
-            write_xyz(os.path.join(odir, "dump_%04d.xyz" % cnt), ps, ps.dump(), cnt)
+        if (output_txt or output_netcdf) and cnt % output_interval == 0:
+            obj_data = ps.dump()
+            if output_txt:
+                write_xyz(os.path.join(odir, "dump_%04d.xyz" % cnt), ps, obj_data, cnt)
+            if output_netcdf:
+                # created on the first frame, then appended to and flushed every interval
+                if nc_writer is None:
+                    from .netcdf_writer import NetCDFTrajectoryWriter
+                    nc_writer = NetCDFTrajectoryWriter(os.path.join(odir, scene_name + ".nc"), ps.domain_size,
+                                                       ps.is_solid, ps.two_d)
+                nc_writer.write(obj_data, sim_time)
         for _ in range(substeps):
             if sim_duration is not None:
                 left = sim_duration - sim_time
@@ -316,6 +327,9 @@ def main(argv=None, output_dir=None, quiet=False):
             break
     if recorder:
         recorder.close()
+    if nc_writer is not None:
+        nc_writer.close()
+        log("[NETCDF] %d frames written to %s.nc" % (nc_writer.frames_written, scene_name))
     wall = time.perf_counter() - wall_t0
     rate = total_steps / wall if wall > 0 else 0.0
     log("\n[WALL CLOCK] %s (%.2f s) for %d solver steps over %d render cycles -- %.0f steps/s"

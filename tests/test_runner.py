@@ -103,7 +103,7 @@ def test_driver_run_matches_the_oracle(gpu, tmp_path):
     sc = solid_block(n=(24, 8, 1), two_d="axisymmetric", grips=0.05, ramp_time=1e-4,
                      div_velocity=(300.0, 0.0, 0.0))
     sc["Time"].update({"stepsPerRenderUpdate": 2, "maxSteps": 2})
-    sc["Output"] = {"constraints": True, "txt": True, "interval": 1}
+    sc["Output"] = {"constraints": True, "txt": True, "netcdf": True, "interval": 1}
     path = tmp_path / "grips.json"
     path.write_text(json.dumps(sc))
     res = R.main(["--scene_file", str(path), "--no-gui"], output_dir=str(tmp_path), quiet=True)
@@ -131,6 +131,15 @@ def test_driver_run_matches_the_oracle(gpu, tmp_path):
     assert os.path.exists(tmp_path / "dump_0000.xyz") and os.path.exists(tmp_path / "dump_0001.xyz")
     head = open(tmp_path / "dump_0001.xyz").read().splitlines()[:2]
     assert int(head[0]) == lay.dims.n and "von_mises:R:1" in head[1]
+    # the trajectory: one frame per dump, the same dump the XYZ file was written from
+    from scipy.io import netcdf_file
+    xyz = np.loadtxt(tmp_path / "dump_0001.xyz", skiprows=2)
+    with netcdf_file(str(tmp_path / "grips.nc"), "r", mmap=False) as ds:
+        assert ds.variables["coordinates"].shape == (2, lay.dims.n, 3)
+        assert ds.variables["time"][0] == 0.0 and ds.variables["time"][1] > 0.0
+        np.testing.assert_allclose(ds.variables["coordinates"][1], xyz[:, :3], rtol=2e-3, atol=1e-9)
+        np.testing.assert_allclose(ds.variables["von_mises"][1], xyz[:, 11], rtol=2e-3, atol=1.0)
+        assert "sigma_xz" not in ds.variables
 
 
 @pytest.mark.gpu

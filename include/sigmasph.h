@@ -210,7 +210,7 @@ typedef struct {
     int32_t kernel_correction;
     int32_t jaumann;         /* OBJECTIVE_RATE "jaumann" (1) or "none" (0)                  */
     int32_t ale_history;     /* PST_ALE_HISTORY: transport s, eps_p and e with the shift    */
-    int32_t sts;             /* HOURGLASS_STS in subcycle mode                              */
+    int32_t sts;             /* HOURGLASS_STS: 0 off, SPH_STS_SUBCYCLE or SPH_STS_RKL        */
     int32_t sts_stages;      /* fixed sub-step count, 0 = sized from dt                     */
     int32_t sts_max_stages;
     int32_t sts_refresh;     /* recompute grad v after the first half kick                  */
@@ -224,7 +224,14 @@ typedef struct {
                               * beyond that atomic adds                                     */
     int32_t pad_;
     double  sur_tol;         /* the surrogate's acceptance tolerance (fraction of sigma_y)   */
+    double  sts_safety;      /* HOURGLASS_STS_SAFETY of the RKL stage count (SOLID.py:1087)  */
 } sph_solid_params;
+
+/* sph_solid_params.sts: how the s passes of a step are spent (hourglass_sts,
+ * SOLID.py:4541-4576): s forward-Euler sub-steps of dt/s, or the s-stage first-order
+ * Runge-Kutta-Legendre recursion Y_j = mu_j Y_j-1 + nu_j Y_j-2 + mu_j w1 dt M Y_j-1. */
+#define SPH_STS_SUBCYCLE 1
+#define SPH_STS_RKL      2
 
 typedef struct sph_mlp sph_mlp;   /* the surrogate's network, defined below */
 

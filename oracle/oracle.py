@@ -200,6 +200,16 @@ def solid_hg_accel(dims, params, sparams, s, y):
     return a
 
 
+def solid_hourglass_sts(dims, params, sparams, s, y, stages, dt):
+    """hourglass_sts on a copy of y: `stages` sub-steps or RKL1 stages (sparams.sts)."""
+    y = np.array(y, dtype=np.float64).reshape(s.n, 3)
+    st = s.c()
+    lib().oracle_solid_hourglass_sts(ctypes.byref(dims), ctypes.byref(params), ctypes.byref(sparams),
+                                     ctypes.byref(st), ctypes.c_int(int(stages)), ctypes.c_double(dt),
+                                     _ptr(y))
+    return y
+
+
 def solid_reduce(s, which):
     st = s.c()
     return lib().oracle_solid_reduce(ctypes.byref(st), int(which))

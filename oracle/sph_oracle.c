@@ -1615,15 +1615,60 @@ static double or_kinetic(const or_state* s) {
     return ke;
 }
 
-/* sts_stage_count (SOLID.py:4510-4539), subcycle mode */
+/* sts_stage_count (SOLID.py:4510-4539): subcycle, s = ceil(rate dt / 1.6) capped, or
+ * RKL, the smallest s with (s^2 + s) safety >= rate dt; dt is clamped to what s carries */
 static int or_sts_stage_count(const sph_solid_params* S, or_state* s) {
     if (S->hg_rate <= 0.0) return 1;
     double want = S->hg_rate * s->scal[SPH_S_DT];
-    int st = S->sts_stages > 0 ? S->sts_stages : (int)fmax(1.0, ceil(want / 1.6));
-    if (st > S->sts_max_stages) st = S->sts_max_stages;
-    double cap = st * 1.6 / S->hg_rate;
+    int st;
+    double cap;
+    if (S->sts != SPH_STS_RKL) {
+        st = S->sts_stages > 0 ? S->sts_stages : (int)fmax(1.0, ceil(want / 1.6));
+        if (st > S->sts_max_stages) st = S->sts_max_stages;
+        cap = st * 1.6 / S->hg_rate;
+    } else {
+        if (S->sts_stages > 0) {
+            st = S->sts_stages;
+        } else {
+            st = 1;
+            while ((st * st + st) * S->sts_safety < want && st < S->sts_max_stages) st += 1;
+        }
+        cap = S->sts_safety * (st * st + st) / S->hg_rate;
+    }
     if (s->scal[SPH_S_DT] > cap) s->scal[SPH_S_DT] = cap;
     return st;
+}
+
+/* hourglass_sts (SOLID.py:4541-4576) on y in place: st forward-Euler sub-steps of dt / st,
+ * or the st-stage RKL1 recursion Y_j = mu Y_j-1 + nu Y_j-2 + mu w1 dt M Y_j-1 with
+ * (mu, nu) = (1, 0) at j = 1 and ((2j - 1)/j, (1 - j)/j) after, w1 = 2 / (st^2 + st),
+ * from Y_0 = Y_1 = y (sts_load, 4485-4490) */
+static void or_hourglass_sts(const sph_dims* d, const sph_fluid_params* P, const sph_solid_params* S,
+                             or_state* s, const or_grid* g, int st, double dt, double* y) {
+    int64_t n = s->n;
+    double* ya = (double*)malloc(sizeof(double) * 3 * n);
+    double* a = (double*)malloc(sizeof(double) * 3 * n);
+    memcpy(ya, y, sizeof(double) * 3 * n);
+    if (S->sts != SPH_STS_RKL) {
+        double sub = dt / st;
+        for (int k = 0; k < st; ++k) {
+            or_hg_accel(d, P, S, s, g, y, a);
+            for (int64_t i = 0; i < 3 * n; ++i) y[i] = 1.0 * y[i] + 0.0 * y[i] + sub * a[i];
+        }
+    } else {
+        double w1 = 2.0 / (double)(st * st + st);
+        for (int j = 1; j <= st; ++j) {
+            double mu = 1.0, nu = 0.0;
+            if (j > 1) { mu = (2.0 * j - 1.0) / j; nu = (1.0 - j) / j; }
+            or_hg_accel(d, P, S, s, g, y, a);
+            for (int64_t i = 0; i < 3 * n; ++i) {
+                double yn = mu * y[i] + nu * ya[i] + mu * w1 * dt * a[i];
+                ya[i] = y[i];
+                y[i] = yn;
+            }
+        }
+    }
+    free(ya); free(a);
 }
 
 /* advect_velocity, hourglass_sts, advect_velocity_finish, advect_position
@@ -1634,7 +1679,6 @@ static void or_sts_advect(const sph_dims* d, const sph_fluid_params* P, const sp
     double dt = s->scal[SPH_S_DT];
     double* vs = (double*)malloc(sizeof(double) * 3 * n);
     double* y = (double*)malloc(sizeof(double) * 3 * n);
-    double* a = (double*)malloc(sizeof(double) * 3 * n);
     for (int64_t i = 0; i < n; ++i)
         for (int k = 0; k < 3; ++k) {
             s->v[3 * i + k] += 0.5 * dt * s->acc[3 * i + k];
@@ -1643,11 +1687,7 @@ static void or_sts_advect(const sph_dims* d, const sph_fluid_params* P, const sp
     if (S->sts_refresh) or_velocity_gradient(d, P, S, s, g);
     double ke0 = or_kinetic(s);
     memcpy(y, s->v, sizeof(double) * 3 * n);
-    double sub = dt / st;
-    for (int k = 0; k < st; ++k) {
-        or_hg_accel(d, P, S, s, g, y, a);
-        for (int64_t i = 0; i < 3 * n; ++i) y[i] = 1.0 * y[i] + 0.0 * y[i] + sub * a[i];
-    }
+    or_hourglass_sts(d, P, S, s, g, st, dt, y);
     memcpy(s->v, y, sizeof(double) * 3 * n);
     s->scal[SPH_S_HG_WORK] += ke0 - or_kinetic(s);
     double h = 0.5 * dt, w = 0.0;
@@ -1663,7 +1703,7 @@ static void or_sts_advect(const sph_dims* d, const sph_fluid_params* P, const sp
         double dr[3] = { dt * s->v[3 * i], dt * s->v[3 * i + 1], dt * s->v[3 * i + 2] };
         or_displace(d, P, s, i, dr);
     }
-    free(vs); free(y); free(a);
+    free(vs); free(y);
 }
 
 /* clear_constraint_forces + apply_constraints (SOLID.py:5019-5055) */
@@ -1778,6 +1818,10 @@ void oracle_solid_stress_accelerations(const sph_dims* d, const sph_fluid_params
 void oracle_solid_hg_accel(const sph_dims* d, const sph_fluid_params* P, const sph_solid_params* S,
                            or_state* s, const double* y, double* a) {
     or_hg_accel(d, P, S, s, &g_grid, y, a);
+}
+void oracle_solid_hourglass_sts(const sph_dims* d, const sph_fluid_params* P, const sph_solid_params* S,
+                                or_state* s, int st, double dt, double* y) {
+    or_hourglass_sts(d, P, S, s, &g_grid, st, dt, y);
 }
 void oracle_solid_update_stress(const sph_dims* d, const sph_fluid_params* P,
                                 const sph_solid_params* S, or_state* s) {

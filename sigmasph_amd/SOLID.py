@@ -10,13 +10,12 @@ Tait and polynomial Mie-Grueneisen equations of state, J2 (linear hardening) and
 Johnson-Cook return maps, the Jaumann rate, ALE transport of the deviator, the plastic
 strain and the internal energy under the shift (with the upwind energy diffusion),
 Morris viscosity, the VNR bulk viscosity and Monaghan's pair viscosity with their dt
-limits, the rate hourglass damper (explicit with its dt cap, or super-time-stepped in
-subcycle mode), the load ramp and displacement grips, axisymmetry and plane strain.
+limits, the rate hourglass damper (explicit with its dt cap, or super-time-stepped:
+subcycle or RKL1), the load ramp and displacement grips, axisymmetry and plane strain.
 Everything else the reference offers on this class raises NotImplementedError when a
 scene asks for it: JWL / HJC, damage and erosion, the conservative axisymmetric forms,
-the viscous / adaptive / saturating hourglass forms, RKL super-time-stepping, the
-velocity limiters, and a table whose materials do not share one EOS sound speed and
-one acoustic impedance.
+the viscous / adaptive / saturating hourglass forms, the velocity limiters, and a table
+whose materials do not share one EOS sound speed and one acoustic impedance.
 """
 import ctypes
 import math
@@ -79,8 +78,6 @@ class SolidSettings:
                 raise NotImplementedError("%s is not part of this build" % what)
         if str(c("HOURGLASS_FORM") or "rate").lower() != "rate":
             raise NotImplementedError("only the rate hourglass form is built")
-        if c("HOURGLASS_STS") and str(c("HOURGLASS_STS_MODE") or "subcycle").lower() != "subcycle":
-            raise NotImplementedError("RKL super-time-stepping is not built (subcycle is)")
         if str(c("DELTA_PAIRS") or "samematerial").lower() != "samematerial":
             raise NotImplementedError("Diffusion.pairs other than sameMaterial is not built")
         self.fs = FluidSettings(ps)
@@ -97,6 +94,12 @@ class SolidSettings:
         self.hg_sts = bool(c("HOURGLASS_STS"))
         self.hg_sts_stages = int(c("HOURGLASS_STS_STAGES") or 0)
         self.hg_sts_max_stages = 32
+        self.hg_sts_mode = str(c("HOURGLASS_STS_MODE") or "subcycle").lower()
+        if self.hg_sts_mode not in ("subcycle", "rkl"):
+            raise ValueError("Hourglass.superTimeStepping.mode is 'subcycle' or 'rkl', got %r"
+                             % (self.hg_sts_mode,))
+        sf = c("HOURGLASS_STS_SAFETY")
+        self.hg_sts_safety = 0.8 if sf is None else float(sf)
         rf = c("HOURGLASS_STS_REFRESH")
         self.hg_sts_refresh = True if rf is None else bool(rf)
         self.hg_neighbour_sum, self.w_dx0 = reference_kernel_sum(ps.two_d, ps.support_radius,
@@ -161,7 +164,9 @@ class SolidSettings:
         S.kernel_correction = int(self.kernel_correction)
         S.jaumann = int(self.jaumann)
         S.ale_history = int(self.pst_ale_history)
-        S.sts = int(self.hg_sts)
+        S.sts = 0 if not self.hg_sts else (N.SPH_STS_RKL if self.hg_sts_mode == "rkl"
+                                           else N.SPH_STS_SUBCYCLE)
+        S.sts_safety = self.hg_sts_safety
         S.sts_stages, S.sts_max_stages = self.hg_sts_stages, self.hg_sts_max_stages
         S.sts_refresh = int(self.hg_sts_refresh)
         S.dt_pair_mode = self.dt_pair_mode

@@ -48,6 +48,8 @@ SPH_S_IE_ALE_DIFF = 21
 SPH_S_SUR_YIELDING = 26
 SPH_S_SUR_FALLBACK = 27
 SPH_NSCAL = 32
+SPH_STS_SUBCYCLE = 1
+SPH_STS_RKL = 2
 # reductions
 SPH_R_KINETIC = 0
 SPH_R_POTENTIAL = 1
@@ -159,7 +161,7 @@ class SphSolidParams(ctypes.Structure):
         "l_eig_tol", "q_l2", "q_cl", "av_alpha", "av_beta", "av_eps")] + [(n, ctypes.c_int32) for n in (
         "kernel_correction", "jaumann", "ale_history", "sts", "sts_stages", "sts_max_stages",
         "sts_refresh", "dt_pair_mode", "visc_work", "thermal", "plastic", "nmat", "ncons",
-        "pad_")] + [("sur_tol", ctypes.c_double)]
+        "pad_")] + [("sur_tol", ctypes.c_double), ("sts_safety", ctypes.c_double)]
 
 
 _vp2 = ctypes.c_void_p * 2

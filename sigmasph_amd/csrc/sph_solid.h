@@ -36,6 +36,8 @@ struct DSolidP {
     double hg_k;     // alpha / h: the damper's prefactor before rho0/rho c_p w_j/V0 W/W0
     int kc, jaumann, transport, transport_ep, sts, sts_stages, sts_max, sts_refresh;
     int sts_bound;   // the most sub-steps any dt of the step can ask for (dt <= CFL h / c_signal)
+    int sts_rkl;     // the stages are RKL1's recursion, not plain sub-steps
+    double sts_safety;
     int dt_pair_mode, visc_work, thermal, plastic, ncons, hg_explicit;
     double q_l2, q_cl;                   // VNR: C_Q l^2, C_Q l
     double av_alpha, av_beta, av_epsh2;  // Monaghan: alpha, beta, eps h^2
@@ -1004,7 +1006,8 @@ template <typename R> struct GradVBody {
 
 // ------------------------------------------------------------------------------------
 // one forward-Euler hourglass sub-step on the stage velocities (_hg_rate_accel,
-// SOLID.py:4260-4300; sts_stage at (mu, nu) = (1, 0), 4492-4503)
+// SOLID.py:4260-4300; sts_stage, 4492-4503: subcycle at (mu, nu) = (1, 0), or the RKL1
+// stage j of hourglass_sts, 4563-4576)
 // ------------------------------------------------------------------------------------
 template <typename R> struct HgBody {
     typedef SolidArgs<R> Args;
@@ -1133,11 +1136,29 @@ template <typename R> struct HgBody {
         const sph_material& M = a.sd.mat[a.sd.eos[i]];
         double rho_i = (double)b.m[i] / (double)b.V[i];
         double k = S.alpha_hg * (M.rho0 / rho_i) * (M.c_p * F.inv_h) * S.inv_V0 * S.inv_wdx0;
-        const double sub = a.st.scal[SPH_S_DT] / a.st.scal[SPH_S_STS_STAGES];
+        const double dt = a.st.scal[SPH_S_DT], s = a.st.scal[SPH_S_STS_STAGES];
         double4 y = yi;
-        y.x = 1.0 * y.x + sub * (k * (double)hx);
-        y.y = 1.0 * y.y + sub * (k * (double)hy);
-        y.z = 1.0 * y.z + sub * (k * (double)hz);
+        if (!S.sts_rkl) {
+            const double sub = dt / s;
+            y.x = 1.0 * y.x + sub * (k * (double)hx);
+            y.y = 1.0 * y.y + sub * (k * (double)hy);
+            y.z = 1.0 * y.z + sub * (k * (double)hz);
+        } else if (a.stage == 0) {
+            // stage 1 at (mu, nu) = (1, 0): Y_1 = Y_0 + w1 dt M Y_0
+            const double mw = 2.0 / (s * s + s) * dt;
+            y.x = y.x + mw * (k * (double)hx);
+            y.y = y.y + mw * (k * (double)hy);
+            y.z = y.z + mw * (k * (double)hz);
+        } else {
+            // stage j: ydst still holds Y_(j-2) (the two buffers alternate)
+            const double j = (double)(a.stage + 1);
+            const double mu = (2.0 * j - 1.0) / j, nu = (1.0 - j) / j;
+            const double mw = mu * (2.0 / (s * s + s)) * dt;
+            const double4 yp = a.ydst[i];
+            y.x = mu * y.x + nu * yp.x + mw * (k * (double)hx);
+            y.y = mu * y.y + nu * yp.y + mw * (k * (double)hy);
+            y.z = mu * y.z + nu * yp.z + mw * (k * (double)hz);
+        }
         a.ydst[i] = y;
     }
     __device__ void reduce(double*, int) {}
@@ -1283,10 +1304,23 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_dt_solid(DFluid F, DSolidP S, dou
             st = 1;
             if (S.hg_rate > 0.0) {
                 double want = S.hg_rate * dt;
-                st = S.sts_stages > 0 ? S.sts_stages : (int)fmax(1.0, ceil(want / 1.6));
-                st = min(st, S.sts_max);
-                st = min(st, S.sts_bound);   // never binds: dt <= CFL h / c_signal
-                double cap = st * 1.6 / S.hg_rate;
+                double cap;
+                if (!S.sts_rkl) {
+                    st = S.sts_stages > 0 ? S.sts_stages : (int)fmax(1.0, ceil(want / 1.6));
+                    st = min(st, S.sts_max);
+                    st = min(st, S.sts_bound);   // never binds: dt <= CFL h / c_signal
+                    cap = st * 1.6 / S.hg_rate;
+                } else {
+                    // the smallest s whose RKL1 interval (s^2 + s) safety covers rate dt
+                    if (S.sts_stages > 0) {
+                        st = S.sts_stages;
+                    } else {
+                        st = 1;
+                        while ((double)(st * st + st) * S.sts_safety < want && st < S.sts_max) ++st;
+                    }
+                    st = min(st, S.sts_bound);   // never binds, as above
+                    cap = S.sts_safety * (double)(st * st + st) / S.hg_rate;
+                }
                 if (dt > cap) dt = cap;
             }
         }
@@ -1724,9 +1758,17 @@ static DSolidP make_solid_params(const sph_ctx* c, const sph_fluid_params* P, co
             double csig2 = P->c_signal * P->c_signal;
             double dt_max = P->cfl * c->dims.h / sqrt(csig2);
             double want = S->hg_rate * dt_max;
-            d.sts_bound = std::min(S->sts_max_stages, (int)fmax(1.0, ceil(want / 1.6)));
+            if (S->sts == SPH_STS_RKL) {
+                int st = 1;
+                while ((double)(st * st + st) * S->sts_safety < want && st < S->sts_max_stages) ++st;
+                d.sts_bound = st;
+            } else {
+                d.sts_bound = std::min(S->sts_max_stages, (int)fmax(1.0, ceil(want / 1.6)));
+            }
         }
     }
+    d.sts_rkl = d.sts && S->sts == SPH_STS_RKL;
+    d.sts_safety = S->sts_safety;
     d.sts_refresh = S->sts_refresh;
     d.dt_pair_mode = S->dt_pair_mode;
     d.visc_work = S->visc_work;

@@ -39,6 +39,11 @@ SCENES = {
     "block3d_explicit": lambda: solid_block(n=(8, 7, 6), two_d=None, sts=False, jc=False,
                                             yield_stress=0.05, kernel_correction=True,
                                             rot_velocity=400.0, jitter=0.05),
+    # the damper's sub-steps spent on the RKL1 recursion (3 stages where subcycle takes 5)
+    "plane_strain_rkl": lambda: solid_block(n=(24, 8, 1), grips=0.05, ramp_time=1e-4, hourglass=8.0,
+                                            div_velocity=(300.0, -150.0, 0.0), jitter=0.02,
+                                            extra={"Hourglass": {"alpha": 8.0,
+                                                                 "superTimeStepping": {"mode": "rkl"}}}),
     "plane_strain_noshift": lambda: solid_block(n=(20, 10, 1), pst=False, viscosity=0.0,
                                                 div_velocity=(-200.0, 250.0, 0.0), jitter=0.03),
     # nine specimens with their own Johnson-Cook rows and 18 grips (past 16 the grip
@@ -148,10 +153,12 @@ def test_five_steps_fp64(gpu, name):
         assert s.scal[N.SPH_S_PLASTIC_WORK] > 0.0      # the return map ran
     if st.hg_sts:
         assert s.scal[N.SPH_S_STS_STAGES] >= 1
+    if name == "plane_strain_rkl":
+        assert st.solid_params().sts == N.SPH_STS_RKL and s.scal[N.SPH_S_STS_STAGES] >= 2
 
 
 @pytest.mark.parametrize("name", ["plane_strain_grips", "axisymmetric_grips", "block3d_explicit",
-                                  "axisymmetric_shock", "taylor3d"])
+                                  "axisymmetric_shock", "taylor3d", "plane_strain_rkl"])
 def test_three_steps_mixed(gpu, name):
     ps, solver, st, s = run_pair(SCENES[name](), 3, precision="mixed")
     compare(ps, st, s, 2e-4, eps=2.0 ** -24)

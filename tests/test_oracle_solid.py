@@ -263,6 +263,66 @@ def test_hourglass_operator():
     assert np.sum(s.m * np.einsum("ij,ij->i", y, a)) < 0.0  # it only removes energy
 
 
+@pytest.mark.parametrize("mode", ["subcycle", "rkl"])
+def test_sts_stages_are_the_stability_polynomial(mode):
+    """hourglass_sts on a linear damper (grad v = 0, so the damper is y -> A y): s
+    sub-steps give (I + dt A / s)^s y, and s RKL1 stages give P_s(I + w1 dt A) y with
+    P_s the Legendre polynomial of degree s and w1 = 2 / (s^2 + s) (its power-series
+    coefficients from numpy.polynomial.legendre, evaluated by Horner, not by the stage
+    recursion), at a dt that reaches 90 % of each scheme's stability interval."""
+    from numpy.polynomial import legendre
+    lay, st, P, S, s = _sorted_solid(solid_block(n=(7, 5, 1), jitter=0.1, pst=False))
+    S.sts = N.SPH_STS_RKL if mode == "rkl" else N.SPH_STS_SUBCYCLE
+    s.grad_v[:] = 0.0
+    n = s.n
+    A = np.zeros((2 * n, 2 * n))
+    for c in range(2 * n):
+        y = np.zeros((n, 3))
+        y[c // 2, c % 2] = 1.0
+        A[:, c] = O.solid_hg_accel(lay.dims, P, S, s, y)[:, :2].reshape(-1)
+    lam_min = np.linalg.eigvals(A).real.min()
+    assert lam_min < 0.0
+    stages = 4
+    I = np.eye(2 * n)
+    rng = np.random.default_rng(12)
+    y0 = rng.normal(size=(n, 3))
+    y0[:, 2] = 0.0
+    if mode == "rkl":
+        w1 = 2.0 / (stages * stages + stages)
+        dt = 0.9 * 2.0 / (w1 * abs(lam_min))        # |1 + w1 z| <= 1 on [-2/w1, 0]
+        X = I + w1 * dt * A
+        coef = legendre.leg2poly([0.0] * stages + [1.0])
+        R = np.zeros_like(A)
+        for c in coef[::-1]:
+            R = R @ X + c * I
+    else:
+        dt = 0.9 * 2.0 * stages / abs(lam_min)
+        R = np.linalg.matrix_power(I + dt / stages * A, stages)
+    want = R @ y0[:, :2].reshape(-1)
+    got = O.solid_hourglass_sts(lay.dims, P, S, s, y0, stages, dt)
+    np.testing.assert_allclose(got[:, :2].reshape(-1), want, rtol=0, atol=1e-10 * np.abs(y0).max())
+    assert np.all(got[:, 2] == 0.0)
+    assert np.abs(got).max() <= 1.01 * np.abs(y0).max() * np.sqrt(2 * n)   # stable at 90 %
+
+
+def test_rkl_stage_count():
+    """RKL's count: the smallest s with (s^2 + s) safety >= rate dt (sts_stage_count,
+    SOLID.py:4527-4536), fewer stages than subcycle's ceil(rate dt / 1.6) for the same deck."""
+    counts = {}
+    for mode in ("subcycle", "rkl"):
+        sc = solid_block(n=(24, 6, 1), hourglass=8.0,
+                         extra={"Hourglass": {"alpha": 8.0, "superTimeStepping": {"mode": mode}}})
+        lay, st, P, S = solid_layout(sc)
+        s = solid_oracle_state(lay, st)
+        O.solid_step(lay.dims, P, S, s)
+        k = int(s.scal[N.SPH_S_STS_STAGES])
+        want = st.hg_rate * s.scal[0]
+        if mode == "rkl":
+            assert S.sts == N.SPH_STS_RKL and (k * k + k) * 0.8 >= want * (1 - 1e-12)
+        counts[mode] = k
+    assert counts["subcycle"] > counts["rkl"] >= 2
+
+
 def test_sts_stage_count_and_grips_follow_the_ramp():
     sc = solid_block(n=(24, 6, 1), grips=0.05, ramp_time=2e-4)
     lay, st, P, S = solid_layout(sc)

@@ -61,9 +61,11 @@ def test_linear_plasticity_and_library_merge():
 
 def test_unsupported_solid_features_raise():
     sc = solid_block()
-    sc["Hourglass"]["superTimeStepping"]["mode"] = "rkl"
-    with pytest.raises(NotImplementedError):
-        SolidSettings(SceneLayout(SimConfig(sc)))
+    sc["Hourglass"]["superTimeStepping"]["mode"] = "rkl"          # built: RKL1 stages
+    S = SolidSettings(SceneLayout(SimConfig(sc))).solid_params()
+    assert S.sts == N.SPH_STS_RKL and S.sts_safety == 0.8
+    sc["Hourglass"]["superTimeStepping"]["safety"] = 0.5
+    assert SolidSettings(SceneLayout(SimConfig(sc))).solid_params().sts_safety == 0.5
     sc = solid_block(extra={"Domain": {"conservativeAxialMomentum": True}})
     with pytest.raises(NotImplementedError):
         SolidSettings(SceneLayout(SimConfig(sc)))

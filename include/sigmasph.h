@@ -387,10 +387,13 @@ int  sph_graphs(sph_ctx* ctx, int32_t enable);
  * of pair pass k (0 volume, 1 force, 2 shift, 3 pair count: cell-tiled, at most the lanes
  * their tiles were sized for; 4 stress, 5 stress divergence, 6 grad v, 7 hourglass
  * stage: neighbour-list passes, any); SPH_TUNE_HG_PAIRS (0/1) switches the stored
- * hourglass operator.  Results do not depend on the knobs beyond summation order.
- * Setting a knob frees captured graphs. */
+ * hourglass operator; SPH_TUNE_LIST_CAP caps the neighbour-list entries a particle may
+ * use (0 = the full list): particles with more neighbours take the traversal fallback,
+ * which this knob lets a test exercise.  Results do not depend on the knobs beyond
+ * summation order.  Setting a knob frees captured graphs. */
 #define SPH_TUNE_LANES     0
 #define SPH_TUNE_HG_PAIRS  8
+#define SPH_TUNE_LIST_CAP  9
 int  sph_tune(sph_ctx* ctx, int32_t knob, int32_t value);
 int  sph_tune_get(sph_ctx* ctx, int32_t knob, int32_t* value);
 

@@ -1282,6 +1282,8 @@ struct sph_ctx {
     void* hg_pair = nullptr;       // [ncomp][nmax][npad] of R: u = sqrt(w_j W_ij) e
     double4* hg_src = nullptr;     // [npad] sum_j w_j W_ij (e . <G> r) e
     bool hg_pairs = true;          // later stages read them instead of recomputing
+    int list_alloc = 0;            // entries per particle the list was allocated with
+    int list_cap = 0;              // SPH_TUNE_LIST_CAP, 0 = none
     // captured steps (sph_graphs): replayed when the same arguments come back
     bool graphs = false;
     GraphEntry graph[4];
@@ -1529,6 +1531,10 @@ extern "C" int sph_tune(sph_ctx* c, int32_t knob, int32_t value) {
         c->lpp[kind] = value;
     } else if (knob == SPH_TUNE_HG_PAIRS) {
         c->hg_pairs = value != 0;
+    } else if (knob == SPH_TUNE_LIST_CAP) {
+        if (value < 0) return set_err(c, SPH_EINVAL, "list cap must be >= 0%s", "", 0);
+        c->list_cap = value;
+        if (c->nl.idx) c->nl.nmax = value > 0 ? std::min(c->list_alloc, (int)value) : c->list_alloc;
     } else {
         return set_err(c, SPH_EINVAL, "unknown tuning knob%s %lld", "", knob);
     }
@@ -1540,6 +1546,7 @@ extern "C" int sph_tune_get(sph_ctx* c, int32_t knob, int32_t* value) {
     if (!c || !value) return SPH_EINVAL;
     if (knob >= SPH_TUNE_LANES && knob < SPH_TUNE_LANES + PASS_KINDS) *value = c->lpp[knob - SPH_TUNE_LANES];
     else if (knob == SPH_TUNE_HG_PAIRS) *value = c->hg_pairs ? 1 : 0;
+    else if (knob == SPH_TUNE_LIST_CAP) *value = c->list_cap;
     else return set_err(c, SPH_EINVAL, "unknown tuning knob%s %lld", "", knob);
     return SPH_OK;
 }

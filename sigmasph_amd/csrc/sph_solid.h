@@ -72,8 +72,7 @@ template <typename R> struct SolidArgs {
     int pstride;
     // hourglass sub-step pass only
     const double4* ysrc; double4* ydst; int stage;
-    // 0 recompute, 1 recompute and store, 2 read stored, 3 recompute overflowed particles only
-    R* hgA; double4* hgb; int hg_mode;
+    R* hgA; double4* hgb; int hg_mode;   // 0 recompute, 1 recompute and store, 2 read stored
     __device__ bool any_out() const { return st.scal[SPH_S_OUT_OF_BOX] > 0.5; }
     // a sub-step pass beyond this step's count has nothing to do
     __device__ bool skip() const { return ydst != nullptr && stage >= (int)st.scal[SPH_S_STS_STAGES]; }
@@ -1647,28 +1646,22 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_list(DGrid g, int n, typename Bod
     body.reduce(red, blockIdx.x);
 }
 
-// One hourglass sub-step over the list.  Mode 1 (the first stage) evaluates each pair in
-// full and stores its stage-invariant operator (the later stages apply it: k_hg_apply);
-// mode 3 (a later stage) runs only the particles whose list overflowed, and returns at
-// once when none did.  Overflowed particles traverse the cells and evaluate in full at
-// every stage.
+// The first hourglass sub-step over the list (mode 1): evaluates each pair in full and
+// stores its stage-invariant operator, which the later stages apply (k_hg_apply).
+// Overflowed particles traverse the cells and evaluate in full, here and at every later
+// stage (k_hg_overflow).
 template <typename R, int LPP>
 __global__ __launch_bounds__(SPH_BLOCK) void k_hg_list(DGrid g, int n, SolidArgs<R> a, NList nl) {
     if (a.skip()) return;   // uniform over the grid
-    if (a.hg_mode == 3 && a.st.scal[SPH_S_LIST_OVER] == 0.0) return;   // uniform too
     HgBody<R> body(a);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = t / LPP, lane = t % LPP;
-    const bool two_d = (a.F.flags & SPH_TWO_D) != 0;
     const size_t cstride = (size_t)nl.nmax * nl.npad;
-    bool listed = false, act = false;
-    int cnt = 0;
-    if (i < n) {
-        cnt = nl.cnt[i];
-        listed = cnt <= nl.nmax;
-        act = !(listed && a.hg_mode == 3);
-    }
+    const bool act = i < n;
+    bool listed = false;
     if (act) {
+        const int cnt = nl.cnt[i];
+        listed = cnt <= nl.nmax;
         double xi, yi, zi;
         body.begin(i, xi, yi, zi);
         if (listed) {
@@ -1697,7 +1690,7 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_hg_list(DGrid g, int n, SolidArgs
 
 // A later hourglass sub-step (mode 2) of the listed particles: per slot the stored u and
 // y_j, y_dst = y + sub k (sum_j u (u . (M y_j - y_i)) + source).  The overflowed
-// particles are k_hg_list's, in mode 3, launched after this.
+// particles are k_hg_overflow's, launched after this.
 template <typename R, int LPP>
 __global__ __launch_bounds__(SPH_BLOCK) void k_hg_apply(int n, SolidArgs<R> a, NList nl) {
     if (a.skip()) return;   // uniform over the grid
@@ -1722,6 +1715,26 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_hg_apply(int n, SolidArgs<R> a, N
     body.template lane_reduce<LPP>();
     if (act && lane == 0) {
         body.source(i, true);
+        body.end(i);
+    }
+}
+
+// A later sub-step of the particles whose list overflowed (none, unless the list build
+// raised the step's flag): each traverses the cells and evaluates its pairs in full.  A
+// few workgroups stride over the particles, so the launch costs next to nothing when
+// the flag is down and still covers every particle when it is up.
+template <typename R>
+__global__ __launch_bounds__(SPH_BLOCK) void k_hg_overflow(DGrid g, int n, SolidArgs<R> a, NList nl) {
+    if (a.skip() || a.st.scal[SPH_S_LIST_OVER] == 0.0) return;   // uniform over the grid
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (nl.cnt[i] <= nl.nmax) continue;
+        HgBody<R> body(a);
+        double xi, yi, zi;
+        body.begin(i, xi, yi, zi);
+        auto f = [&](int j, int mir, double rx, double ry, double rz) { body.pair(j, mir, rx, ry, rz); };
+        Traverse tr{g, a.s.start, a.s.cellid, a.X, a.any_out()};
+        tr(i, xi, yi, zi, f);
+        body.source(i, false);
         body.end(i);
     }
 }
@@ -1917,10 +1930,7 @@ static void launch_hg_pass(sph_ctx* c, const SolidArgs<R>& a, hipStream_t hs) {
         case 2: hipLaunchKernelGGL((k_hg_apply<R, 2>), gb, bb, 0, hs, n, a, c->nl); break;
         default: hipLaunchKernelGGL((k_hg_apply<R, 1>), gb, bb, 0, hs, n, a, c->nl); break;
         }
-        SolidArgs<R> b = a;
-        b.hg_mode = 3;
-        hipLaunchKernelGGL((k_hg_list<R, 1>), dim3((n + SPH_BLOCK - 1) / SPH_BLOCK), bb, 0, hs, c->grid, n, b,
-                           c->nl);
+        hipLaunchKernelGGL((k_hg_overflow<R>), dim3(8), bb, 0, hs, c->grid, n, a, c->nl);
         return;
     }
     switch (lpp) {
@@ -2067,13 +2077,14 @@ extern "C" int sph_solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const sp
         int npad = (int)((d.n + 63) / 64 * 64);
         HIPCHECK(c, hipMalloc((void**)&c->nl.idx, sizeof(int32_t) * (size_t)nmax * npad));
         HIPCHECK(c, hipMalloc((void**)&c->nl.cnt, sizeof(int32_t) * (size_t)npad));
-        c->nl.nmax = nmax;
+        c->list_alloc = nmax;
+        c->nl.nmax = c->list_cap > 0 ? std::min(nmax, c->list_cap) : nmax;
         c->nl.npad = npad;
     }
     if (S->sts && S->hourglass > 0.0 && c->hg_pairs && !c->hg_pair) {
         const int ncomp = (c->dims.flags & SPH_TWO_D) ? 2 : 3;
         const size_t rsz = c->dims.precision == SPH_PREC_MIXED ? sizeof(float) : sizeof(double);
-        HIPCHECK(c, hipMalloc(&c->hg_pair, rsz * ncomp * (size_t)c->nl.nmax * c->nl.npad));
+        HIPCHECK(c, hipMalloc(&c->hg_pair, rsz * ncomp * (size_t)c->list_alloc * c->nl.npad));
         HIPCHECK(c, hipMalloc((void**)&c->hg_src, sizeof(double4) * (size_t)c->nl.npad));
     }
     // replay: a step is fixed by its arguments and the host-side buffer indices, which

@@ -69,3 +69,38 @@ def test_autotune_leaves_valid_shapes(gpu):
     torch.cuda.synchronize()
     assert float(sv.sim_time) > t0 and np.all(np.isfinite(ps.x.to_numpy()))
     ps.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_list_overflow_fallback_matches_the_oracle(gpu, precision):
+    """With the list capped at 16 entries, interior particles (about 28 neighbours with
+    their mirror images) overflow and take the traversal in every list pass and in every
+    hourglass sub-step (k_hg_overflow), while corner particles stay listed; the run must
+    still match the oracle step for step."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from oracle import oracle as O
+    from sigmasph_amd.SOLID import SolidSettings
+    from helpers import solid_oracle_state
+    from test_gpu_solid import compare
+    ps = ParticleSystem(SimConfig(_deck()), precision=precision)
+    solver = ps.build_solver()
+    st = SolidSettings(ps)
+    s = solid_oracle_state(ps, st)
+    solver.initialize()
+    assert ps._lib.sph_tune(ps.ctx, N.SPH_TUNE_LIST_CAP, 16) == N.SPH_OK
+    v = ctypes.c_int32()
+    assert ps._lib.sph_tune_get(ps.ctx, N.SPH_TUNE_LIST_CAP, ctypes.byref(v)) == N.SPH_OK and v.value == 16
+    for k in range(3):
+        print("step %d" % k, flush=True)
+        solver.step()
+        O.solid_step(ps.dims, st.params(), st.solid_params(), s)
+    torch.cuda.synchronize()
+    assert ps.scal[28].item() == 1.0                 # the list build saw overflows
+    cnt = np.bincount(ps.neighbor_pairs()[:, 0], minlength=ps.particle_max_num)
+    assert (cnt > 16).any() and (cnt <= 16).any()  # both paths ran
+    if precision == "fp64":
+        compare(ps, st, s, 1e-9)
+    else:
+        compare(ps, st, s, 2e-4, eps=2.0 ** -24)
+    assert ps._lib.sph_tune(ps.ctx, N.SPH_TUNE_LIST_CAP, -1) == N.SPH_EINVAL

@@ -97,6 +97,9 @@ extern "C" {
 #define SPH_PLASTIC_NONE     0
 #define SPH_PLASTIC_LINEAR   1  /* J2 with linear isotropic hardening (MAT001)          */
 #define SPH_PLASTIC_JC       2  /* Johnson-Cook (MAT002)                                */
+#define SPH_DAMAGE_NONE      0
+#define SPH_DAMAGE_THRESHOLD 1  /* D = 1 once the tension -p exceeds spall_p (materials.py:416) */
+#define SPH_DAMAGE_JC        4  /* Johnson-Cook fracture with regularised softening (DAM004) */
 
 /* Cell lattice.  A particle at x sits in cell c = floor((x - origin) / cell) per axis.
  * Cells inside the dense box [box_lo, box_lo + box_n) are addressed directly; every
@@ -190,6 +193,13 @@ typedef struct {
     double mg_linear_exp;                /* 1: C2 = C3 = 0 for mu < 0              */
     int32_t eos_kind;                    /* SPH_EOS_*     */
     int32_t plastic_kind;                /* SPH_PLASTIC_* */
+    /* damage (materials.py:351-425): the spall cutoff (SPH_DAMAGE_THRESHOLD) and the
+     * Johnson-Cook fracture card: eps_f = max([D1 + D2 exp(D3 eta)] [1 + D4 ln max(1,
+     * rate)] [1 + D5 T*], EFMIN), failure displacement u_f (DAM004_johnson_cook.py) */
+    double spall_p;
+    double jcf_D1, jcf_D2, jcf_D3, jcf_D4, jcf_D5, jcf_efmin, jcf_uf;
+    int32_t damage_kind;                 /* SPH_DAMAGE_*  */
+    int32_t pad_;
 } sph_material;
 
 /* Parameters of the hypoelastic solid step (the HypoElasticSolver constructor,
@@ -225,6 +235,9 @@ typedef struct {
     int32_t pad_;
     double  sur_tol;         /* the surrogate's acceptance tolerance (fraction of sigma_y)   */
     double  sts_safety;      /* HOURGLASS_STS_SAFETY of the RKL stage count (SOLID.py:1087)  */
+    double  jcf_lc;          /* the JC softening length L_c = dx0 (SOLID.py:1675-1682)       */
+    int32_t damage;          /* some material has a damage model: sph_solid.damage is live   */
+    int32_t jc_failure;      /* some material has the JC fracture card: jc_omega is live     */
 } sph_solid_params;
 
 /* sph_solid_params.sts: how the s passes of a step are spent (hourglass_sts,
@@ -283,6 +296,11 @@ typedef struct {
     float*   harvest_y;      /* [harvest_cap][8]                                               */
     int64_t  harvest_cap;
     int64_t* harvest_count;  /* device counter                                                 */
+    /* damage history (history fields like eps_p, following hcur; NULL without a damage
+     * model): the damage D every model shares (SOLID.py:2858-2872, 3216-3382) and the
+     * Johnson-Cook initiation accumulator omega (DAM004) */
+    void*    damage[2];      /* [n] D in [0, 1]                                                */
+    void*    jc_omega[2];    /* [n] omega = Sum d_eps_p / eps_f                                */
 } sph_solid;
 
 /* ---- context ---------------------------------------------------------------------- */

@@ -33,7 +33,8 @@ class OrState(ctypes.Structure):
                 ("eps_p", _dp), ("e_int", _dp), ("T", _dp), ("de_int", _dp), ("de_hg", _dp),
                 ("eos_id", _ip), ("bc_flag", _ip), ("bc_id", _ip),
                 ("bc_disp", _dp), ("grad_v", _dp), ("cons", _dp),
-                ("mat", ctypes.c_void_p), ("q", _dp), ("av_mu", _dp)]
+                ("mat", ctypes.c_void_p), ("q", _dp), ("av_mu", _dp),
+                ("damage", _dp), ("jc_omega", _dp), ("d_eps_prod", _dp)]
 
 
 def build():
@@ -109,7 +110,8 @@ class OracleState:
     FIELDS3 = ("x", "x0", "u", "v", "shift", "acc", "lam_grad")
     FIELDS1 = ("V", "m", "p", "lam", "div", "de_visc")
     SOLID_D = {"sig": 6, "eps_p": 0, "e_int": 0, "T": 0, "de_int": 0, "de_hg": 0,
-               "bc_disp": 3, "grad_v": 9, "q": 0, "av_mu": 0}
+               "bc_disp": 3, "grad_v": 9, "q": 0, "av_mu": 0, "d_eps_prod": 0}
+    DAMAGE_D = ("damage", "jc_omega")   # allocated for a deck with a damage model
     SOLID_I = ("eos_id", "bc_flag", "bc_id")
 
     def __init__(self, n):
@@ -123,13 +125,17 @@ class OracleState:
         self.obj = np.zeros(n, dtype=np.int32)
         self.scal = np.zeros(32)
         self.solid = False
+        self.damage = self.jc_omega = None
 
-    def make_solid(self, materials, ncons=0):
-        """Attach the solid fields (zero stress, e = e0, T = T0 of each row's material)
-        and the material table (a ctypes array of SphMaterial)."""
+    def make_solid(self, materials, ncons=0, damage=False):
+        """Attach the solid fields (zero stress, e = e0, T = T0 of each row's material,
+        D = omega = 0 when `damage`) and the material table (a ctypes array of SphMaterial)."""
         n = self.n
         for f, w in self.SOLID_D.items():
             setattr(self, f, np.zeros((n, w)) if w else np.zeros(n))
+        if damage:
+            for f in self.DAMAGE_D:
+                setattr(self, f, np.zeros(n))
         for f in self.SOLID_I:
             setattr(self, f, np.zeros(n, dtype=np.int32))
         self.bc_id[:] = -1
@@ -158,6 +164,9 @@ class OracleState:
                 setattr(st, f, _ptr(getattr(self, f)))
             for f in self.SOLID_I:
                 setattr(st, f, _ptr(getattr(self, f), _ip))
+            for f in self.DAMAGE_D:
+                if getattr(self, f) is not None:
+                    setattr(st, f, _ptr(getattr(self, f)))
             st.mat = ctypes.cast(self.mat, ctypes.c_void_p)
         self._c = st
         return st
@@ -208,6 +217,15 @@ def solid_hourglass_sts(dims, params, sparams, s, y, stages, dt):
                                      ctypes.byref(st), ctypes.c_int(int(stages)), ctypes.c_double(dt),
                                      _ptr(y))
     return y
+
+
+def jc_damage_step(mat, d_eps_p, eta, rate_ratio, t_star, l_c, omega, D):
+    """One Johnson-Cook fracture increment on a material row (SphMaterial): (omega, D)."""
+    om, d = ctypes.c_double(omega), ctypes.c_double(D)
+    lib().oracle_jc_damage_step(ctypes.byref(mat), ctypes.c_double(d_eps_p), ctypes.c_double(eta),
+                                ctypes.c_double(rate_ratio), ctypes.c_double(t_star), ctypes.c_double(l_c),
+                                ctypes.byref(om), ctypes.byref(d))
+    return om.value, d.value
 
 
 def solid_reduce(s, which):

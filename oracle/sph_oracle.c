@@ -52,6 +52,8 @@ typedef struct {
     const sph_material* mat;             /* [nmat] */
     double *q;                           /* [n] VNR bulk viscosity */
     double *av_mu;                       /* [n] max_j |mu_ij| of Monaghan's Pi */
+    double *damage, *jc_omega;           /* [n] damage D, JC omega (NULL without damage) */
+    double *d_eps_prod;                  /* [n] the return map's production of the step */
 } or_state;
 
 typedef struct {
@@ -167,6 +169,8 @@ int64_t oracle_sort(const sph_dims* d, or_state* s, or_grid* g) {
         or_permute_d(s->sig, perm, n, 6);
         or_permute_d(s->eps_p, perm, n, 1);
         or_permute_d(s->e_int, perm, n, 1);
+        if (s->damage) or_permute_d(s->damage, perm, n, 1);
+        if (s->jc_omega) or_permute_d(s->jc_omega, perm, n, 1);
         or_permute_d(s->T, perm, n, 1);
         or_permute_d(s->de_int, perm, n, 1);
         or_permute_d(s->de_hg, perm, n, 1);
@@ -1172,8 +1176,9 @@ static void or_velocity_gradient(const sph_dims* d, const sph_fluid_params* P,
     }
 }
 
-/* compute_ale_transport_task and _eps_p_task (SOLID.py:1799-1862): the conservative
- * difference form of (du . grad) phi for the deviator (acc 0-8) and eps_p (acc 9) */
+/* compute_ale_transport_task, _eps_p_task and _jcf_task (SOLID.py:1799-1862, 1925-1945):
+ * the conservative difference form of (du . grad) phi for the deviator (acc 0-8), eps_p
+ * (acc 9) and, on same-material pairs, the JC fracture history omega, D (acc 10, 11) */
 static void or_ale_hist_task(void* c, int64_t i, int64_t j, int mir, const double* r) {
     or_tctx* t = (or_tctx*)c;
     const or_state* s = t->s;
@@ -1202,6 +1207,30 @@ static void or_ale_hist_task(void* c, int64_t i, int64_t j, int mir, const doubl
         double flux = (rho_j * djg) * ej - (rho_i * dig) * ei;
         t->acc[9] += wj * (flux - ei * drd) / rho_i;
     }
+    if (t->S->damage && t->S->jc_failure && s->eos_id[i] == s->eos_id[j]) {
+        double fi[2] = { s->jc_omega[i], s->damage[i] }, fj[2] = { s->jc_omega[j], s->damage[j] };
+        for (int k = 0; k < 2; ++k) {
+            double flux = (rho_j * djg) * fj[k] - (rho_i * dig) * fi[k];
+            t->acc[10 + k] += wj * (flux - fi[k] * drd) / rho_i;
+        }
+    }
+}
+
+/* jc_damage_step (DAM004_johnson_cook.py:104-149): omega += d_eps_p / eps_f,
+ * eps_f = max([D1 + D2 exp(D3 eta)] [1 + D4 ln max(1, rate)] [1 + D5 clamp(T*)], EFMIN,
+ * 1e-6); the part of the increment past omega = 1 softens D by L_c (that part) / u_f */
+void oracle_jc_damage_step(const sph_material* M, double d_eps_p, double eta, double rate_ratio,
+                           double t_star, double l_c, double* omega, double* D) {
+    if (d_eps_p <= 0.0) return;
+    double ef = (M->jcf_D1 + M->jcf_D2 * exp(M->jcf_D3 * eta))
+                * (1.0 + M->jcf_D4 * log(fmax(1.0, rate_ratio)))
+                * (1.0 + M->jcf_D5 * fmin(1.0, fmax(0.0, t_star)));
+    ef = fmax(ef, fmax(M->jcf_efmin, 1.0e-6));
+    double d_omega = d_eps_p / ef;
+    double soft = d_eps_p;
+    if (*omega < 1.0) soft = fmax(0.0, *omega + d_omega - 1.0) * ef;
+    *omega += d_omega;
+    if (soft > 0.0) *D = fmin(1.0, *D + l_c * soft / M->jcf_uf);
 }
 
 /* update_stress (SOLID.py:2691-3004), with the J2 and Johnson-Cook return maps */
@@ -1211,8 +1240,10 @@ static void or_update_stress(const sph_dims* d, const sph_fluid_params* P,
     double dt = s->scal[SPH_S_DT];
     double* dsig = (double*)malloc(sizeof(double) * 9 * n);
     double* dep = (double*)calloc(n, sizeof(double));
+    double* djc = (double*)calloc(2 * n, sizeof(double));
     double* pw = (double*)calloc(n, sizeof(double));
     int ale = P->pst_enabled && S->ale_history;
+    int dmg = S->damage && s->damage;
     #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < n; ++i) {
         const double* Lv = s->grad_v + 9 * i;
@@ -1240,6 +1271,8 @@ static void or_update_stress(const sph_dims* d, const sph_fluid_params* P,
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < 3; ++b) sd[a][b] += t.acc[3 * a + b];
             dep[i] = t.acc[9];
+            djc[2 * i] = t.acc[10];
+            djc[2 * i + 1] = t.acc[11];
         }
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) dsig[9 * i + 3 * a + b] = sd[a][b];
@@ -1256,12 +1289,27 @@ static void or_update_stress(const sph_dims* d, const sph_fluid_params* P,
         const sph_material* M = or_mat(s, i);
         if (M->G <= 0.0) for (int k = 0; k < 9; ++k) sy[k] = 0.0;
         if (ale && S->plastic) s->eps_p[i] = fmax(0.0, s->eps_p[i] + dt * dep[i]);
+        /* the JC fracture history moved with the shift, on JC-fracture particles (2812-2818) */
+        if (dmg && ale && S->jc_failure && M->damage_kind == SPH_DAMAGE_JC) {
+            s->jc_omega[i] = fmax(0.0, s->jc_omega[i] + dt * djc[2 * i]);
+            s->damage[i] = fmin(1.0, fmax(0.0, s->damage[i] + dt * djc[2 * i + 1]));
+        }
         double dp = 0.0;
         if (S->plastic) {
-            if (M->plastic_kind == SPH_PLASTIC_LINEAR)
-                oracle_j2_return(sy, M->G, M->sigma_y0, M->hardening, s->eps_p[i], &dp);
+            /* the yield surface scaled by s_d = 1 - D; at s_d = 0 a plastic particle returns
+             * to the origin, d_eps_p = sigma_vM / 3G (2849-2880) */
+            double s_d = dmg ? fmax(0.0, 1.0 - s->damage[i]) : 1.0;
+            if (dmg && s_d <= 0.0 && M->plastic_kind != SPH_PLASTIC_NONE) {
+                if (M->G > 0.0) {
+                    double ss = 0.0;
+                    for (int k = 0; k < 9; ++k) ss += sy[k] * sy[k];
+                    dp = sqrt(1.5 * ss) / (3.0 * M->G);
+                    for (int k = 0; k < 9; ++k) sy[k] = 0.0;
+                }
+            } else if (M->plastic_kind == SPH_PLASTIC_LINEAR)
+                oracle_j2_return(sy, M->G, s_d * M->sigma_y0, s_d * M->hardening, s->eps_p[i], &dp);
             else if (M->plastic_kind == SPH_PLASTIC_JC)
-                oracle_jc_return(sy, M->G, M->jc_A, M->jc_B, M->jc_n, M->jc_C, M->jc_eps0_dot,
+                oracle_jc_return(sy, M->G, s_d * M->jc_A, s_d * M->jc_B, M->jc_n, M->jc_C, M->jc_eps0_dot,
                                  M->jc_T0, M->jc_Tm, M->jc_m, s->eps_p[i], s->T[i], dt, &dp, NULL);
             s->eps_p[i] += dp;
             if (dp > 0.0) {
@@ -1270,13 +1318,14 @@ static void or_update_stress(const sph_dims* d, const sph_fluid_params* P,
                 pw[i] = s->V[i] * sqrt(1.5 * ss) * dp;
             }
         }
+        if (s->d_eps_prod) s->d_eps_prod[i] = dp;
         double* q = s->sig + 6 * i;
         q[0] = sy[0]; q[1] = sy[4]; q[2] = sy[8]; q[3] = sy[1]; q[4] = sy[2]; q[5] = sy[5];
     }
     double w = 0.0;
     for (int64_t i = 0; i < n; ++i) w += pw[i];
     s->scal[SPH_S_PLASTIC_WORK] += w;
-    free(dsig); free(dep); free(pw);
+    free(dsig); free(dep); free(djc); free(pw);
 }
 
 /* compute_ale_transport_e_task (SOLID.py:1949-2045): acc[0] advection, acc[1] diffusion */
@@ -1533,7 +1582,26 @@ static void or_stress_accelerations(const sph_dims* d, const sph_fluid_params* P
     for (int64_t i = 0; i < n; ++i) {
         for (int k = 0; k < 3; ++k) s->acc[3 * i + k] = P->g[k];
         double rho = s->m[i] / s->V[i];
-        s->p[i] = or_solid_pressure(P, or_mat(s, i), rho, s->e_int[i]);
+        const sph_material* M = or_mat(s, i);
+        double p_raw = or_solid_pressure(P, M, rho, s->e_int[i]);
+        s->p[i] = p_raw;
+        if (S->damage && s->damage) {
+            /* the spall threshold, JC fracture on the step's production at the triaxiality
+             * of the returned deviator, then the unilateral split (SOLID.py:3216-3382) */
+            if (M->damage_kind == SPH_DAMAGE_THRESHOLD && M->spall_p > 0.0 && -p_raw > M->spall_p)
+                s->damage[i] = 1.0;
+            if (S->jc_failure && M->damage_kind == SPH_DAMAGE_JC && s->d_eps_prod[i] > 0.0) {
+                const double* q = s->sig + 6 * i;
+                double vm = sqrt(1.5 * (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]
+                                        + 2.0 * (q[3] * q[3] + q[4] * q[4] + q[5] * q[5])));
+                double eta = vm > 0.0 ? -p_raw / vm : 0.0;
+                double rr = s->d_eps_prod[i] / dt / M->jc_eps0_dot;
+                double ts = M->jc_Tm > M->jc_T0 ? (s->T[i] - M->jc_T0) / (M->jc_Tm - M->jc_T0) : 0.0;
+                oracle_jc_damage_step(M, s->d_eps_prod[i], eta, rr, ts, S->jcf_lc, &s->jc_omega[i], &s->damage[i]);
+            }
+            double D = s->damage[i];
+            s->p[i] = fmax(p_raw, 0.0) + (1.0 - D) * fmin(p_raw, 0.0);
+        }
         if (S->q_l2 > 0.0) {             /* VNR q from tr(grad v) (SOLID.py:3392-3407) */
             const double* G = s->grad_v + 9 * i;
             double dv = G[0] + G[4] + G[8];

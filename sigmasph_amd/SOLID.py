@@ -143,6 +143,17 @@ class SolidSettings:
         self.e_guard_on = self.mg_on
         self.allow_negative_pressure = bool(c("allowNegativePressure"))
         self.plastic = any(m.plastic_kind != N.PLASTIC_NONE for m in mats)
+        # damage (SOLID.py:449-480): D is live when some material has a damage model or a
+        # spall cutoff; omega when one has the JC fracture card.  A damaged material with
+        # no yield surface would take its damage through the pair force instead
+        # (damage_elastic, SOLID.py:459-465), which this build does not do.
+        self.has_damage = any(m.damage_kind != N.DAMAGE_NONE for m in mats)
+        self.has_jc_failure = any(m.damage_kind == N.DAMAGE_JC for m in mats)
+        for m in mats:
+            if m.damage_kind != N.DAMAGE_NONE and m.plastic_kind == N.PLASTIC_NONE:
+                raise NotImplementedError("material %r: damage on a material without plasticity is "
+                                          "not part of this build" % m.name)
+        self.jcf_lc = self.dx0                                           # SOLID.py:1675-1682
         self.ncons = len(getattr(ps, "constraints", []) or [])
         self.mat_table = material_table(self.materials)
 
@@ -175,6 +186,9 @@ class SolidSettings:
         S.plastic = int(self.plastic)
         S.nmat = len(self.materials)
         S.ncons = self.ncons
+        S.damage = int(self.has_damage)
+        S.jc_failure = int(self.has_jc_failure)
+        S.jcf_lc = self.jcf_lc
         return S
 
     def load_factor(self, t):

@@ -69,6 +69,7 @@ SPH_TUNE_LIST_CAP = 9
 # material kinds
 EOS_LINEAR, EOS_TAIT, EOS_MG = 0, 1, 3
 PLASTIC_NONE, PLASTIC_LINEAR, PLASTIC_JC = 0, 1, 2
+DAMAGE_NONE, DAMAGE_THRESHOLD, DAMAGE_JC = 0, 1, 4
 
 _d3 = ctypes.c_double * 3
 _i3 = ctypes.c_int32 * 3
@@ -153,7 +154,10 @@ class SphMaterial(ctypes.Structure):
         "eos_S", "eos_gamma", "delta",
         "jc_A", "jc_B", "jc_n", "jc_C", "jc_eps0_dot", "jc_T0", "jc_Tm", "jc_m",
         "mg_C0", "mg_C1", "mg_C2", "mg_C3", "mg_C4", "mg_C5", "mg_c0_ref", "mg_linear_exp")] + [
-        ("eos_kind", ctypes.c_int32), ("plastic_kind", ctypes.c_int32)]
+        ("eos_kind", ctypes.c_int32), ("plastic_kind", ctypes.c_int32)] + [
+        (n, ctypes.c_double) for n in ("spall_p", "jcf_D1", "jcf_D2", "jcf_D3", "jcf_D4", "jcf_D5",
+                                       "jcf_efmin", "jcf_uf")] + [
+        ("damage_kind", ctypes.c_int32), ("pad_", ctypes.c_int32)]
 
 
 class SphSolidParams(ctypes.Structure):
@@ -162,7 +166,8 @@ class SphSolidParams(ctypes.Structure):
         "l_eig_tol", "q_l2", "q_cl", "av_alpha", "av_beta", "av_eps")] + [(n, ctypes.c_int32) for n in (
         "kernel_correction", "jaumann", "ale_history", "sts", "sts_stages", "sts_max_stages",
         "sts_refresh", "dt_pair_mode", "visc_work", "thermal", "plastic", "nmat", "ncons",
-        "pad_")] + [("sur_tol", ctypes.c_double), ("sts_safety", ctypes.c_double)]
+        "pad_")] + [("sur_tol", ctypes.c_double), ("sts_safety", ctypes.c_double),
+                    ("jcf_lc", ctypes.c_double), ("damage", ctypes.c_int32), ("jc_failure", ctypes.c_int32)]
 
 
 _vp2 = ctypes.c_void_p * 2
@@ -184,6 +189,7 @@ class SphSolid(ctypes.Structure):
         ("sur_x", ctypes.c_void_p), ("sur_y", ctypes.c_void_p), ("sur_flag", ctypes.c_void_p),
         ("harvest_x", ctypes.c_void_p), ("harvest_y", ctypes.c_void_p),
         ("harvest_cap", ctypes.c_int64), ("harvest_count", ctypes.c_void_p),
+        ("damage", _vp2), ("jc_omega", _vp2),
     ]
 
 

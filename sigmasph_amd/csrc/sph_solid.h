@@ -44,6 +44,8 @@ struct DSolidP {
     int q_on, av_on;
     int sur_on, harvest_on;              // the surrogate answers / plain returns are harvested
     double sur_tol;
+    int damage_on, jcf_on, transport_jcf;   // damage D live; JC fracture omega live; moved by the shift
+    double jcf_lc;                       // the JC softening length L_c = dx0
 };
 
 template <typename R> struct DSolid {
@@ -58,6 +60,8 @@ template <typename R> struct DSolid {
     R* q;                             // [n] VNR q (when q_on)
     float* sur_x; float* sur_y; int32_t* sur_flag;
     float* hx; float* hy; long long hcap; unsigned long long* hcount;
+    const R* dmg_in; R* dmg_out;      // damage D (history buffers, when damage_on)
+    const R* om_in; R* om_out;        // JC fracture omega (when jcf_on)
     const sph_material* mat;
     double* cons;
 };
@@ -166,13 +170,15 @@ __device__ __forceinline__ double cold_energy(const sph_material& M, double rho,
 }
 
 // _jc_residual (MAT002_johnson_cook.py:251-277)
-__device__ __forceinline__ double jc_residual(double d, double vm, double G, const sph_material& M,
-                                              double m_temp, double eps_p, double dt, double* dg) {
+// with A and B given (the damaged surface scales both by 1 - D)
+__device__ __forceinline__ double jc_residual(double d, double vm, double G, double A, double B,
+                                              const sph_material& M, double m_temp, double eps_p, double dt,
+                                              double* dg) {
     double et = eps_p + d;
-    double ss = M.jc_A, dss = 0.0;
-    if (et > 0.0 && M.jc_B > 0.0 && M.jc_n > 0.0) {
-        ss += M.jc_B * pow(et, M.jc_n);
-        if (et > 1e-12) dss = M.jc_n * M.jc_B * pow(et, M.jc_n - 1.0);
+    double ss = A, dss = 0.0;
+    if (et > 0.0 && B > 0.0 && M.jc_n > 0.0) {
+        ss += B * pow(et, M.jc_n);
+        if (et > 1e-12) dss = M.jc_n * B * pow(et, M.jc_n - 1.0);
     }
     double rate = 1.0;
     if (dt > 0.0 && M.jc_eps0_dot > 0.0) rate = (d / dt) / M.jc_eps0_dot;
@@ -184,13 +190,15 @@ __device__ __forceinline__ double jc_residual(double d, double vm, double G, con
 
 // The return maps on the trial deviator (in f64): j2_radial_return
 // (MAT001_linear_plasticity.py:65-81) and jc_radial_return (MAT002_johnson_cook.py:280-354,
-// bracketed Newton-bisection, at most 40 iterations).  Returns d_eps_p; scales q.
-__device__ double return_map(const sph_material& M, double q[6], double eps_p, double temp, double dt) {
+// bracketed Newton-bisection, at most 40 iterations).  Returns d_eps_p; scales q.  The
+// yield surface is scaled by s_d = 1 - D: sigma_y0 and H, or A and B (SOLID.py:2873-2900).
+__device__ double return_map(const sph_material& M, double q[6], double eps_p, double temp, double dt,
+                             double s_d = 1.0) {
     double ss = q[0] * q[0] + q[1] * q[1] + q[3] * q[3] + 2.0 * (q[2] * q[2] + q[4] * q[4] + q[5] * q[5]);
     double vm = sqrt(1.5 * ss);
     double sc = 1.0, d = 0.0;
     if (M.plastic_kind == SPH_PLASTIC_LINEAR) {
-        double sy = M.sigma_y0, H = M.hardening, G = M.G;
+        double sy = s_d * M.sigma_y0, H = s_d * M.hardening, G = M.G;
         if (sy > 0.0) {
             double sig_y = sy + H * eps_p;
             if (vm > sig_y && (3.0 * G + H) > 0.0) {
@@ -199,27 +207,27 @@ __device__ double return_map(const sph_material& M, double q[6], double eps_p, d
             }
         }
     } else if (M.plastic_kind == SPH_PLASTIC_JC) {
-        double G = M.G;
-        if (G > 0.0 && M.jc_A > 0.0) {
+        double G = M.G, A = s_d * M.jc_A, B = s_d * M.jc_B;
+        if (G > 0.0 && A > 0.0) {
             double m_temp = 1.0;
             if (M.jc_Tm > M.jc_T0 && temp > M.jc_T0) {
                 double ts = fmin(1.0, fmax(0.0, (temp - M.jc_T0) / (M.jc_Tm - M.jc_T0)));
                 m_temp = fmax(0.0, 1.0 - pow(ts, M.jc_m));
             }
-            double s0 = M.jc_A;
-            if (eps_p > 0.0 && M.jc_B > 0.0 && M.jc_n > 0.0) s0 += M.jc_B * pow(eps_p, M.jc_n);
+            double s0 = A;
+            if (eps_p > 0.0 && B > 0.0 && M.jc_n > 0.0) s0 += B * pow(eps_p, M.jc_n);
             double sy0 = s0 * m_temp;
             if (vm > sy0 && sy0 > 0.0) {
                 double lo = 0.0, hi = (vm - sy0) / (3.0 * G), dg;
                 double d_ref = 0.0;
                 if (dt > 0.0 && M.jc_eps0_dot > 0.0 && M.jc_C > 0.0) d_ref = dt * M.jc_eps0_dot;
                 if (d_ref > 0.0 && d_ref < hi) {
-                    double g = jc_residual(d_ref, vm, G, M, m_temp, eps_p, dt, &dg);
+                    double g = jc_residual(d_ref, vm, G, A, B, M, m_temp, eps_p, dt, &dg);
                     if (g > 0.0) lo = d_ref; else hi = d_ref;
                 }
                 d = lo > 0.0 ? lo : hi;
                 for (int it = 0; it < 40; ++it) {
-                    double g = jc_residual(d, vm, G, M, m_temp, eps_p, dt, &dg);
+                    double g = jc_residual(d, vm, G, A, B, M, m_temp, eps_p, dt, &dg);
                     if (g > 0.0) lo = d; else hi = d;
                     if (fabs(g) <= 1e-6 * vm || hi - lo <= 1e-6 * hi) break;
                     double dn = d - g / dg;
@@ -234,6 +242,23 @@ __device__ double return_map(const sph_material& M, double q[6], double eps_p, d
     if (d > 0.0)
         for (int k = 0; k < 6; ++k) q[k] *= sc;
     return d;
+}
+
+// One Johnson-Cook fracture increment (jc_damage_step, DAM004_johnson_cook.py:118-149):
+// omega grows by d_eps_p / eps_f at the triaxiality eta, the plastic strain-rate ratio and
+// the homologous temperature; the part of the increment that carries omega past 1
+// softens D by L_c (that part) / u_f.
+__device__ __forceinline__ void jcf_step(const sph_material& M, double dep, double eta, double rate_ratio,
+                                         double t_star, double lc, double& om, double& D) {
+    if (!(dep > 0.0)) return;
+    double ef = (M.jcf_D1 + M.jcf_D2 * exp(M.jcf_D3 * eta)) * (1.0 + M.jcf_D4 * log(fmax(1.0, rate_ratio))) *
+                (1.0 + M.jcf_D5 * fmin(1.0, fmax(0.0, t_star)));
+    ef = fmax(ef, fmax(M.jcf_efmin, 1.0e-6));
+    const double dom = dep / ef;
+    double soft = dep;
+    if (om < 1.0) soft = fmax(0.0, om + dom - 1.0) * ef;
+    om += dom;
+    if (soft > 0.0) D = fmin(1.0, D + lc * soft / M.jcf_uf);
 }
 
 // The Johnson-Cook yield stress at zero increment (rate factor 1) and the thermal factor:
@@ -277,7 +302,9 @@ __device__ void sur_features(const sph_material& M, const double q[6], double vm
 // ------------------------------------------------------------------------------------
 // pass SA
 // ------------------------------------------------------------------------------------
-template <typename R> struct StressBody {
+// DMG false compiles the damage models out (a deck without one), which keeps their
+// registers off the common path: the two agree whenever damage_on is 0.
+template <typename R, bool DMG = false> struct StressBody {
     typedef SolidArgs<R> Args;
     const Args& a;
     double w_diff, w_plastic, w_iepair, w_ieadv, w_iediff, w_ieale;
@@ -290,6 +317,7 @@ template <typename R> struct StressBody {
     R e00, e11, e22, e01, e02, e12, rate, drate;
     R G[3][3];
     R as_xx, as_yy, as_xy, as_zz, as_xz, as_yz, a_ep, a_eadv, a_ediff;
+    R dm_i, om_i, a_dm, a_om;   // damage D and JC omega, and their ALE transport
 
     __device__ StressBody(const Args& a_)
         : a(a_), w_diff(0), w_plastic(0), w_iepair(0), w_ieadv(0), w_iediff(0), w_ieale(0), mg_c2(0), q_c(0) {}
@@ -327,6 +355,9 @@ template <typename R> struct StressBody {
             for (int q = 0; q < 3; ++q) G[p][q] = 0;
         as_xx = as_yy = as_xy = as_zz = as_xz = as_yz = 0;
         a_ep = a_eadv = a_ediff = 0;
+        dm_i = om_i = a_dm = a_om = 0;
+        if (DMG && a.S.damage_on) dm_i = a.sd.dmg_in[i];
+        if (DMG && a.S.jcf_on) om_i = a.sd.om_in[i];
     }
 
     __device__ void pair(int j, int mir, double rxd, double ryd, double rzd) {
@@ -393,6 +424,12 @@ template <typename R> struct StressBody {
             as_xy += A * (sj.xy - si.xy); as_zz += A * (sj.zz - si.zz);
             if (!two_d) { as_xz += A * (sj.xz - si.xz); as_yz += A * (sj.yz - si.yz); }
             if (S.transport_ep) a_ep += A * (a.sd.ep_in[j] - epi);
+            if (DMG && S.transport_jcf && same) {
+                // the JC fracture history (omega, D), same-material pairs only
+                // (compute_ale_transport_jcf_task, SOLID.py:1925-1945)
+                a_om += A * (a.sd.om_in[j] - om_i);
+                a_dm += A * (a.sd.dmg_in[j] - dm_i);
+            }
             R ej = a.sd.e_in[j];
             a_eadv += A * (ej - ei);
             if (S.ale_diff > 0.0 && same) {
@@ -416,6 +453,7 @@ template <typename R> struct StressBody {
         as_xx = lane_sum<LPP>(as_xx); as_yy = lane_sum<LPP>(as_yy);
         as_xy = lane_sum<LPP>(as_xy); as_zz = lane_sum<LPP>(as_zz);
         a_ep = lane_sum<LPP>(a_ep); a_eadv = lane_sum<LPP>(a_eadv); a_ediff = lane_sum<LPP>(a_ediff);
+        if (DMG && a.S.transport_jcf) { a_om = lane_sum<LPP>(a_om); a_dm = lane_sum<LPP>(a_dm); }
         if (!two_d) {
             e22 = lane_sum<LPP>(e22); e02 = lane_sum<LPP>(e02); e12 = lane_sum<LPP>(e12);
             G[0][2] = lane_sum<LPP>(G[0][2]); G[1][2] = lane_sum<LPP>(G[1][2]);
@@ -525,7 +563,26 @@ template <typename R> struct StressBody {
         double ep = (double)epi;
         if (S.transport_ep) ep = fmax(0.0, ep + dt * (double)a_ep);
         double Ti = (double)sd.T[i];
-        if (S.plastic) {
+        // the JC fracture history moved with the shift, on JC-fracture particles
+        // (SOLID.py:2812-2818), before the return map reads D
+        double Dm = (double)dm_i, om = (double)om_i;
+        if (DMG && S.transport_jcf && M.damage_kind == SPH_DAMAGE_JC) {
+            om = fmax(0.0, om + dt * (double)a_om);
+            Dm = fmin(1.0, fmax(0.0, Dm + dt * (double)a_dm));
+        }
+        double dprod = 0.0;   // this step's plastic production (d_eps_prod)
+        // the yield surface shrinks by s_d = 1 - D; a plastic particle at D = 1 returns to the
+        // origin with the whole trial invariant as plastic strain (SOLID.py:2849-2880)
+        const double s_d = (DMG && S.damage_on) ? fmax(0.0, 1.0 - Dm) : 1.0;
+        if (DMG && S.plastic && S.damage_on && s_d <= 0.0 && M.plastic_kind != SPH_PLASTIC_NONE) {
+            if (M.G > 0.0) {
+                double ss = q6[0] * q6[0] + q6[1] * q6[1] + q6[3] * q6[3] +
+                            2.0 * (q6[2] * q6[2] + q6[4] * q6[4] + q6[5] * q6[5]);
+                dprod = sqrt(1.5 * ss) / (3.0 * M.G);
+                for (int k = 0; k < 6; ++k) q6[k] = 0.0;
+                ep += dprod;
+            }
+        } else if (S.plastic) {
             bool deferred = false;
             if (S.sur_on) {
                 // outside the yield surface at zero increment: the surrogate answers and
@@ -548,7 +605,8 @@ template <typename R> struct StressBody {
                 double q0[6];
                 if (S.harvest_on)
                     for (int k = 0; k < 6; ++k) q0[k] = q6[k];
-                double dep = return_map(M, q6, ep, Ti, dt);
+                double dep = return_map(M, q6, ep, Ti, dt, s_d);
+                dprod = dep;
                 if (dep > 0.0) {
                     double ss = q6[0] * q6[0] + q6[1] * q6[1] + q6[3] * q6[3] +
                                 2.0 * (q6[2] * q6[2] + q6[4] * q6[4] + q6[5] * q6[5]);
@@ -624,7 +682,28 @@ template <typename R> struct StressBody {
         st.rho_new[i] = (R)(m / Vfull);
         // ---- the pressure from the particle's EOS row (SOLID.py:3212-3252)
         const double rho_n = m / Vfull;
-        b.p[i] = (R)solid_eos(M, rho_n, e_new, F.allow_neg);
+        const double p_raw = solid_eos(M, rho_n, e_new, F.allow_neg);
+        double p_st = p_raw;
+        if (DMG && S.damage_on) {
+            // the damage models (SOLID.py:3216-3305): the spall threshold, then Johnson-Cook
+            // fracture on this step's production at eta = -p / sigma_vM of the returned
+            // deviator; then the unilateral split, tension carried in proportion 1 - D
+            if (M.damage_kind == SPH_DAMAGE_THRESHOLD && M.spall_p > 0.0 && -p_raw > M.spall_p) Dm = 1.0;
+            if (M.damage_kind == SPH_DAMAGE_JC && dprod > 0.0) {
+                double ss = q6[0] * q6[0] + q6[1] * q6[1] + q6[3] * q6[3] +
+                            2.0 * (q6[2] * q6[2] + q6[4] * q6[4] + q6[5] * q6[5]);
+                double vm = sqrt(1.5 * ss);
+                double eta = vm > 0.0 ? -p_raw / vm : 0.0;
+                double rr = dprod / dt / M.jc_eps0_dot;
+                double Tn = (double)sd.T[i];
+                double ts = M.jc_Tm > M.jc_T0 ? (Tn - M.jc_T0) / (M.jc_Tm - M.jc_T0) : 0.0;
+                jcf_step(M, dprod, eta, rr, ts, S.jcf_lc, om, Dm);
+            }
+            p_st = fmax(p_raw, 0.0) + (1.0 - Dm) * fmin(p_raw, 0.0);
+            sd.dmg_out[i] = (R)Dm;
+            if (S.jcf_on) sd.om_out[i] = (R)om;
+        }
+        b.p[i] = (R)p_st;
         if (M.eos_kind == SPH_EOS_MG) {
             // clamp_dt_mie_gruneisen (SOLID.py:3886-3907): c^2 + (4/3) G / rho
             double c2;
@@ -1233,7 +1312,7 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_sur_apply(int n, DSolid<R> sd, co
         if (ok) {
             double mt, dg;
             jc_static_yield(M, ep, temp, &mt);
-            const double g = jc_residual(d, vm, M.G, M, mt, ep, dt, &dg);
+            const double g = jc_residual(d, vm, M.G, M.jc_A, M.jc_B, M, mt, ep, dt, &dg);
             const double sy = vm - 3.0 * M.G * d - g;     // sigma_y at the answer
             ok = sy > 0.0 && vm - 3.0 * M.G * d > 0.0 && fabs(g) <= tol * sy;
             const double mag = (vm - 3.0 * M.G * d) / un;
@@ -1795,6 +1874,10 @@ static DSolidP make_solid_params(const sph_ctx* c, const sph_fluid_params* P, co
     d.av_beta = S->av_beta;
     d.av_epsh2 = S->av_eps * c->dims.h * c->dims.h;
     d.av_on = S->av_alpha > 0.0 || S->av_beta > 0.0;
+    d.damage_on = S->damage != 0;
+    d.jcf_on = S->damage && S->jc_failure;
+    d.transport_jcf = d.jcf_on && d.transport;
+    d.jcf_lc = S->jcf_lc;
     return d;
 }
 
@@ -1813,6 +1896,8 @@ template <typename R> static DSolid<R> dsolid(const sph_state* st, const sph_sol
     d.sur_x = sd->sur_x; d.sur_y = sd->sur_y; d.sur_flag = sd->sur_flag;
     d.hx = sd->harvest_x; d.hy = sd->harvest_y; d.hcap = (long long)sd->harvest_cap;
     d.hcount = (unsigned long long*)sd->harvest_count;
+    d.dmg_in = (const R*)sd->damage[h]; d.dmg_out = (R*)sd->damage[1 - h];
+    d.om_in = (const R*)sd->jc_omega[h]; d.om_out = (R*)sd->jc_omega[1 - h];
     d.mat = sd->mat;
     d.cons = sd->cons;
     return d;
@@ -1882,6 +1967,8 @@ extern "C" int sph_solid_sort(sph_ctx* c, sph_state* st, sph_solid* sd, void* st
     add(sd->sig[h], sd->sig[1 - h], 6 * rb);
     add(sd->eps_p[h], sd->eps_p[1 - h], rb);
     add(sd->e_int[h], sd->e_int[1 - h], rb);
+    if (sd->damage[0] && sd->damage[1]) add(sd->damage[h], sd->damage[1 - h], rb);
+    if (sd->jc_omega[0] && sd->jc_omega[1]) add(sd->jc_omega[h], sd->jc_omega[1 - h], rb);
     add(sd->T[k], sd->T[1 - k], rb);
     add(sd->de_int[k], sd->de_int[1 - k], rb);
     add(sd->eos_id[k], sd->eos_id[1 - k], 4);
@@ -1956,7 +2043,9 @@ static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F,
     }
     // pass SA: reads history[hcur], writes history[1 - hcur]
     timer_begin(c, 4, hs);
-    int slots_a = launch_list_pass<StressBody<R>>(c, solid_args<R>(c, st, sd, F, S), hs, PASS_STRESS);
+    int slots_a = S.damage_on
+                      ? launch_list_pass<StressBody<R, true>>(c, solid_args<R>(c, st, sd, F, S), hs, PASS_STRESS)
+                      : launch_list_pass<StressBody<R, false>>(c, solid_args<R>(c, st, sd, F, S), hs, PASS_STRESS);
     timer_end(c, 4, hs);
     if ((rc = check_launch(c, "solid stress pass"))) return rc;
     int slots_s = 0;
@@ -2037,7 +2126,8 @@ static int solid_step_launch(sph_ctx* c, sph_state* st, sph_solid* sd, const sph
     int rc;
     DFluid F = make_fluid(c, P);
     DSolidP D = make_solid_params(c, P, S);
-    D.sur_on = sd->surrogate && sd->sur_x && sd->sur_y && sd->sur_flag && S->plastic;
+    // (the network answers the undamaged surface only)
+    D.sur_on = sd->surrogate && sd->sur_x && sd->sur_y && sd->sur_flag && S->plastic && !S->damage;
     D.harvest_on = sd->harvest_x && sd->harvest_y && sd->harvest_count && sd->harvest_cap > 0 && S->plastic;
     D.sur_tol = S->sur_tol > 0.0 ? S->sur_tol : 0.01;
     hipLaunchKernelGGL(k_solid_begin, dim3(1), dim3(256), 0, hs, st->scal, D, sd->cons, S->ncons);
@@ -2055,6 +2145,10 @@ extern "C" int sph_solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const sp
     if (S->nmat <= 0 || S->ncons < 0 || S->ncons > SPH_MAX_CONS)
         return set_err(c, SPH_EINVAL, "material or constraint count out of range%s: %lld", "", S->ncons);
     if (!(S->V0 > 0.0)) return set_err(c, SPH_EINVAL, "reference cell volume must be positive%s", "", 0);
+    if (S->damage && (!sd->damage[0] || !sd->damage[1] || (S->jc_failure && (!sd->jc_omega[0] || !sd->jc_omega[1]))))
+        return set_err(c, SPH_EINVAL, "damage model without its damage / omega buffers%s", "", 0);
+    if (S->damage && S->jc_failure && !(S->jcf_lc > 0.0))
+        return set_err(c, SPH_EINVAL, "the JC softening length must be positive%s", "", 0);
     if (S->hourglass > 0.0 && !(S->hg_rate > 0.0 && S->w_dx0 > 0.0))
         return set_err(c, SPH_EINVAL, "hourglass rate and W(dx0) must be positive%s", "", 0);
     if (S->sts && (S->sts_max_stages < 1 || S->sts_max_stages > 1024 || S->sts_stages < 0 ||

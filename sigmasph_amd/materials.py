@@ -17,7 +17,7 @@ import ctypes
 import math
 
 from ._native import (SphMaterial, EOS_LINEAR, EOS_TAIT, EOS_MG, PLASTIC_NONE, PLASTIC_LINEAR,
-                      PLASTIC_JC)
+                      PLASTIC_JC, DAMAGE_NONE, DAMAGE_THRESHOLD, DAMAGE_JC)
 
 
 class MaterialError(ValueError):
@@ -83,7 +83,13 @@ def normalize_name(name):
     return ALIASES.get(clean, clean)
 
 
-ENTRY_KEYS = {"name", "library", "density0", "specificHeat", "delta", "eos", "strength"}
+ENTRY_KEYS = {"name", "library", "density0", "specificHeat", "delta", "eos", "strength", "damage",
+              "spallPressure"}
+# the Johnson-Cook fracture card (derive_jc_damage, DAM004_johnson_cook.py:65-99):
+# key, accepted names, default (None: required)
+JCF_KEYS = (("D1", ("JCF_D1", "D1"), None), ("D2", ("JCF_D2", "D2"), None), ("D3", ("JCF_D3", "D3"), None),
+            ("D4", ("JCF_D4", "D4"), 0.0), ("D5", ("JCF_D5", "D5"), 0.0), ("EFMIN", ("JCF_EFMIN", "EFMIN"), 0.0),
+            ("u_f", ("JCF_UF", "failureDisplacement"), None))
 
 
 def _merge(base, over):
@@ -241,9 +247,58 @@ class Material:
             raise MaterialError("%s.eos: type %r is not part of this step (linear, tait, "
                                 "mie_gruneisen)" % (where, kind))
         self.c_p = math.sqrt((self.K + 4.0 * self.G / 3.0) / self.rho0)
+        self._damage(entry, eos, where)
         # reference_temperature / reference_energy (materials.py:429-445)
         self.T0 = self.jc["T0"] if self.plastic_kind == PLASTIC_JC else 293.0
         self.e0 = self.mg["e0"] if self.eos_kind == EOS_MG else 0.0
+
+    def _damage(self, entry, eos, where):
+        """The damage half of the material (materials.py:184-197, 351-425): the spall
+        cutoff (`spallPressure` on the entry or its EOS card) and the `damage` card.
+        Built: 'johnson_cook' fracture (on Johnson-Cook plasticity) and the threshold
+        spall model; 'cocks_ashby' and 'hjc' are refused."""
+        spall = entry.get("spallPressure")
+        if spall is None:
+            spall = eos.get("spallPressure")
+        self.spall_pressure = float(spall) if spall is not None else 0.0
+        self.damage_kind = DAMAGE_NONE
+        self.jcf = dict(D1=0.0, D2=0.0, D3=0.0, D4=0.0, D5=0.0, EFMIN=0.0, u_f=1.0)
+        card = entry.get("damage")
+        model = ""
+        if card is not None:
+            if not isinstance(card, dict):
+                raise MaterialError("%s.damage: expected an object" % where)
+            model = str(card.get("model") or "").lower()
+        if model in ("johnson_cook", "jc"):
+            if self.plastic_kind != PLASTIC_JC:
+                raise MaterialError("%s: damage model 'johnson_cook' needs plasticity model "
+                                    "'johnson_cook' (or 'hollomon')." % where)
+            for key, names, default in JCF_KEYS:
+                v = next((card[n] for n in names if card.get(n) is not None), default)
+                if v is None:
+                    raise MaterialError("%s.damage: Johnson-Cook failure requires parameter %r."
+                                        % (where, "failureDisplacement" if key == "u_f" else key))
+                try:
+                    self.jcf[key] = float(v)
+                except (TypeError, ValueError):
+                    raise MaterialError("%s.damage: Johnson-Cook failure parameter %r must be numeric, "
+                                        "got %r." % (where, key, v))
+            if not self.jcf["u_f"] > 0.0:
+                raise MaterialError("%s.damage: Johnson-Cook failureDisplacement must be positive, got %g."
+                                    % (where, self.jcf["u_f"]))
+            if self.jcf["EFMIN"] < 0.0:
+                raise MaterialError("%s.damage: Johnson-Cook EFMIN must be non-negative, got %g."
+                                    % (where, self.jcf["EFMIN"]))
+            self.damage_kind = DAMAGE_JC
+        elif model in ("cocks_ashby", "qamar", "hjc"):
+            raise MaterialError("%s.damage: model %r is not part of this build (johnson_cook and the "
+                                "spall threshold are)" % (where, model))
+        elif self.spall_pressure > 0.0 or model in ("threshold", "spall"):
+            self.damage_kind = DAMAGE_THRESHOLD
+            if card is not None and card.get("spallPressure") is not None:
+                self.spall_pressure = float(card["spallPressure"])
+        elif model:
+            raise MaterialError("%s.damage: unknown model %r" % (where, model))
 
     def row(self):
         m = SphMaterial()
@@ -258,6 +313,10 @@ class Material:
         (m.mg_C0, m.mg_C1, m.mg_C2, m.mg_C3, m.mg_C4, m.mg_C5, m.mg_c0_ref,
          m.mg_linear_exp) = g["C0"], g["C1"], g["C2"], g["C3"], g["C4"], g["C5"], g["c0_ref"], g["linear_exp"]
         m.eos_kind, m.plastic_kind = self.eos_kind, self.plastic_kind
+        m.damage_kind, m.spall_p = self.damage_kind, self.spall_pressure
+        f = self.jcf
+        (m.jcf_D1, m.jcf_D2, m.jcf_D3, m.jcf_D4, m.jcf_D5, m.jcf_efmin,
+         m.jcf_uf) = f["D1"], f["D2"], f["D3"], f["D4"], f["D5"], f["EFMIN"], f["u_f"]
         return m
 
     def describe(self):

@@ -487,6 +487,11 @@ class ParticleSystem(SceneLayout):
         dev, R = self.device, self.real
         z = lambda *shape, dt=R: torch.zeros(*shape, dtype=dt, device=dev)
         self._hist = [{"sig": z(n, 6), "eps_p": z(n), "e_int": z(n)} for _ in range(2)]
+        # the damage history (D, JC omega) only for a deck with a damage model
+        self.has_damage = any(m.damage_kind != N.DAMAGE_NONE for m in self.materials.values())
+        if self.has_damage:
+            for h in self._hist:
+                h["damage"], h["jc_omega"] = z(n), z(n)
         self._sbufs = [{"T": z(n), "de_int": z(n), "eos_id": z(n, dt=torch.int32),
                         "bc_flag": z(n, dt=torch.int32), "bc_id": z(n, dt=torch.int32),
                         "bc_disp": z(n, 4)} for _ in range(2)]
@@ -505,6 +510,9 @@ class ParticleSystem(SceneLayout):
         for k in range(2):
             for name in ("sig", "eps_p", "e_int"):
                 getattr(sd, name)[k] = self._hist[k][name].data_ptr()
+            if self.has_damage:
+                sd.damage[k] = self._hist[k]["damage"].data_ptr()
+                sd.jc_omega[k] = self._hist[k]["jc_omega"].data_ptr()
             for name in ("T", "de_int", "eos_id", "bc_flag", "bc_id", "bc_disp"):
                 getattr(sd, name)[k] = self._sbufs[k][name].data_ptr()
             sd.hg_y[k] = self._sderived["hg_y%d" % k].data_ptr()
@@ -519,6 +527,9 @@ class ParticleSystem(SceneLayout):
         self.sigma_dev = sym_field(lambda: H()["sig"], "sigma_dev")
         self.eps_plastic = Field(lambda: H()["eps_p"], 1, "eps_plastic")
         self.e_int = Field(lambda: H()["e_int"], 1, "e_int")
+        if self.has_damage:
+            self.damage = Field(lambda: H()["damage"], 1, "damage")
+            self.jc_omega = Field(lambda: H()["jc_omega"], 1, "jc_omega")
         self.temperature = Field(lambda: B()["T"], 1, "temperature")
         self.de_int = Field(lambda: B()["de_int"], 1, "de_int")
         self.eos_id = Field(lambda: B()["eos_id"], 1, "eos_id")
@@ -556,6 +567,9 @@ class ParticleSystem(SceneLayout):
             out["eroded"] = np.zeros(int(mask.sum()), dtype=np.int32)
             out["e_int"] = self.e_int.to_numpy()[mask]
             out["temperature"] = self.temperature.to_numpy()[mask]
+            if getattr(self, "has_damage", False):
+                out["damage"] = self.damage.to_numpy()[mask]
+                out["jc_omega"] = self.jc_omega.to_numpy()[mask]
         return out
 
     def _live(self):

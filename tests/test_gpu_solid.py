@@ -44,6 +44,20 @@ SCENES = {
                                             div_velocity=(300.0, -150.0, 0.0), jitter=0.02,
                                             extra={"Hourglass": {"alpha": 8.0,
                                                                  "superTimeStepping": {"mode": "rkl"}}}),
+    # Johnson-Cook fracture on stainless steel with a card that initiates after two steps
+    # of plastic flow and then softens (D ~ 0.1 after five), gripped and stretched
+    "jc_fracture": lambda: solid_block(
+        n=(24, 8, 1), grips=0.05, ramp_time=1e-4, div_velocity=(300.0, -150.0, 0.0), jitter=0.02,
+        material={"name": "steel", "library": "stainless_steel",
+                  "damage": {"model": "johnson_cook", "D1": 3e-3, "D2": 3e-3, "D3": -1.5, "D4": 0.01,
+                             "D5": 0.5, "failureDisplacement": 0.05}}),
+    # the spall threshold on linear-hardening steel under triaxial expansion
+    "spall_threshold": lambda: solid_block(
+        n=(20, 10, 1), jc=False, div_velocity=(100.0, 100.0, 0.0), jitter=0.02,
+        material={"name": "steel", "density0": 7.9e-06, "specificHeat": 500.0, "eos": {"type": "linear"},
+                  "strength": {"youngsModulus": 193.0, "poissonRatio": 0.3,
+                               "plasticity": {"model": "linear", "yieldStress": 0.31, "hardeningModulus": 1.0}},
+                  "spallPressure": 0.8}),
     "plane_strain_noshift": lambda: solid_block(n=(20, 10, 1), pst=False, viscosity=0.0,
                                                 div_velocity=(-200.0, 250.0, 0.0), jitter=0.03),
     # nine specimens with their own Johnson-Cook rows and 18 grips (past 16 the grip
@@ -115,6 +129,9 @@ def compare(ps, st, s, tol, eps=0.0):
              ("grad_v", s.by_pid("grad_v").reshape(-1, 3, 3))]
     if st.q_l2 > 0.0:
         pairs.append(("q_visc", s.by_pid("q")))
+    if s.damage is not None:
+        pairs += [("damage", s.by_pid("damage")), ("jc_omega", s.by_pid("jc_omega"))]
+        floors.update({"damage": 1.0, "jc_omega": 1.0})
     order = np.argsort(ps.pid.to_numpy(), kind="stable")
     errs = {}
     for name, ref in pairs:
@@ -153,12 +170,17 @@ def test_five_steps_fp64(gpu, name):
         assert s.scal[N.SPH_S_PLASTIC_WORK] > 0.0      # the return map ran
     if st.hg_sts:
         assert s.scal[N.SPH_S_STS_STAGES] >= 1
+    if name in ("jc_fracture", "spall_threshold"):
+        assert s.damage.max() > 0.0                      # the damage model acted
+        if name == "jc_fracture":
+            assert s.jc_omega.max() > 1.0 and 0.0 < s.damage.max() < 1.0
     if name == "plane_strain_rkl":
         assert st.solid_params().sts == N.SPH_STS_RKL and s.scal[N.SPH_S_STS_STAGES] >= 2
 
 
 @pytest.mark.parametrize("name", ["plane_strain_grips", "axisymmetric_grips", "block3d_explicit",
-                                  "axisymmetric_shock", "taylor3d", "plane_strain_rkl"])
+                                  "axisymmetric_shock", "taylor3d", "plane_strain_rkl", "jc_fracture",
+                                  "spall_threshold"])
 def test_three_steps_mixed(gpu, name):
     ps, solver, st, s = run_pair(SCENES[name](), 3, precision="mixed")
     compare(ps, st, s, 2e-4, eps=2.0 ** -24)

@@ -1,0 +1,143 @@
+"""The damage models of the oracle (oracle/sph_oracle.c): the spall threshold and the
+Johnson-Cook fracture criterion with regularised softening (DAM004_johnson_cook.py),
+and what the shared damage D does to the step (SOLID.py:2849-2900, 3216-3382).
+
+Pinned by what the model's construction fixes rather than by its formula:
+- the increment is additive: one step of d_eps_p = a + b leaves (omega, D) where a step
+  of a then a step of b does (the part of an increment past omega = 1 softens, the rest
+  initiates), at any split, including splits across omega = 1;
+- after initiation D reaches 1 after a plastic opening L_c Delta eps_p = u_f, whatever
+  L_c (the regularisation's point);
+- eps_f reduces to D1 + D2 at zero triaxiality, to D1 at D2 = 0, and its rate and
+  temperature factors are 1 at a rate ratio below 1 and at T* <= 0;
+- tension is carried in proportion 1 - D and compression untouched; a threshold
+  material in tension beyond its cutoff is fully damaged and carries no tension;
+- a plastic particle at D = 1 returns to the origin with d_eps_p = sigma_vM / 3G; at
+  0 < D < 1 the J2 return lands on the scaled surface (1 - D)(sigma_y0 + H eps_p).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sigmasph_amd import _native as N
+from sigmasph_amd.materials import resolve
+from sigmasph_amd.scenes import solid_block
+from helpers import solid_layout, solid_oracle_state
+
+
+JCF = {"model": "johnson_cook", "D1": 0.05, "D2": 3.44, "D3": -2.12, "D4": 0.002, "D5": 0.61,
+       "failureDisplacement": 0.02}
+
+
+def _row(damage=JCF, **kw):
+    m = resolve([dict({"name": "s", "library": "stainless_steel", "damage": damage}, **kw)])["s"]
+    return m.row()
+
+
+def test_jc_fracture_increment_is_additive():
+    M = _row()
+    rng = np.random.default_rng(0)
+    for omega0 in (0.0, 0.5, 0.97, 1.2):
+        for _ in range(20):
+            a, b = rng.uniform(0.0, 0.3, 2)
+            eta, rr, ts = rng.uniform(-0.5, 1.0), rng.uniform(0.5, 50.0), rng.uniform(-0.2, 1.2)
+            one = O.jc_damage_step(M, a + b, eta, rr, ts, 0.1, omega0, 0.0)
+            om, D = O.jc_damage_step(M, a, eta, rr, ts, 0.1, omega0, 0.0)
+            two = O.jc_damage_step(M, b, eta, rr, ts, 0.1, om, D)
+            assert one[0] == pytest.approx(two[0], rel=1e-13)
+            if one[1] < 1.0:
+                assert one[1] == pytest.approx(two[1], rel=1e-12, abs=1e-15)
+    assert O.jc_damage_step(M, 0.0, 0.3, 1.0, 0.0, 0.1, 0.4, 0.2) == (0.4, 0.2)
+    assert O.jc_damage_step(M, -1e-3, 0.3, 1.0, 0.0, 0.1, 0.4, 0.2) == (0.4, 0.2)
+
+
+@pytest.mark.parametrize("l_c", [0.05, 0.1, 0.4])
+def test_full_damage_after_an_opening_of_u_f(l_c):
+    M = _row()
+    u_f = M.jcf_uf
+    om, D = 1.0, 0.0                                   # initiated
+    need = u_f / l_c                                   # plastic strain to D = 1
+    steps = 40
+    for _ in range(steps - 1):
+        om, D = O.jc_damage_step(M, need / steps, 0.3, 1.0, 0.0, l_c, om, D)
+    assert D == pytest.approx(1.0 - 1.0 / steps, rel=1e-12)
+    om, D = O.jc_damage_step(M, need / steps, 0.3, 1.0, 0.0, l_c, om, D)
+    assert D == pytest.approx(1.0, rel=1e-12)
+    om, D = O.jc_damage_step(M, need, 0.3, 1.0, 0.0, l_c, om, D)
+    assert D == 1.0                                    # saturates
+
+
+def test_failure_strain_special_cases():
+    """omega after a small increment e is e / eps_f; eps_f's factors checked where the
+    model fixes them."""
+    e = 1e-4
+    ef = lambda M, eta, rr, ts: e / O.jc_damage_step(M, e, eta, rr, ts, 0.1, 0.0, 0.0)[0]
+    M = _row()
+    assert ef(M, 0.0, 1.0, 0.0) == pytest.approx(0.05 + 3.44, rel=1e-13)     # zero triaxiality
+    assert ef(M, 0.0, 0.2, 0.0) == pytest.approx(0.05 + 3.44, rel=1e-13)     # rate ratio < 1
+    assert ef(M, 0.0, 1.0, -0.5) == pytest.approx(0.05 + 3.44, rel=1e-13)    # T* < 0
+    assert ef(M, 0.0, 1.0, 7.0) == pytest.approx(ef(M, 0.0, 1.0, 1.0), rel=1e-13)   # T* > 1
+    hi, lo = ef(M, 0.6, 1.0, 0.0), ef(M, 0.3, 1.0, 0.0)
+    assert hi < lo < 3.49                                # D3 < 0: triaxial tension fails sooner
+    assert ef(M, 0.3, math.e, 0.0) == pytest.approx(lo * (1.0 + 0.002), rel=1e-13)
+    assert ef(M, 0.3, 1.0, 0.5) == pytest.approx(lo * (1.0 + 0.61 * 0.5), rel=1e-13)
+    M1 = _row(dict(JCF, D2=0.0))
+    assert ef(M1, 0.9, 1.0, 0.0) == pytest.approx(0.05, rel=1e-13)
+    Mf = _row(dict(JCF, D1=-1.0, D2=0.0, EFMIN=0.02))
+    assert ef(Mf, 0.0, 1.0, 0.0) == pytest.approx(0.02, rel=1e-13)          # EFMIN floor
+    M0 = _row(dict(JCF, D1=-1.0, D2=0.0, EFMIN=0.0))
+    assert ef(M0, 0.0, 1.0, 0.0) == pytest.approx(1e-6, rel=1e-13)          # the 1e-6 floor
+
+
+def _damaged_state(material, n=(12, 6, 1), D=None):
+    sc = solid_block(n=n, material=material, jitter=0.05, pst=False)
+    lay, st, P, S = solid_layout(sc)
+    assert S.damage == 1
+    s = solid_oracle_state(lay, st)
+    O.sort_only(lay.dims, s)
+    if D is not None:
+        s.damage[:] = D
+    return lay, st, P, S, s
+
+
+def test_tension_split_and_spall_threshold():
+    mat = {"name": "steel", "library": "stainless_steel", "spallPressure": 1.0}
+    lay, st, P, S, s = _damaged_state(mat, D=0.25)
+    rng = np.random.default_rng(3)
+    s.V *= 1.0 + 0.004 * rng.uniform(-1.0, 1.0, s.n)          # |p| up to ~0.7 GPa
+    rho = s.m / s.V
+    K, rho0 = st.default_material.K, st.default_material.rho0
+    p_raw = K * (rho / rho0 - 1.0)
+    O.solid_phase("stress_accelerations", lay.dims, P, S, s)
+    np.testing.assert_allclose(s.p, np.where(p_raw > 0, p_raw, 0.75 * p_raw), rtol=1e-12, atol=1e-15)
+    assert np.all(s.damage == 0.25)
+    s.V[:3] *= 1.02                                            # -p ~ 3 GPa > 1 GPa
+    O.solid_phase("stress_accelerations", lay.dims, P, S, s)
+    assert np.all(s.damage[:3] == 1.0) and np.all(s.p[:3] == 0.0)
+    assert np.all(s.damage[3:] == 0.25)
+
+
+def test_damaged_return_maps():
+    mat = {"name": "steel", "density0": 7.9e-06, "eos": {"type": "linear"},
+           "strength": {"youngsModulus": 193.0, "poissonRatio": 0.3,
+                        "plasticity": {"model": "linear", "yieldStress": 0.3, "hardeningModulus": 1.0}},
+           "spallPressure": 100.0}
+    lay, st, P, S, s = _damaged_state(mat)
+    M = st.default_material
+    rng = np.random.default_rng(4)
+    q = rng.normal(size=(s.n, 6)) * 0.5                       # far outside the surface
+    q[:, :3] -= q[:, :3].mean(axis=1, keepdims=True)
+    s.sig[:] = q
+    s.grad_v[:] = 0.0
+    s.eps_p[:] = 0.01
+    s.damage[:] = np.where(np.arange(s.n) % 2 == 0, 1.0, 0.4)
+    vm0 = np.sqrt(1.5 * (np.sum(q[:, :3] ** 2, 1) + 2 * np.sum(q[:, 3:] ** 2, 1)))
+    O.solid_phase("update_stress", lay.dims, P, S, s)
+    full = s.damage == 1.0
+    assert np.all(s.sig[full] == 0.0)
+    np.testing.assert_allclose(s.eps_p[full], 0.01 + vm0[full] / (3.0 * M.G), rtol=1e-13)
+    vm1 = np.sqrt(1.5 * (np.sum(s.sig[~full, :3] ** 2, 1) + 2 * np.sum(s.sig[~full, 3:] ** 2, 1)))
+    np.testing.assert_allclose(vm1, 0.6 * (0.3 + 1.0 * s.eps_p[~full]), rtol=1e-12)
+    np.testing.assert_allclose(s.d_eps_prod[full], vm0[full] / (3.0 * M.G), rtol=1e-13)

@@ -99,6 +99,7 @@ extern "C" {
 #define SPH_PLASTIC_JC       2  /* Johnson-Cook (MAT002)                                */
 #define SPH_DAMAGE_NONE      0
 #define SPH_DAMAGE_THRESHOLD 1  /* D = 1 once the tension -p exceeds spall_p (materials.py:416) */
+#define SPH_DAMAGE_CA        2  /* Cocks-Ashby void growth, Chu-Needleman nucleation (DAM001) */
 #define SPH_DAMAGE_JC        4  /* Johnson-Cook fracture with regularised softening (DAM004) */
 
 /* Cell lattice.  A particle at x sits in cell c = floor((x - origin) / cell) per axis.
@@ -200,6 +201,12 @@ typedef struct {
     double jcf_D1, jcf_D2, jcf_D3, jcf_D4, jcf_D5, jcf_efmin, jcf_uf;
     int32_t damage_kind;                 /* SPH_DAMAGE_*  */
     int32_t pad_;
+    /* the Cocks-Ashby card (derive_cocks_ashby, DAM001_cocks_ashby.py:78-204): growth c1,
+     * c2, c4, c5, degradation a1, nucleation fn0, sigma_hM, sigma_hS, rate exponent m,
+     * porosity cap f_max, rate coupling mode (0 effective .. 4 unconstrained_stress) */
+    double ca_c1, ca_c2, ca_c4, ca_c5, ca_a1, ca_fn0, ca_shm, ca_shs, ca_m, ca_fmax;
+    int32_t ca_rate_mode;
+    int32_t pad2_;
 } sph_material;
 
 /* Parameters of the hypoelastic solid step (the HypoElasticSolver constructor,
@@ -238,6 +245,8 @@ typedef struct {
     double  jcf_lc;          /* the JC softening length L_c = dx0 (SOLID.py:1675-1682)       */
     int32_t damage;          /* some material has a damage model: sph_solid.damage is live   */
     int32_t jc_failure;      /* some material has the JC fracture card: jc_omega is live     */
+    int32_t cocks_ashby;     /* some material has the Cocks-Ashby card: porosity is live     */
+    int32_t pad2_;
 } sph_solid_params;
 
 /* sph_solid_params.sts: how the s passes of a step are spent (hourglass_sts,
@@ -301,6 +310,8 @@ typedef struct {
      * Johnson-Cook initiation accumulator omega (DAM004) */
     void*    damage[2];      /* [n] D in [0, 1]                                                */
     void*    jc_omega[2];    /* [n] omega = Sum d_eps_p / eps_f                                */
+    void*    porosity[2];    /* [n] Cocks-Ashby void fraction f (NULL without the card)        */
+    void*    sigma_h_peak[2];/* [n] its peak hydrostatic tension max_t (-p)                    */
 } sph_solid;
 
 /* ---- context ---------------------------------------------------------------------- */

@@ -34,7 +34,8 @@ class OrState(ctypes.Structure):
                 ("eos_id", _ip), ("bc_flag", _ip), ("bc_id", _ip),
                 ("bc_disp", _dp), ("grad_v", _dp), ("cons", _dp),
                 ("mat", ctypes.c_void_p), ("q", _dp), ("av_mu", _dp),
-                ("damage", _dp), ("jc_omega", _dp), ("d_eps_prod", _dp)]
+                ("damage", _dp), ("jc_omega", _dp), ("d_eps_prod", _dp),
+                ("porosity", _dp), ("sigma_h_peak", _dp)]
 
 
 def build():
@@ -94,6 +95,12 @@ def lib():
         L.oracle_mg_pressure.restype = ctypes.c_double
         L.oracle_cold_energy.argtypes = [vp, ctypes.c_double, ctypes.c_int]
         L.oracle_cold_energy.restype = ctypes.c_double
+        L.oracle_erf_as.argtypes = [ctypes.c_double]
+        L.oracle_erf_as.restype = ctypes.c_double
+        L.oracle_ca_nucleation.argtypes = [vp, ctypes.c_double]
+        L.oracle_ca_nucleation.restype = ctypes.c_double
+        L.oracle_ca_growth.restype = None
+        L.oracle_jc_damage_step.restype = None
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_max_threads.restype = ctypes.c_int
         _lib = L
@@ -111,7 +118,7 @@ class OracleState:
     FIELDS1 = ("V", "m", "p", "lam", "div", "de_visc")
     SOLID_D = {"sig": 6, "eps_p": 0, "e_int": 0, "T": 0, "de_int": 0, "de_hg": 0,
                "bc_disp": 3, "grad_v": 9, "q": 0, "av_mu": 0, "d_eps_prod": 0}
-    DAMAGE_D = ("damage", "jc_omega")   # allocated for a deck with a damage model
+    DAMAGE_D = ("damage", "jc_omega", "porosity", "sigma_h_peak")   # for a deck with a damage model
     SOLID_I = ("eos_id", "bc_flag", "bc_id")
 
     def __init__(self, n):
@@ -125,7 +132,7 @@ class OracleState:
         self.obj = np.zeros(n, dtype=np.int32)
         self.scal = np.zeros(32)
         self.solid = False
-        self.damage = self.jc_omega = None
+        self.damage = self.jc_omega = self.porosity = self.sigma_h_peak = None
 
     def make_solid(self, materials, ncons=0, damage=False):
         """Attach the solid fields (zero stress, e = e0, T = T0 of each row's material,
@@ -226,6 +233,22 @@ def jc_damage_step(mat, d_eps_p, eta, rate_ratio, t_star, l_c, omega, D):
                                 ctypes.c_double(rate_ratio), ctypes.c_double(t_star), ctypes.c_double(l_c),
                                 ctypes.byref(om), ctypes.byref(d))
     return om.value, d.value
+
+
+def ca_nucleation(mat, sh_peak):
+    return lib().oracle_ca_nucleation(ctypes.byref(mat), ctypes.c_double(sh_peak))
+
+
+def ca_growth(mat, sigma_h, tau_f, dot_eps_p, div_v, f, fn):
+    fe, fs = ctypes.c_double(), ctypes.c_double()
+    lib().oracle_ca_growth(ctypes.byref(mat), ctypes.c_double(sigma_h), ctypes.c_double(tau_f),
+                           ctypes.c_double(dot_eps_p), ctypes.c_double(div_v), ctypes.c_double(f),
+                           ctypes.c_double(fn), ctypes.byref(fe), ctypes.byref(fs))
+    return fe.value, fs.value
+
+
+def erf_as(x):
+    return lib().oracle_erf_as(ctypes.c_double(x))
 
 
 def solid_reduce(s, which):

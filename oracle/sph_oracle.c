@@ -54,6 +54,7 @@ typedef struct {
     double *av_mu;                       /* [n] max_j |mu_ij| of Monaghan's Pi */
     double *damage, *jc_omega;           /* [n] damage D, JC omega (NULL without damage) */
     double *d_eps_prod;                  /* [n] the return map's production of the step */
+    double *porosity, *sigma_h_peak;     /* [n] Cocks-Ashby f and peak tension (NULL without) */
 } or_state;
 
 typedef struct {
@@ -171,6 +172,8 @@ int64_t oracle_sort(const sph_dims* d, or_state* s, or_grid* g) {
         or_permute_d(s->e_int, perm, n, 1);
         if (s->damage) or_permute_d(s->damage, perm, n, 1);
         if (s->jc_omega) or_permute_d(s->jc_omega, perm, n, 1);
+        if (s->porosity) or_permute_d(s->porosity, perm, n, 1);
+        if (s->sigma_h_peak) or_permute_d(s->sigma_h_peak, perm, n, 1);
         or_permute_d(s->T, perm, n, 1);
         or_permute_d(s->de_int, perm, n, 1);
         or_permute_d(s->de_hg, perm, n, 1);
@@ -1207,11 +1210,67 @@ static void or_ale_hist_task(void* c, int64_t i, int64_t j, int mir, const doubl
         double flux = (rho_j * djg) * ej - (rho_i * dig) * ei;
         t->acc[9] += wj * (flux - ei * drd) / rho_i;
     }
+    if (t->S->damage && t->S->cocks_ashby) {        /* the porosity, every pair (1865-1885) */
+        double fi = s->porosity[i], fj = s->porosity[j];
+        double flux = (rho_j * djg) * fj - (rho_i * dig) * fi;
+        t->acc[12] += wj * (flux - fi * drd) / rho_i;
+    }
     if (t->S->damage && t->S->jc_failure && s->eos_id[i] == s->eos_id[j]) {
         double fi[2] = { s->jc_omega[i], s->damage[i] }, fj[2] = { s->jc_omega[j], s->damage[j] };
         for (int k = 0; k < 2; ++k) {
             double flux = (rho_j * djg) * fj[k] - (rho_i * dig) * fi[k];
             t->acc[10 + k] += wj * (flux - fi[k] * drd) / rho_i;
+        }
+    }
+}
+
+/* erf by Abramowitz & Stegun 7.1.26, as the reference's nucleation kernel evaluates it
+ * (erf_approx, DAM001_cocks_ashby.py:207-226) */
+double oracle_erf_as(double x) {
+    double ax = fabs(x), t = 1.0 / (1.0 + 0.3275911 * ax);
+    double poly = t * (0.254829592 + t * (-0.284496736 + t * (1.421413741 + t * (-1.453152027 + t * 1.061405429))));
+    double r = 1.0 - poly * exp(-ax * ax);
+    return x < 0.0 ? -r : r;
+}
+
+/* chu_needleman_nucleation (DAM001:229-240) */
+double oracle_ca_nucleation(const sph_material* M, double sh_peak) {
+    if (!(M->ca_fn0 > 0.0 && M->ca_shs > 0.0)) return 0.0;
+    double arg = (sh_peak - M->ca_shm) / (M->ca_shs * 1.41421356237);
+    return fmax(0.0, 0.5 * M->ca_fn0 * (1.0 + oracle_erf_as(arg)));
+}
+
+/* cocks_ashby_growth_ext (DAM001:243-317): (dot f_e, dot f_s) */
+void oracle_ca_growth(const sph_material* M, double sigma_h, double tau_f, double dot_eps_p, double div_v,
+                      double f, double fn, double* dot_fe, double* dot_fs) {
+    double m = M->ca_m;
+    int mode = M->ca_rate_mode;
+    double tau_safe = fmax(1.0e-6, tau_f);
+    double f_safe = fmin(0.99, fmax(0.0, f));
+    double f_star = f_safe;
+    if (sigma_h <= 0.0) f_star = fmax(0.0, f_safe - fn);
+    double drive = dot_eps_p;
+    if (sigma_h > 0.0) {
+        if (mode == 2) drive = fmax(dot_eps_p, fmax(0.0, div_v));                       /* volumetric */
+        else if (mode == 3) drive = fmax(dot_eps_p, 1.0);                               /* unconstrained */
+        else if (mode == 4) drive = fmax(dot_eps_p, 0.001 * pow(fmax(0.0, sigma_h / tau_safe), m));
+    }
+    double m_term = (m - 0.5) / (m + 0.5);
+    double sinh_arg = fmin(50.0, fmax(-50.0, M->ca_c2 * m_term * (sigma_h / tau_safe)));
+    double sinh_val = 0.5 * (exp(sinh_arg) - exp(-sinh_arg));
+    double one_minus_f = fmax(0.01, 1.0 - f_star);
+    double term_void = 1.0 / pow(one_minus_f, m) - one_minus_f;
+    *dot_fe = M->ca_c1 * sinh_val * term_void * drive;
+    *dot_fs = 0.0;
+    if (sigma_h > 0.0 && f_star > 0.0) {
+        double ratio_pow = pow(fmax(0.0, M->ca_c5 * (sigma_h / tau_safe)), m);
+        double denom_f = fmax(0.01, 1.0 - f_safe);
+        double kappa = fmin(0.95, fmax(0.0, (M->ca_c4 * f_star / (denom_f * (m + 1.0))) * ratio_pow));
+        if (mode == 1) {                                                                /* volumetric_fs */
+            *dot_fs = (M->ca_c4 * f_star / (m + 1.0)) * ratio_pow * fmax(0.0, div_v);
+        } else {
+            double eff = (drive + fmax(0.0, *dot_fe) / denom_f) / (1.0 - kappa);
+            *dot_fs = (M->ca_c4 * f_star / (m + 1.0)) * ratio_pow * eff;
         }
     }
 }
@@ -1241,6 +1300,7 @@ static void or_update_stress(const sph_dims* d, const sph_fluid_params* P,
     double* dsig = (double*)malloc(sizeof(double) * 9 * n);
     double* dep = (double*)calloc(n, sizeof(double));
     double* djc = (double*)calloc(2 * n, sizeof(double));
+    double* dpor = (double*)calloc(n, sizeof(double));
     double* pw = (double*)calloc(n, sizeof(double));
     int ale = P->pst_enabled && S->ale_history;
     int dmg = S->damage && s->damage;
@@ -1273,6 +1333,7 @@ static void or_update_stress(const sph_dims* d, const sph_fluid_params* P,
             dep[i] = t.acc[9];
             djc[2 * i] = t.acc[10];
             djc[2 * i + 1] = t.acc[11];
+            dpor[i] = t.acc[12];
         }
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) dsig[9 * i + 3 * a + b] = sd[a][b];
@@ -1294,6 +1355,8 @@ static void or_update_stress(const sph_dims* d, const sph_fluid_params* P,
             s->jc_omega[i] = fmax(0.0, s->jc_omega[i] + dt * djc[2 * i]);
             s->damage[i] = fmin(1.0, fmax(0.0, s->damage[i] + dt * djc[2 * i + 1]));
         }
+        if (dmg && ale && S->cocks_ashby)                      /* the porosity (2791-2793) */
+            s->porosity[i] = fmax(0.0, s->porosity[i] + dt * dpor[i]);
         double dp = 0.0;
         if (S->plastic) {
             /* the yield surface scaled by s_d = 1 - D; at s_d = 0 a plastic particle returns
@@ -1325,7 +1388,7 @@ static void or_update_stress(const sph_dims* d, const sph_fluid_params* P,
     double w = 0.0;
     for (int64_t i = 0; i < n; ++i) w += pw[i];
     s->scal[SPH_S_PLASTIC_WORK] += w;
-    free(dsig); free(dep); free(djc); free(pw);
+    free(dsig); free(dep); free(djc); free(dpor); free(pw);
 }
 
 /* compute_ale_transport_e_task (SOLID.py:1949-2045): acc[0] advection, acc[1] diffusion */
@@ -1590,6 +1653,27 @@ static void or_stress_accelerations(const sph_dims* d, const sph_fluid_params* P
              * of the returned deviator, then the unilateral split (SOLID.py:3216-3382) */
             if (M->damage_kind == SPH_DAMAGE_THRESHOLD && M->spall_p > 0.0 && -p_raw > M->spall_p)
                 s->damage[i] = 1.0;
+            if (S->cocks_ashby && M->damage_kind == SPH_DAMAGE_CA) {
+                /* Cocks-Ashby (3224-3268) */
+                double sh = -p_raw, sh_old = s->sigma_h_peak[i], sh_new = fmax(sh_old, sh);
+                s->sigma_h_peak[i] = sh_new;
+                double fn_old = oracle_ca_nucleation(M, sh_old), fn_new = oracle_ca_nucleation(M, sh_new);
+                double df_n = fmax(0.0, fn_new - fn_old);
+                const double* q = s->sig + 6 * i;
+                double vm = sqrt(1.5 * (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]
+                                        + 2.0 * (q[3] * q[3] + q[4] * q[4] + q[5] * q[5])));
+                double tau_f = fmax(vm, M->sigma_y0);
+                const double* G = s->grad_v + 9 * i;
+                double fe, fs;
+                oracle_ca_growth(M, sh, tau_f, s->d_eps_prod[i] / dt, G[0] + G[4] + G[8], s->porosity[i],
+                                 fn_new, &fe, &fs);
+                double f = s->porosity[i] + df_n + (fe + fs) * dt;
+                f = fmin(M->ca_fmax, fmax(0.0, f));
+                s->porosity[i] = f;
+                double th = tanh(fmin(20.0, M->ca_a1 * fmax(0.0, f)));     /* degradation_factor */
+                s->damage[i] = fmin(1.0, th);
+                if (M->spall_p > 0.0 && -p_raw > M->spall_p) s->damage[i] = 1.0;
+            }
             if (S->jc_failure && M->damage_kind == SPH_DAMAGE_JC && s->d_eps_prod[i] > 0.0) {
                 const double* q = s->sig + 6 * i;
                 double vm = sqrt(1.5 * (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]

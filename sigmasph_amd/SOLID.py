@@ -149,6 +149,7 @@ class SolidSettings:
         # (damage_elastic, SOLID.py:459-465), which this build does not do.
         self.has_damage = any(m.damage_kind != N.DAMAGE_NONE for m in mats)
         self.has_jc_failure = any(m.damage_kind == N.DAMAGE_JC for m in mats)
+        self.has_cocks_ashby = any(m.damage_kind == N.DAMAGE_CA for m in mats)
         for m in mats:
             if m.damage_kind != N.DAMAGE_NONE and m.plastic_kind == N.PLASTIC_NONE:
                 raise NotImplementedError("material %r: damage on a material without plasticity is "
@@ -188,6 +189,7 @@ class SolidSettings:
         S.ncons = self.ncons
         S.damage = int(self.has_damage)
         S.jc_failure = int(self.has_jc_failure)
+        S.cocks_ashby = int(self.has_cocks_ashby)
         S.jcf_lc = self.jcf_lc
         return S
 

@@ -69,7 +69,7 @@ SPH_TUNE_LIST_CAP = 9
 # material kinds
 EOS_LINEAR, EOS_TAIT, EOS_MG = 0, 1, 3
 PLASTIC_NONE, PLASTIC_LINEAR, PLASTIC_JC = 0, 1, 2
-DAMAGE_NONE, DAMAGE_THRESHOLD, DAMAGE_JC = 0, 1, 4
+DAMAGE_NONE, DAMAGE_THRESHOLD, DAMAGE_CA, DAMAGE_JC = 0, 1, 2, 4
 
 _d3 = ctypes.c_double * 3
 _i3 = ctypes.c_int32 * 3
@@ -157,7 +157,10 @@ class SphMaterial(ctypes.Structure):
         ("eos_kind", ctypes.c_int32), ("plastic_kind", ctypes.c_int32)] + [
         (n, ctypes.c_double) for n in ("spall_p", "jcf_D1", "jcf_D2", "jcf_D3", "jcf_D4", "jcf_D5",
                                        "jcf_efmin", "jcf_uf")] + [
-        ("damage_kind", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+        ("damage_kind", ctypes.c_int32), ("pad_", ctypes.c_int32)] + [
+        (n, ctypes.c_double) for n in ("ca_c1", "ca_c2", "ca_c4", "ca_c5", "ca_a1", "ca_fn0", "ca_shm",
+                                       "ca_shs", "ca_m", "ca_fmax")] + [
+        ("ca_rate_mode", ctypes.c_int32), ("pad2_", ctypes.c_int32)]
 
 
 class SphSolidParams(ctypes.Structure):
@@ -167,7 +170,8 @@ class SphSolidParams(ctypes.Structure):
         "kernel_correction", "jaumann", "ale_history", "sts", "sts_stages", "sts_max_stages",
         "sts_refresh", "dt_pair_mode", "visc_work", "thermal", "plastic", "nmat", "ncons",
         "pad_")] + [("sur_tol", ctypes.c_double), ("sts_safety", ctypes.c_double),
-                    ("jcf_lc", ctypes.c_double), ("damage", ctypes.c_int32), ("jc_failure", ctypes.c_int32)]
+                    ("jcf_lc", ctypes.c_double), ("damage", ctypes.c_int32), ("jc_failure", ctypes.c_int32),
+                    ("cocks_ashby", ctypes.c_int32), ("pad2_", ctypes.c_int32)]
 
 
 _vp2 = ctypes.c_void_p * 2
@@ -189,7 +193,7 @@ class SphSolid(ctypes.Structure):
         ("sur_x", ctypes.c_void_p), ("sur_y", ctypes.c_void_p), ("sur_flag", ctypes.c_void_p),
         ("harvest_x", ctypes.c_void_p), ("harvest_y", ctypes.c_void_p),
         ("harvest_cap", ctypes.c_int64), ("harvest_count", ctypes.c_void_p),
-        ("damage", _vp2), ("jc_omega", _vp2),
+        ("damage", _vp2), ("jc_omega", _vp2), ("porosity", _vp2), ("sigma_h_peak", _vp2),
     ]
 
 

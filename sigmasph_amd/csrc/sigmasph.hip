@@ -235,8 +235,9 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_rank(int n, DScratch s, const int
 }
 
 struct PermField { const void* src; void* dst; int bytes; };
+#define PERM_MAX_FIELDS 28
 struct PermTable {
-    PermField f[24];
+    PermField f[PERM_MAX_FIELDS];
     int nf;
     int real_bytes;                        // 8: fp64, 4: mixed
     const void *m, *V, *V_lo;              // source mass / volume (V_lo: mixed only)

@@ -45,6 +45,7 @@ struct DSolidP {
     int sur_on, harvest_on;              // the surrogate answers / plain returns are harvested
     double sur_tol;
     int damage_on, jcf_on, transport_jcf;   // damage D live; JC fracture omega live; moved by the shift
+    int ca_on, transport_ca;             // Cocks-Ashby porosity live; moved by the shift
     double jcf_lc;                       // the JC softening length L_c = dx0
 };
 
@@ -62,6 +63,8 @@ template <typename R> struct DSolid {
     float* hx; float* hy; long long hcap; unsigned long long* hcount;
     const R* dmg_in; R* dmg_out;      // damage D (history buffers, when damage_on)
     const R* om_in; R* om_out;        // JC fracture omega (when jcf_on)
+    const R* f_in; R* f_out;          // Cocks-Ashby porosity (when ca_on)
+    const R* shp_in; R* shp_out;      // its peak hydrostatic tension
     const sph_material* mat;
     double* cons;
 };
@@ -261,6 +264,58 @@ __device__ __forceinline__ void jcf_step(const sph_material& M, double dep, doub
     if (soft > 0.0) D = fmin(1.0, D + lc * soft / M.jcf_uf);
 }
 
+// The error function by Abramowitz & Stegun 7.1.26 (|error| < 1.5e-7), the one the
+// reference's nucleation kernel evaluates (erf_approx, DAM001_cocks_ashby.py:207-226).
+__device__ __host__ __forceinline__ double erf_as(double x) {
+    const double ax = fabs(x);
+    const double t = 1.0 / (1.0 + 0.3275911 * ax);
+    const double poly = t * (0.254829592 + t * (-0.284496736 + t * (1.421413741 + t * (-1.453152027 + t * 1.061405429))));
+    const double r = 1.0 - poly * exp(-ax * ax);
+    return x < 0.0 ? -r : r;
+}
+
+// Chu-Needleman nucleation: f_n = fn0/2 (1 + erf((sigma_h* - sigma_hM) / (sigma_hS sqrt 2)))
+// at the peak hydrostatic tension sigma_h* (chu_needleman_nucleation, DAM001:229-240)
+__device__ __forceinline__ double ca_nucleation(const sph_material& M, double sh_peak) {
+    if (!(M.ca_fn0 > 0.0 && M.ca_shs > 0.0)) return 0.0;
+    return fmax(0.0, 0.5 * M.ca_fn0 * (1.0 + erf_as((sh_peak - M.ca_shm) / (M.ca_shs * 1.41421356237))));
+}
+
+// Cocks-Ashby void growth rates (dot f_e, dot f_s) under the card's rate coupling
+// (cocks_ashby_growth_ext, DAM001:243-317): the deviatoric growth
+// c1 sinh(c2 (m - 1/2)/(m + 1/2) sigma_h / tau_f) ((1 - f*)^-m - (1 - f*)) rate, and under
+// triaxial tension the hydrostatic growth c4 f*/(m + 1) (c5 sigma_h / tau_f)^m rate_eff.
+__device__ __forceinline__ void ca_growth(const sph_material& M, double sh, double tau_f, double dot_ep,
+                                          double div_v, double f, double fn, double& dfe, double& dfs) {
+    const double m = M.ca_m;
+    const int mode = M.ca_rate_mode;
+    const double tau = fmax(1.0e-6, tau_f);
+    const double fs = fmin(0.99, fmax(0.0, f));
+    const double fst = sh <= 0.0 ? fmax(0.0, fs - fn) : fs;
+    double drive = dot_ep;
+    if (sh > 0.0) {
+        if (mode == 2) drive = fmax(dot_ep, fmax(0.0, div_v));
+        else if (mode == 3) drive = fmax(dot_ep, 1.0);
+        else if (mode == 4) drive = fmax(dot_ep, 0.001 * pow(fmax(0.0, sh / tau), m));
+    }
+    const double arg = fmin(50.0, fmax(-50.0, M.ca_c2 * ((m - 0.5) / (m + 0.5)) * (sh / tau)));
+    const double sinh_v = 0.5 * (exp(arg) - exp(-arg));
+    const double omf = fmax(0.01, 1.0 - fst);
+    dfe = M.ca_c1 * sinh_v * (1.0 / pow(omf, m) - omf) * drive;
+    dfs = 0.0;
+    if (sh > 0.0 && fst > 0.0) {
+        const double rp = pow(fmax(0.0, M.ca_c5 * (sh / tau)), m);
+        const double den = fmax(0.01, 1.0 - fs);
+        if (mode == 1) {
+            dfs = (M.ca_c4 * fst / (m + 1.0)) * rp * fmax(0.0, div_v);
+        } else {
+            const double kap = fmin(0.95, fmax(0.0, (M.ca_c4 * fst / (den * (m + 1.0))) * rp));
+            const double eff = (drive + fmax(0.0, dfe) / den) / (1.0 - kap);
+            dfs = (M.ca_c4 * fst / (m + 1.0)) * rp * eff;
+        }
+    }
+}
+
 // The Johnson-Cook yield stress at zero increment (rate factor 1) and the thermal factor:
 // the elastic test of jc_radial_return (MAT002_johnson_cook.py:280-300)
 __device__ __forceinline__ double jc_static_yield(const sph_material& M, double eps_p, double temp, double* m_temp) {
@@ -318,6 +373,7 @@ template <typename R, bool DMG = false> struct StressBody {
     R G[3][3];
     R as_xx, as_yy, as_xy, as_zz, as_xz, as_yz, a_ep, a_eadv, a_ediff;
     R dm_i, om_i, a_dm, a_om;   // damage D and JC omega, and their ALE transport
+    R f_i, a_f;                 // Cocks-Ashby porosity and its ALE transport
 
     __device__ StressBody(const Args& a_)
         : a(a_), w_diff(0), w_plastic(0), w_iepair(0), w_ieadv(0), w_iediff(0), w_ieale(0), mg_c2(0), q_c(0) {}
@@ -356,8 +412,10 @@ template <typename R, bool DMG = false> struct StressBody {
         as_xx = as_yy = as_xy = as_zz = as_xz = as_yz = 0;
         a_ep = a_eadv = a_ediff = 0;
         dm_i = om_i = a_dm = a_om = 0;
+        f_i = a_f = 0;
         if (DMG && a.S.damage_on) dm_i = a.sd.dmg_in[i];
         if (DMG && a.S.jcf_on) om_i = a.sd.om_in[i];
+        if (DMG && a.S.ca_on) f_i = a.sd.f_in[i];
     }
 
     __device__ void pair(int j, int mir, double rxd, double ryd, double rzd) {
@@ -430,6 +488,8 @@ template <typename R, bool DMG = false> struct StressBody {
                 a_om += A * (a.sd.om_in[j] - om_i);
                 a_dm += A * (a.sd.dmg_in[j] - dm_i);
             }
+            // the porosity, every pair (compute_ale_transport_porosity_task, SOLID.py:1865-1885)
+            if (DMG && S.transport_ca) a_f += A * (a.sd.f_in[j] - f_i);
             R ej = a.sd.e_in[j];
             a_eadv += A * (ej - ei);
             if (S.ale_diff > 0.0 && same) {
@@ -454,6 +514,7 @@ template <typename R, bool DMG = false> struct StressBody {
         as_xy = lane_sum<LPP>(as_xy); as_zz = lane_sum<LPP>(as_zz);
         a_ep = lane_sum<LPP>(a_ep); a_eadv = lane_sum<LPP>(a_eadv); a_ediff = lane_sum<LPP>(a_ediff);
         if (DMG && a.S.transport_jcf) { a_om = lane_sum<LPP>(a_om); a_dm = lane_sum<LPP>(a_dm); }
+        if (DMG && a.S.transport_ca) a_f = lane_sum<LPP>(a_f);
         if (!two_d) {
             e22 = lane_sum<LPP>(e22); e02 = lane_sum<LPP>(e02); e12 = lane_sum<LPP>(e12);
             G[0][2] = lane_sum<LPP>(G[0][2]); G[1][2] = lane_sum<LPP>(G[1][2]);
@@ -570,6 +631,9 @@ template <typename R, bool DMG = false> struct StressBody {
             om = fmax(0.0, om + dt * (double)a_om);
             Dm = fmin(1.0, fmax(0.0, Dm + dt * (double)a_dm));
         }
+        // the porosity moved with the shift, every particle (SOLID.py:2791-2793)
+        double fpor = (double)f_i;
+        if (DMG && S.transport_ca) fpor = fmax(0.0, fpor + dt * (double)a_f);
         double dprod = 0.0;   // this step's plastic production (d_eps_prod)
         // the yield surface shrinks by s_d = 1 - D; a plastic particle at D = 1 returns to the
         // origin with the whole trial invariant as plastic strain (SOLID.py:2849-2880)
@@ -689,6 +753,27 @@ template <typename R, bool DMG = false> struct StressBody {
             // fracture on this step's production at eta = -p / sigma_vM of the returned
             // deviator; then the unilateral split, tension carried in proportion 1 - D
             if (M.damage_kind == SPH_DAMAGE_THRESHOLD && M.spall_p > 0.0 && -p_raw > M.spall_p) Dm = 1.0;
+            if (S.ca_on) {
+                double shp = (double)sd.shp_in[i];
+                if (M.damage_kind == SPH_DAMAGE_CA) {
+                    // Cocks-Ashby (SOLID.py:3224-3268): the peak tension, nucleation by its
+                    // increase, growth at the returned flow stress, D = tanh(a1 f)
+                    const double sh = -p_raw, sh_new = fmax(shp, sh);
+                    const double fn_new = ca_nucleation(M, sh_new);
+                    const double df_n = fmax(0.0, fn_new - ca_nucleation(M, shp));
+                    shp = sh_new;
+                    double ss = q6[0] * q6[0] + q6[1] * q6[1] + q6[3] * q6[3] +
+                                2.0 * (q6[2] * q6[2] + q6[4] * q6[4] + q6[5] * q6[5]);
+                    const double tau_f = fmax(sqrt(1.5 * ss), M.sigma_y0);
+                    double dfe, dfs;
+                    ca_growth(M, sh, tau_f, dprod / dt, Gd[0][0] + Gd[1][1] + Gd[2][2], fpor, fn_new, dfe, dfs);
+                    fpor = fmin(M.ca_fmax, fmax(0.0, fpor + df_n + (dfe + dfs) * dt));
+                    Dm = fmin(1.0, tanh(fmin(20.0, M.ca_a1 * fmax(0.0, fpor))));
+                    if (M.spall_p > 0.0 && -p_raw > M.spall_p) Dm = 1.0;
+                }
+                sd.f_out[i] = (R)fpor;
+                sd.shp_out[i] = (R)shp;
+            }
             if (M.damage_kind == SPH_DAMAGE_JC && dprod > 0.0) {
                 double ss = q6[0] * q6[0] + q6[1] * q6[1] + q6[3] * q6[3] +
                             2.0 * (q6[2] * q6[2] + q6[4] * q6[4] + q6[5] * q6[5]);
@@ -1877,6 +1962,8 @@ static DSolidP make_solid_params(const sph_ctx* c, const sph_fluid_params* P, co
     d.damage_on = S->damage != 0;
     d.jcf_on = S->damage && S->jc_failure;
     d.transport_jcf = d.jcf_on && d.transport;
+    d.ca_on = S->damage && S->cocks_ashby;
+    d.transport_ca = d.ca_on && d.transport;
     d.jcf_lc = S->jcf_lc;
     return d;
 }
@@ -1898,6 +1985,8 @@ template <typename R> static DSolid<R> dsolid(const sph_state* st, const sph_sol
     d.hcount = (unsigned long long*)sd->harvest_count;
     d.dmg_in = (const R*)sd->damage[h]; d.dmg_out = (R*)sd->damage[1 - h];
     d.om_in = (const R*)sd->jc_omega[h]; d.om_out = (R*)sd->jc_omega[1 - h];
+    d.f_in = (const R*)sd->porosity[h]; d.f_out = (R*)sd->porosity[1 - h];
+    d.shp_in = (const R*)sd->sigma_h_peak[h]; d.shp_out = (R*)sd->sigma_h_peak[1 - h];
     d.mat = sd->mat;
     d.cons = sd->cons;
     return d;
@@ -1958,7 +2047,10 @@ extern "C" int sph_solid_sort(sph_ctx* c, sph_state* st, sph_solid* sd, void* st
     PermTable T;
     int rb = mixed ? 4 : 8, r4 = mixed ? 16 : 32;
     int nf = 0;
-    auto add = [&](const void* s_, void* d_, int bytes) { T.f[nf].src = s_; T.f[nf].dst = d_; T.f[nf].bytes = bytes; ++nf; };
+    auto add = [&](const void* s_, void* d_, int bytes) {
+        if (nf >= PERM_MAX_FIELDS) return;   // never: the solid registers at most 24 fields
+        T.f[nf].src = s_; T.f[nf].dst = d_; T.f[nf].bytes = bytes; ++nf;
+    };
     add(A.pid, B.pid, 4); add(A.obj, B.obj, 4);
     add(A.x, B.x, 32); add(A.x0, B.x0, 32); add(A.u, B.u, 32); add(A.v, B.v, 32);
     add(A.shift, B.shift, r4);
@@ -1969,6 +2061,8 @@ extern "C" int sph_solid_sort(sph_ctx* c, sph_state* st, sph_solid* sd, void* st
     add(sd->e_int[h], sd->e_int[1 - h], rb);
     if (sd->damage[0] && sd->damage[1]) add(sd->damage[h], sd->damage[1 - h], rb);
     if (sd->jc_omega[0] && sd->jc_omega[1]) add(sd->jc_omega[h], sd->jc_omega[1 - h], rb);
+    if (sd->porosity[0] && sd->porosity[1]) add(sd->porosity[h], sd->porosity[1 - h], rb);
+    if (sd->sigma_h_peak[0] && sd->sigma_h_peak[1]) add(sd->sigma_h_peak[h], sd->sigma_h_peak[1 - h], rb);
     add(sd->T[k], sd->T[1 - k], rb);
     add(sd->de_int[k], sd->de_int[1 - k], rb);
     add(sd->eos_id[k], sd->eos_id[1 - k], 4);
@@ -2147,6 +2241,9 @@ extern "C" int sph_solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const sp
     if (!(S->V0 > 0.0)) return set_err(c, SPH_EINVAL, "reference cell volume must be positive%s", "", 0);
     if (S->damage && (!sd->damage[0] || !sd->damage[1] || (S->jc_failure && (!sd->jc_omega[0] || !sd->jc_omega[1]))))
         return set_err(c, SPH_EINVAL, "damage model without its damage / omega buffers%s", "", 0);
+    if (S->damage && S->cocks_ashby &&
+        (!sd->porosity[0] || !sd->porosity[1] || !sd->sigma_h_peak[0] || !sd->sigma_h_peak[1]))
+        return set_err(c, SPH_EINVAL, "Cocks-Ashby damage without its porosity buffers%s", "", 0);
     if (S->damage && S->jc_failure && !(S->jcf_lc > 0.0))
         return set_err(c, SPH_EINVAL, "the JC softening length must be positive%s", "", 0);
     if (S->hourglass > 0.0 && !(S->hg_rate > 0.0 && S->w_dx0 > 0.0))

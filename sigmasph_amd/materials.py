@@ -17,7 +17,7 @@ import ctypes
 import math
 
 from ._native import (SphMaterial, EOS_LINEAR, EOS_TAIT, EOS_MG, PLASTIC_NONE, PLASTIC_LINEAR,
-                      PLASTIC_JC, DAMAGE_NONE, DAMAGE_THRESHOLD, DAMAGE_JC)
+                      PLASTIC_JC, DAMAGE_NONE, DAMAGE_THRESHOLD, DAMAGE_CA, DAMAGE_JC)
 
 
 class MaterialError(ValueError):
@@ -37,7 +37,7 @@ def _mg(c0, s, gamma0):
 
 
 # material_models/material_library.py:28-474 (mm / ms / GPa / kg units).  The presets
-# that need a damage model or the HJC EOS (tantalum_spall, concrete_hjc_*) are not here.
+# on the HJC EOS (concrete_hjc_*) are not here.
 LIBRARY = {
     "aluminium": _jc_preset(2.70e-06, 896.0, 69.0, 0.33, 0.324, 0.114, 0.42, 0.002, 925.0, 1.34),
     "al2017_t4": _jc_preset(2.79e-06, 875.0, 72.4, 0.33, 0.265, 0.426, 0.34, 0.015, 775.0, 1.00,
@@ -54,6 +54,12 @@ LIBRARY = {
     "aluminium_kim2025": _jc_preset(2.71e-06, 875.0, 73.416, 0.33, 0.175, 0.380, 0.34, 0.0015,
                                     775.0, 1.0, eos=_mg(5300.0, 1.5, 1.7), T0=273.0),
 }
+# tantalum with Cocks-Ashby spall damage (material_library.py:241-286, after Qamar et al. 2025)
+LIBRARY["tantalum_spall"] = _jc_preset(1.669e-05, 140.0, 185.7, 0.345, 0.733, 0.540, 0.28, 0.054, 3269.0, 0.44,
+                                       eos=_mg(3410.0, 1.20, 1.60))
+LIBRARY["tantalum_spall"]["damage"] = {"model": "cocks_ashby", "c1": 1.5, "c2": 0.72, "c4": 1.0, "c5": 0.1,
+                                       "a1": 25.0, "fn0": 0.0001, "sigma_hM": 0.778, "sigma_hS": 0.0389,
+                                       "m": 4.0, "f_max": 0.5}
 # gelatin's Johnson-Cook Cp is 0 in the library (no adiabatic heating of the yield)
 for _g in ("gelatin_10", "gelatin_20"):
     LIBRARY[_g]["strength"]["plasticity"]["Cp"] = 0.0
@@ -90,6 +96,16 @@ ENTRY_KEYS = {"name", "library", "density0", "specificHeat", "delta", "eos", "st
 JCF_KEYS = (("D1", ("JCF_D1", "D1"), None), ("D2", ("JCF_D2", "D2"), None), ("D3", ("JCF_D3", "D3"), None),
             ("D4", ("JCF_D4", "D4"), 0.0), ("D5", ("JCF_D5", "D5"), 0.0), ("EFMIN", ("JCF_EFMIN", "EFMIN"), 0.0),
             ("u_f", ("JCF_UF", "failureDisplacement"), None))
+# the Cocks-Ashby card (derive_cocks_ashby, DAM001_cocks_ashby.py:78-204): key, accepted
+# names, default
+CA_KEYS = (("c1", ("c1", "DAM_C1"), 1.5), ("c2", ("c2", "DAM_C2"), 0.72), ("c4", ("c4", "DAM_C4"), 1.0),
+           ("c5", ("c5", "DAM_C5"), 0.1), ("a1", ("a1", "DAM_A1"), 25.0), ("fn0", ("fn0", "DAM_FN0"), 1e-4),
+           ("sigma_hM", ("sigma_hM", "DAM_SIGMA_HM", "sigma_hm", "sigma_h_mean"), 0.778),
+           ("sigma_hS", ("sigma_hS", "DAM_SIGMA_HS", "sigma_hs", "sigma_h_std"), 0.0389),
+           ("m", ("m", "DAM_M"), 4.0), ("f_max", ("f_max", "DAM_FMAX"), 0.5),
+           ("spallPressure", ("spallPressure", "DAM_SPALL_PRESSURE"), 0.0))
+CA_RATE_MODES = {"effective": 0, "qamar": 0, "volumetric_fs": 1, "volumetric": 2, "unconstrained": 3,
+                 "unconstrained_stress": 4}
 
 
 def _merge(base, over):
@@ -255,14 +271,16 @@ class Material:
     def _damage(self, entry, eos, where):
         """The damage half of the material (materials.py:184-197, 351-425): the spall
         cutoff (`spallPressure` on the entry or its EOS card) and the `damage` card.
-        Built: 'johnson_cook' fracture (on Johnson-Cook plasticity) and the threshold
-        spall model; 'cocks_ashby' and 'hjc' are refused."""
+        Built: 'johnson_cook' fracture (on Johnson-Cook plasticity), 'cocks_ashby' void
+        growth and the threshold spall model; 'hjc' is refused."""
         spall = entry.get("spallPressure")
         if spall is None:
             spall = eos.get("spallPressure")
         self.spall_pressure = float(spall) if spall is not None else 0.0
         self.damage_kind = DAMAGE_NONE
         self.jcf = dict(D1=0.0, D2=0.0, D3=0.0, D4=0.0, D5=0.0, EFMIN=0.0, u_f=1.0)
+        self.ca = dict(c1=0.0, c2=0.0, c4=0.0, c5=0.0, a1=0.0, fn0=0.0, sigma_hM=0.0, sigma_hS=1.0, m=1.0,
+                       f_max=0.0, rateMode=0)
         card = entry.get("damage")
         model = ""
         if card is not None:
@@ -290,9 +308,32 @@ class Material:
                 raise MaterialError("%s.damage: Johnson-Cook EFMIN must be non-negative, got %g."
                                     % (where, self.jcf["EFMIN"]))
             self.damage_kind = DAMAGE_JC
-        elif model in ("cocks_ashby", "qamar", "hjc"):
-            raise MaterialError("%s.damage: model %r is not part of this build (johnson_cook and the "
-                                "spall threshold are)" % (where, model))
+        elif model in ("cocks_ashby", "qamar"):
+            for key, names, default in CA_KEYS:
+                v = next((card[n] for n in names if card.get(n) is not None), default)
+                try:
+                    self.ca[key] = float(v)
+                except (TypeError, ValueError):
+                    raise MaterialError("%s.damage: Cocks-Ashby parameter %r must be numeric, got %r"
+                                        % (where, key, v))
+            if self.ca["sigma_hS"] <= 0.0:
+                raise MaterialError("%s.damage: sigma_hS must be strictly positive, got %g."
+                                    % (where, self.ca["sigma_hS"]))
+            if self.ca["m"] <= 0.5:
+                raise MaterialError("%s.damage: Rate sensitivity exponent m must be > 0.5, got %g."
+                                    % (where, self.ca["m"]))
+            if not 0.0 < self.ca["f_max"] < 1.0:
+                raise MaterialError("%s.damage: f_max must be in (0, 1), got %g." % (where, self.ca["f_max"]))
+            mode = str(card.get("rateMode") or card.get("DAM_RATE_MODE") or "effective").lower()
+            if mode not in CA_RATE_MODES:
+                raise MaterialError("%s.damage: Unknown rateMode %r; must be one of %s"
+                                    % (where, mode, list(CA_RATE_MODES)))
+            self.ca["rateMode"] = CA_RATE_MODES[mode]
+            self.spall_pressure = self.ca["spallPressure"]      # the card's own cutoff
+            self.damage_kind = DAMAGE_CA
+        elif model == "hjc":
+            raise MaterialError("%s.damage: model 'hjc' is not part of this build (johnson_cook, "
+                                "cocks_ashby and the spall threshold are)" % where)
         elif self.spall_pressure > 0.0 or model in ("threshold", "spall"):
             self.damage_kind = DAMAGE_THRESHOLD
             if card is not None and card.get("spallPressure") is not None:
@@ -317,6 +358,11 @@ class Material:
         f = self.jcf
         (m.jcf_D1, m.jcf_D2, m.jcf_D3, m.jcf_D4, m.jcf_D5, m.jcf_efmin,
          m.jcf_uf) = f["D1"], f["D2"], f["D3"], f["D4"], f["D5"], f["EFMIN"], f["u_f"]
+        a = self.ca
+        (m.ca_c1, m.ca_c2, m.ca_c4, m.ca_c5, m.ca_a1, m.ca_fn0, m.ca_shm, m.ca_shs, m.ca_m,
+         m.ca_fmax) = (a["c1"], a["c2"], a["c4"], a["c5"], a["a1"], a["fn0"], a["sigma_hM"], a["sigma_hS"],
+                       a["m"], a["f_max"])
+        m.ca_rate_mode = int(a["rateMode"])
         return m
 
     def describe(self):

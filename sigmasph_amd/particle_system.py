@@ -492,6 +492,7 @@ class ParticleSystem(SceneLayout):
         if self.has_damage:
             for h in self._hist:
                 h["damage"], h["jc_omega"] = z(n), z(n)
+                h["porosity"], h["sigma_h_peak"] = z(n), z(n)
         self._sbufs = [{"T": z(n), "de_int": z(n), "eos_id": z(n, dt=torch.int32),
                         "bc_flag": z(n, dt=torch.int32), "bc_id": z(n, dt=torch.int32),
                         "bc_disp": z(n, 4)} for _ in range(2)]
@@ -513,6 +514,8 @@ class ParticleSystem(SceneLayout):
             if self.has_damage:
                 sd.damage[k] = self._hist[k]["damage"].data_ptr()
                 sd.jc_omega[k] = self._hist[k]["jc_omega"].data_ptr()
+                sd.porosity[k] = self._hist[k]["porosity"].data_ptr()
+                sd.sigma_h_peak[k] = self._hist[k]["sigma_h_peak"].data_ptr()
             for name in ("T", "de_int", "eos_id", "bc_flag", "bc_id", "bc_disp"):
                 getattr(sd, name)[k] = self._sbufs[k][name].data_ptr()
             sd.hg_y[k] = self._sderived["hg_y%d" % k].data_ptr()
@@ -530,6 +533,8 @@ class ParticleSystem(SceneLayout):
         if self.has_damage:
             self.damage = Field(lambda: H()["damage"], 1, "damage")
             self.jc_omega = Field(lambda: H()["jc_omega"], 1, "jc_omega")
+            self.porosity = Field(lambda: H()["porosity"], 1, "porosity")
+            self.sigma_h_peak = Field(lambda: H()["sigma_h_peak"], 1, "sigma_h_peak")
         self.temperature = Field(lambda: B()["T"], 1, "temperature")
         self.de_int = Field(lambda: B()["de_int"], 1, "de_int")
         self.eos_id = Field(lambda: B()["eos_id"], 1, "eos_id")
@@ -570,6 +575,7 @@ class ParticleSystem(SceneLayout):
             if getattr(self, "has_damage", False):
                 out["damage"] = self.damage.to_numpy()[mask]
                 out["jc_omega"] = self.jc_omega.to_numpy()[mask]
+                out["porosity"] = self.porosity.to_numpy()[mask]
         return out
 
     def _live(self):

@@ -58,6 +58,10 @@ SCENES = {
                   "strength": {"youngsModulus": 193.0, "poissonRatio": 0.3,
                                "plasticity": {"model": "linear", "yieldStress": 0.31, "hardeningModulus": 1.0}},
                   "spallPressure": 0.8}),
+    # Cocks-Ashby void growth in tantalum under triaxial expansion: nucleation once the
+    # tension passes ~0.78 GPa, then growth (D = tanh(25 f) ~ 0.01 after five steps)
+    "cocks_ashby": lambda: solid_block(n=(20, 10, 1), div_velocity=(300.0, 300.0, 0.0), jitter=0.02,
+                                       density=1.669e-05, material={"name": "ta", "library": "tantalum_spall"}),
     "plane_strain_noshift": lambda: solid_block(n=(20, 10, 1), pst=False, viscosity=0.0,
                                                 div_velocity=(-200.0, 250.0, 0.0), jitter=0.03),
     # nine specimens with their own Johnson-Cook rows and 18 grips (past 16 the grip
@@ -130,8 +134,9 @@ def compare(ps, st, s, tol, eps=0.0):
     if st.q_l2 > 0.0:
         pairs.append(("q_visc", s.by_pid("q")))
     if s.damage is not None:
-        pairs += [("damage", s.by_pid("damage")), ("jc_omega", s.by_pid("jc_omega"))]
-        floors.update({"damage": 1.0, "jc_omega": 1.0})
+        pairs += [("damage", s.by_pid("damage")), ("jc_omega", s.by_pid("jc_omega")),
+                  ("porosity", s.by_pid("porosity")), ("sigma_h_peak", s.by_pid("sigma_h_peak"))]
+        floors.update({"damage": 1.0, "jc_omega": 1.0, "porosity": 1e-3, "sigma_h_peak": 1e-6 * M.K})
     order = np.argsort(ps.pid.to_numpy(), kind="stable")
     errs = {}
     for name, ref in pairs:
@@ -170,6 +175,8 @@ def test_five_steps_fp64(gpu, name):
         assert s.scal[N.SPH_S_PLASTIC_WORK] > 0.0      # the return map ran
     if st.hg_sts:
         assert s.scal[N.SPH_S_STS_STAGES] >= 1
+    if name == "cocks_ashby":
+        assert s.porosity.max() > 2e-4 and s.damage.max() > 0.005   # nucleated and grew
     if name in ("jc_fracture", "spall_threshold"):
         assert s.damage.max() > 0.0                      # the damage model acted
         if name == "jc_fracture":
@@ -180,7 +187,7 @@ def test_five_steps_fp64(gpu, name):
 
 @pytest.mark.parametrize("name", ["plane_strain_grips", "axisymmetric_grips", "block3d_explicit",
                                   "axisymmetric_shock", "taylor3d", "plane_strain_rkl", "jc_fracture",
-                                  "spall_threshold"])
+                                  "spall_threshold", "cocks_ashby"])
 def test_three_steps_mixed(gpu, name):
     ps, solver, st, s = run_pair(SCENES[name](), 3, precision="mixed")
     compare(ps, st, s, 2e-4, eps=2.0 ** -24)

@@ -141,3 +141,72 @@ def test_damaged_return_maps():
     vm1 = np.sqrt(1.5 * (np.sum(s.sig[~full, :3] ** 2, 1) + 2 * np.sum(s.sig[~full, 3:] ** 2, 1)))
     np.testing.assert_allclose(vm1, 0.6 * (0.3 + 1.0 * s.eps_p[~full]), rtol=1e-12)
     np.testing.assert_allclose(s.d_eps_prod[full], vm0[full] / (3.0 * M.G), rtol=1e-13)
+
+
+# ---- Cocks-Ashby (DAM001_cocks_ashby.py) ----------------------------------------------
+def _ta():
+    return resolve([{"name": "ta", "library": "tantalum_spall"}])["ta"].row()
+
+
+def test_erf_is_abramowitz_stegun_within_its_bound():
+    for x in np.linspace(-4.0, 4.0, 161):
+        assert abs(O.erf_as(float(x)) - math.erf(float(x))) < 1.5e-7
+    assert O.erf_as(-1.3) == -O.erf_as(1.3)
+
+
+def test_nucleation_is_a_normal_cdf_of_the_peak_tension():
+    from scipy.stats import norm
+    M = _ta()
+    for sh in np.linspace(0.5, 1.1, 25):
+        want = M.ca_fn0 * norm.cdf(sh, loc=M.ca_shm, scale=M.ca_shs)
+        assert abs(O.ca_nucleation(M, float(sh)) - want) < 1.5e-7 * M.ca_fn0
+    assert O.ca_nucleation(M, 10.0) == pytest.approx(M.ca_fn0, rel=1e-12)
+
+
+def test_growth_rates_by_construction():
+    """dot f_e is odd in sigma_h and linear in the plastic rate, zero without voids;
+    dot f_s vanishes in compression; in the effective mode the hydrostatic growth solves
+    its implicit rate, eps_eff = eps_p + (dot f_e + dot f_s) / (1 - f); the rate modes
+    drive as stated."""
+    M = _ta()
+    fe, fs = O.ca_growth(M, 0.9, 1.0, 2.0, 0.0, 0.0, 0.0)
+    assert fe == 0.0 and fs == 0.0                                   # f* = 0: nothing grows
+    a = O.ca_growth(M, 0.9, 1.0, 2.0, 0.0, 0.05, 0.0)
+    b = O.ca_growth(M, -0.9, 1.0, 2.0, 0.0, 0.05, 0.0)
+    assert a[0] == pytest.approx(-b[0], rel=1e-14) and a[0] > 0.0    # sinh is odd
+    assert b[1] == 0.0                                               # no hydrostatic growth in compression
+    c = O.ca_growth(M, 0.9, 1.0, 4.0, 0.0, 0.05, 0.0)
+    assert c[0] == pytest.approx(2.0 * a[0], rel=1e-14)
+    f, dot_ep = 0.05, 2.0
+    eff = dot_ep + (max(a[0], 0.0) + a[1]) / (1.0 - f)
+    kap_fs = a[1] / eff                                              # c4 f/(m+1) (c5 s/t)^m
+    ratio = (M.ca_c5 * 0.9 / 1.0) ** M.ca_m
+    assert kap_fs == pytest.approx(M.ca_c4 * f / (M.ca_m + 1.0) * ratio, rel=1e-12)
+    Mv = _ta(); Mv.ca_rate_mode = 2                                  # volumetric: max(eps_p, div v)
+    assert O.ca_growth(Mv, 0.9, 1.0, 2.0, 7.0, 0.05, 0.0)[0] == pytest.approx(
+        O.ca_growth(M, 0.9, 1.0, 7.0, 0.0, 0.05, 0.0)[0], rel=1e-14)
+    Mu = _ta(); Mu.ca_rate_mode = 3                                  # unconstrained: max(eps_p, 1)
+    assert O.ca_growth(Mu, 0.9, 1.0, 0.2, 0.0, 0.05, 0.0)[0] == pytest.approx(
+        O.ca_growth(M, 0.9, 1.0, 1.0, 0.0, 0.05, 0.0)[0], rel=1e-14)
+    Mf = _ta(); Mf.ca_rate_mode = 1                                  # volumetric_fs: div v drives f_s
+    fsv = O.ca_growth(Mf, 0.9, 1.0, 2.0, 3.0, 0.05, 0.0)[1]
+    assert fsv == pytest.approx(M.ca_c4 * f / (M.ca_m + 1.0) * ratio * 3.0, rel=1e-12)
+
+
+def test_cocks_ashby_state_through_a_step():
+    """A tantalum block under triaxial expansion: the peak tension only rises, the
+    porosity starts at the nucleated fraction and grows, D = tanh(a1 f) (SOLID.py:3260-3263)."""
+    sc = solid_block(n=(20, 10, 1), div_velocity=(300.0, 300.0, 0.0), jitter=0.02, density=1.669e-05,
+                     material={"name": "ta", "library": "tantalum_spall"})
+    lay, st, P, S = solid_layout(sc)
+    assert S.cocks_ashby == 1
+    s = solid_oracle_state(lay, st)
+    peak = np.zeros(s.n)
+    for _ in range(5):
+        O.solid_step(lay.dims, P, S, s)
+        now = s.by_pid("sigma_h_peak")
+        assert np.all(now >= peak - 1e-15)
+        peak = now
+    f, D = s.porosity, s.damage
+    assert f.max() <= 0.5 and f.max() > 2e-4
+    np.testing.assert_allclose(D, np.tanh(25.0 * f), rtol=1e-12, atol=1e-15)

@@ -246,7 +246,10 @@ typedef struct {
     int32_t damage;          /* some material has a damage model: sph_solid.damage is live   */
     int32_t jc_failure;      /* some material has the JC fracture card: jc_omega is live     */
     int32_t cocks_ashby;     /* some material has the Cocks-Ashby card: porosity is live     */
-    int32_t pad2_;
+    int32_t erosion_cap;     /* Erosion.mode "cap": J = V rho0 / m clamped into [j_min, j_max] */
+    double  erosion_j_min, erosion_j_max;   /* (apply_erosion, SOLID.py:4906-4959)          */
+    double  v_cap;           /* VelocityLimit.cap: |v| ceiling of unconstrained particles,
+                              * 0 = off (cap_velocity, SOLID.py:4789-4806)                  */
 } sph_solid_params;
 
 /* sph_solid_params.sts: how the s passes of a step are spent (hourglass_sts,

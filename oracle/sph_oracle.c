@@ -1937,6 +1937,15 @@ void oracle_solid_step(const sph_dims* d, const sph_fluid_params* P, const sph_s
     or_internal_energy(d, P, S, s, &g_grid);
     if (S->thermal) or_temperature(P, s);
     or_volume(d, P, s, &g_grid);
+    if (S->erosion_cap) {
+        /* apply_erosion, "cap" (SOLID.py:4906-4959): J = V rho0 / m into [j_min, j_max] */
+        for (int64_t i = 0; i < s->n; ++i) {
+            double V0 = s->m[i] / or_mat(s, i)->rho0, J = s->V[i] / V0;
+            int nanJ = isnan(J);
+            if (nanJ || J < S->erosion_j_min || J > S->erosion_j_max)
+                s->V[i] = nanJ ? V0 : fmin(fmax(J, S->erosion_j_min), S->erosion_j_max) * V0;
+        }
+    }
     or_stress_accelerations(d, P, S, s, &g_grid);
     or_adaptive_dt(d, P, s, &g_grid, S);
     or_solid_clamps(d, P, S, s);
@@ -1952,6 +1961,19 @@ void oracle_solid_step(const sph_dims* d, const sph_fluid_params* P, const sph_s
         or_advect(d, P, s);
     }
     s->scal[SPH_S_STS_STAGES] = st;
+    if (S->v_cap > 0.0) {
+        /* cap_velocity (SOLID.py:4789-4806) on unconstrained particles */
+        for (int64_t i = 0; i < s->n; ++i) {
+            if (s->bc_flag[i] != 0) continue;
+            double* v = s->v + 3 * i;
+            double sp = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            if (sp > S->v_cap) {
+                double k = S->v_cap / sp - 1.0, dr[3];
+                for (int c = 0; c < 3; ++c) { double dv = v[c] * k; v[c] += dv; dr[c] = dv * s->scal[SPH_S_DT]; }
+                or_displace(d, P, s, i, dr);
+            }
+        }
+    }
     or_constraints(d, P, S, s);
     if (P->pst_enabled) or_apply_pst(d, P, s, &g_grid);
     or_boundaries(d, P, s);

@@ -155,6 +155,17 @@ class SolidSettings:
                 raise NotImplementedError("material %r: damage on a material without plasticity is "
                                           "not part of this build" % m.name)
         self.jcf_lc = self.dx0                                           # SOLID.py:1675-1682
+        # the runaway limiters (SOLID.py:989-1013, 4687-4976): the hard velocity cap and
+        # erosion's "cap" mode are built; the momentum-conserving limiter and "erode",
+        # which takes particles out of every neighbour sum, are not
+        self.v_cap = float(c("VELOCITY_CAP") or 0.0)
+        if c("VELOCITY_LIMIT"):
+            raise NotImplementedError("VelocityLimit.limiter is not part of this build (VelocityLimit.cap is)")
+        self.erosion_mode = str(c("EROSION_MODE") or "off").lower()
+        if self.erosion_mode == "erode":
+            raise NotImplementedError("Erosion.mode 'erode' is not part of this build ('cap' is)")
+        self.erosion_j_min = float(c("EROSION_J_MIN") if c("EROSION_J_MIN") is not None else 0.3)
+        self.erosion_j_max = float(c("EROSION_J_MAX") if c("EROSION_J_MAX") is not None else 1.5)
         self.ncons = len(getattr(ps, "constraints", []) or [])
         self.mat_table = material_table(self.materials)
 
@@ -191,6 +202,9 @@ class SolidSettings:
         S.jc_failure = int(self.has_jc_failure)
         S.cocks_ashby = int(self.has_cocks_ashby)
         S.jcf_lc = self.jcf_lc
+        S.erosion_cap = int(self.erosion_mode == "cap")
+        S.erosion_j_min, S.erosion_j_max = self.erosion_j_min, self.erosion_j_max
+        S.v_cap = self.v_cap
         return S
 
     def load_factor(self, t):

@@ -171,7 +171,9 @@ class SphSolidParams(ctypes.Structure):
         "sts_refresh", "dt_pair_mode", "visc_work", "thermal", "plastic", "nmat", "ncons",
         "pad_")] + [("sur_tol", ctypes.c_double), ("sts_safety", ctypes.c_double),
                     ("jcf_lc", ctypes.c_double), ("damage", ctypes.c_int32), ("jc_failure", ctypes.c_int32),
-                    ("cocks_ashby", ctypes.c_int32), ("pad2_", ctypes.c_int32)]
+                    ("cocks_ashby", ctypes.c_int32), ("erosion_cap", ctypes.c_int32),
+                    ("erosion_j_min", ctypes.c_double), ("erosion_j_max", ctypes.c_double),
+                    ("v_cap", ctypes.c_double)]
 
 
 _vp2 = ctypes.c_void_p * 2

@@ -129,6 +129,14 @@ PARAMS = [
     P("HOURGLASS_STS_STAGES", "Hourglass.superTimeStepping", "stages", "int", 0),
     P("HOURGLASS_STS_REFRESH", "Hourglass.superTimeStepping", "refreshGradV", "bool", True),
     P("HOURGLASS_STS_SAFETY", "Hourglass.superTimeStepping", "safety", "float", 0.8),
+    # VelocityLimit and Erosion (params.py:1193-1223)
+    P("VELOCITY_CAP", "VelocityLimit", "cap", "float", 0.0),
+    P("VELOCITY_LIMIT_U", "VelocityLimit.limiter", "u", "float", None),
+    P("VELOCITY_LIMIT_CONSERVE", "VelocityLimit.limiter", "conserve", "bool", True),
+    P("EROSION_MODE", "Erosion", "mode", "str", "off", choices=("off", "cap", "erode")),
+    P("EROSION_J_MIN", "Erosion", "minVolumeRatio", "float", 0.3),
+    P("EROSION_J_MAX", "Erosion", "maxVolumeRatio", "float", 1.5),
+    P("EROSION_EPS_P_MAX", "Erosion", "maxPlasticStrain", "float", 0.0),
     # Output (params.py:1226-1316): read by the driver, not by the step
     P("exportTxt", "Output", "txt", "bool", False),
     P("exportNetcdf", "Output", "netcdf", "bool", False),
@@ -152,11 +160,13 @@ PARAMS = [
     P("exponent", "Materials", "exponent", "float", 7.0),
 ]
 BY_NAME = {p.name: p for p in PARAMS}
-CARDS = ("Domain", "Time", "Solver", "Load", "Diffusion", "Shifting", "Hourglass", "Output")
+CARDS = ("Domain", "Time", "Solver", "Load", "Diffusion", "Shifting", "Hourglass", "Output",
+         "VelocityLimit", "Erosion")
 #: sub-cards; those whose presence switches a feature on name their flag
 #: (params.py:1326-1340)
-SUBCARDS = ("Hourglass.superTimeStepping", "Solver.bulkViscosity", "Solver.artificialViscosity")
-PRESENCE = {"Hourglass.superTimeStepping": "HOURGLASS_STS"}
+SUBCARDS = ("Hourglass.superTimeStepping", "Solver.bulkViscosity", "Solver.artificialViscosity",
+            "VelocityLimit.limiter")
+PRESENCE = {"Hourglass.superTimeStepping": "HOURGLASS_STS", "VelocityLimit.limiter": "VELOCITY_LIMIT"}
 CONSTRAINT_KEYS = {"name", "region", "components", "displacement"}
 BLOCK_KEYS = {"objectId", "start", "end", "translation", "scale", "velocity", "density",
               "color", "rotVelocity", "divVelocity", "shape", "material",

@@ -46,6 +46,9 @@ struct DSolidP {
     double sur_tol;
     int damage_on, jcf_on, transport_jcf;   // damage D live; JC fracture omega live; moved by the shift
     int ca_on, transport_ca;             // Cocks-Ashby porosity live; moved by the shift
+    int ero_cap;                         // erosion "cap": J clamped into [ero_jmin, ero_jmax]
+    double ero_jmin, ero_jmax;
+    double v_cap;                        // |v| ceiling of unconstrained particles, 0 = off
     double jcf_lc;                       // the JC softening length L_c = dx0
 };
 
@@ -741,6 +744,18 @@ template <typename R, bool DMG = false> struct StressBody {
         } else {
             Vn = Vi + (R)dt * dVdt;
             Vfull = (double)Vn;
+        }
+        if (S.ero_cap) {
+            // apply_erosion, "cap" (SOLID.py:4906-4959): J = V rho0 / m back into its window
+            // before the EOS reads V (a NaN J to J = 1)
+            const double V0 = m / M.rho0, J = Vfull / V0;
+            const bool nanJ = isnan(J);
+            if (nanJ || J < S.ero_jmin || J > S.ero_jmax) {
+                const double Vc = nanJ ? V0 : fmin(fmax(J, S.ero_jmin), S.ero_jmax) * V0;
+                Vn = (R)Vc;
+                if (sizeof(R) == 4 && F.comp_vol) b.V_lo[i] = (R)(Vc - (double)Vn);
+                Vfull = sizeof(R) == 4 && F.comp_vol ? Vc : (double)Vn;
+            }
         }
         st.V_new[i] = Vn;
         st.rho_new[i] = (R)(m / Vfull);
@@ -1590,8 +1605,9 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_sts_kick2(DGrid g, DFluid F, DSta
 // clear_constraint_forces + apply_constraints (SOLID.py:5019-5055), and max |v| for a
 // global shifting Mach number (taken after the grips, as DSPH.py:712-723 runs there)
 template <typename R>
-__global__ __launch_bounds__(SPH_BLOCK) void k_constraints(DFluid F, DState<R> st, DSolid<R> sd, DScratch s,
-                                                           double* cons_part, int ncons, int nblk) {
+__global__ __launch_bounds__(SPH_BLOCK) void k_constraints(DGrid g, DFluid F, DState<R> st, DSolid<R> sd,
+                                                           DScratch s, double* cons_part, int ncons, int nblk,
+                                                           double v_cap) {
     __shared__ double red[3 * SPH_BLOCK / SPH_WAVE];
     int i = blockIdx.x * SPH_BLOCK + threadIdx.x;
     double fx = 0.0, fy = 0.0, fz = 0.0, vmax = 0.0;
@@ -1630,6 +1646,27 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_constraints(DFluid F, DState<R> s
             b.u[i] = make_double4(us[0], us[1], us[2], 0.0);
             typename R4t<R>::type ao; ao.x = (R)av[0]; ao.y = (R)av[1]; ao.z = (R)av[2]; ao.w = 0;
             st.acc[i] = ao;
+        } else if (v_cap > 0.0) {
+            // cap_velocity (SOLID.py:4789-4806): back onto |v| = v_cap along v, and the
+            // position corrected by the same increment (before the grips, which it never touches)
+            const double sp = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+            if (sp > v_cap) {
+                const double k = v_cap / sp - 1.0, dt = st.scal[SPH_S_DT];
+                const double dx = v.x * k, dy = v.y * k, dz = v.z * k;
+                v.x += dx; v.y += dy; v.z += dz;
+                b.v[i] = v;
+                double4 x = b.x[i];
+                if (F.disp_pos) {
+                    double4 u = b.u[i], x0 = b.x0[i];
+                    u.x += dx * dt; u.y += dy * dt; u.z += dz * dt;
+                    b.u[i] = u;
+                    x.x = x0.x + u.x; x.y = x0.y + u.y; x.z = x0.z + u.z;
+                } else {
+                    x.x += dx * dt; x.y += dy * dt; x.z += dz * dt;
+                }
+                x.x = wrap_pos(g, 0, x.x); x.y = wrap_pos(g, 1, x.y); x.z = wrap_pos(g, 2, x.z);
+                b.x[i] = x;
+            }
         }
         if (F.ma_mode == 1) {
             double4 x = b.x[i];
@@ -1962,6 +1999,10 @@ static DSolidP make_solid_params(const sph_ctx* c, const sph_fluid_params* P, co
     d.damage_on = S->damage != 0;
     d.jcf_on = S->damage && S->jc_failure;
     d.transport_jcf = d.jcf_on && d.transport;
+    d.ero_cap = S->erosion_cap != 0;
+    d.v_cap = S->v_cap;
+    d.ero_jmin = S->erosion_j_min;
+    d.ero_jmax = S->erosion_j_max;
     d.ca_on = S->damage && S->cocks_ashby;
     d.transport_ca = d.ca_on && d.transport;
     d.jcf_lc = S->jcf_lc;
@@ -2197,8 +2238,8 @@ static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F,
         hipLaunchKernelGGL((k_advect_solid<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st));
     }
     if ((rc = check_launch(c, "solid advect"))) return rc;
-    hipLaunchKernelGGL((k_constraints<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, F, dstate<R>(c, st),
-                       dsolid<R>(st, sd), c->s, c->cons_part, S.ncons, nb);
+    hipLaunchKernelGGL((k_constraints<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st),
+                       dsolid<R>(st, sd), c->s, c->cons_part, S.ncons, nb, S.v_cap);
     if (F.ma_mode == 1) hipLaunchKernelGGL(k_publish_vmax, dim3(1), dim3(64), 0, hs, st->scal, c->s);
     if (F.pst_en) {
         PassArgs<R> pa = pass_args<R>(c, st, F);

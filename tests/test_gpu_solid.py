@@ -62,6 +62,11 @@ SCENES = {
     # tension passes ~0.78 GPa, then growth (D = tanh(25 f) ~ 0.01 after five steps)
     "cocks_ashby": lambda: solid_block(n=(20, 10, 1), div_velocity=(300.0, 300.0, 0.0), jitter=0.02,
                                        density=1.669e-05, material={"name": "ta", "library": "tantalum_spall"}),
+    # the runaway limiters: erosion's cap on J and the hard velocity cap, both firing
+    "limiters": lambda: solid_block(
+        n=(16, 8, 1), div_velocity=(-300.0, 200.0, 0.0), jitter=0.02, grips=0.05, ramp_time=1e-4,
+        extra={"Erosion": {"mode": "cap", "minVolumeRatio": 0.995, "maxVolumeRatio": 1.004},
+               "VelocityLimit": {"cap": 150.0}}),
     "plane_strain_noshift": lambda: solid_block(n=(20, 10, 1), pst=False, viscosity=0.0,
                                                 div_velocity=(-200.0, 250.0, 0.0), jitter=0.03),
     # nine specimens with their own Johnson-Cook rows and 18 grips (past 16 the grip
@@ -187,7 +192,7 @@ def test_five_steps_fp64(gpu, name):
 
 @pytest.mark.parametrize("name", ["plane_strain_grips", "axisymmetric_grips", "block3d_explicit",
                                   "axisymmetric_shock", "taylor3d", "plane_strain_rkl", "jc_fracture",
-                                  "spall_threshold", "cocks_ashby"])
+                                  "spall_threshold", "cocks_ashby", "limiters"])
 def test_three_steps_mixed(gpu, name):
     ps, solver, st, s = run_pair(SCENES[name](), 3, precision="mixed")
     compare(ps, st, s, 2e-4, eps=2.0 ** -24)

@@ -210,3 +210,47 @@ def test_cocks_ashby_state_through_a_step():
     f, D = s.porosity, s.damage
     assert f.max() <= 0.5 and f.max() > 2e-4
     np.testing.assert_allclose(D, np.tanh(25.0 * f), rtol=1e-12, atol=1e-15)
+
+
+# ---- the runaway limiters: erosion "cap" and the velocity cap (SOLID.py:4789-4959) --------
+def _fast_block(extra, pst=True):
+    return solid_block(n=(16, 8, 1), div_velocity=(-300.0, 200.0, 0.0), jitter=0.02, grips=0.05,
+                       ramp_time=1e-4, extra=extra, pst=pst)
+
+
+def test_erosion_cap_bounds_the_jacobian_and_the_pressure():
+    sc = _fast_block({"Erosion": {"mode": "cap", "minVolumeRatio": 0.995, "maxVolumeRatio": 1.004}})
+    lay, st, P, S = solid_layout(sc)
+    assert S.erosion_cap == 1
+    s = solid_oracle_state(lay, st)
+    M = st.default_material
+    hit = False
+    for _ in range(4):
+        O.solid_step(lay.dims, P, S, s)
+        J = s.V * M.rho0 / s.m
+        assert J.min() >= 0.995 * (1 - 1e-14) and J.max() <= 1.004 * (1 + 1e-14)
+        hit |= bool(np.any(np.isclose(J, 0.995, rtol=1e-12)) or np.any(np.isclose(J, 1.004, rtol=1e-12)))
+        K = M.K
+        assert s.p.max() <= K * (1 / 0.995 - 1) * (1 + 1e-12) and s.p.min() >= K * (1 / 1.004 - 1) * (1 + 1e-12)
+    assert hit                                           # the window was reached
+
+
+def test_velocity_cap_and_its_position_correction():
+    # without the shift, which would move the particles by their new velocities too
+    free = _fast_block(None, pst=False)
+    capped = _fast_block({"VelocityLimit": {"cap": 150.0}}, pst=False)
+    lay0, st0, P0, S0 = solid_layout(free)
+    lay1, st1, P1, S1 = solid_layout(capped)
+    assert S1.v_cap == 150.0 and S0.v_cap == 0.0
+    a, b = solid_oracle_state(lay0, st0), solid_oracle_state(lay1, st1)
+    O.solid_step(lay0.dims, P0, S0, a)
+    O.solid_step(lay1.dims, P1, S1, b)
+    va, vb = a.by_pid("v"), b.by_pid("v")
+    free_p = a.by_pid("bc_flag") == 0
+    sp = np.linalg.norm(va, axis=1)
+    over = free_p & (sp > 150.0)
+    assert over.any()
+    np.testing.assert_allclose(np.linalg.norm(vb[over], axis=1), 150.0, rtol=1e-12)
+    np.testing.assert_array_equal(vb[~over], va[~over])
+    dt = b.scal[0]
+    np.testing.assert_allclose(b.by_pid("x") - a.by_pid("x"), (vb - va) * dt, rtol=1e-9, atol=1e-15)

@@ -1187,18 +1187,35 @@ template <typename R> struct GradVBody {
 // SOLID.py:4260-4300; sts_stage, 4492-4503: subcycle at (mu, nu) = (1, 0), or the RKL1
 // stage j of hourglass_sts, 4563-4576)
 // ------------------------------------------------------------------------------------
+//
+// In mixed precision the stage buffers hold the increment d = y - v0 over the stage
+// start v0 (the synchronised velocity, sd.vs) as float4, not y as double4: the later
+// stages then gather 16 B per neighbour instead of 32, and the operator's v0 part,
+// sum_j A_ij (M v0_j - v0_i), is stage-invariant and joins the stored source.
 template <typename R> struct HgBody {
     typedef SolidArgs<R> Args;
+    static constexpr bool DEL = sizeof(R) == 4;   // stage buffers hold d = y - v0 (float4)
     const Args& a;
-    double4 yi;
+    double4 yi;     // the full stage value y_i
+    float4 di;      // d_i (fast path, DEL)
     R Gi[3][3];
     R hx, hy, hz;
     R bx, by, bz;   // the stage-invariant part, sum_j w_j W (e . <G> r) e (mode 1)
     __device__ HgBody(const Args& a_) : a(a_), bx(0), by(0), bz(0) {}
+    // the full stage value of particle j from a stage buffer
+    __device__ double4 yload(const double4* buf, int j) const {
+        if (DEL) {
+            const float4 d = ((const float4*)buf)[j];
+            double4 v = a.sd.vs[j];
+            v.x += (double)d.x; v.y += (double)d.y; v.z += (double)d.z;
+            return v;
+        }
+        return buf[j];
+    }
     __device__ void begin(int i, double& x, double& y, double& z) {
         double4 xi = a.st.b.x[i];
         x = xi.x; y = xi.y; z = xi.z;
-        yi = a.ysrc[i];
+        yi = yload(a.ysrc, i);
         load_gv(a.sd.gv, i, (a.F.flags & SPH_TWO_D) != 0, Gi);
         hx = hy = hz = 0;
     }
@@ -1209,7 +1226,7 @@ template <typename R> struct HgBody {
         R rx = (R)rxd, ry = (R)ryd, rz = (R)rzd;
         R rn = fsqrt<R>(rx * rx + ry * ry + rz * rz);
         if (!(rn > (R)1e-9)) return;
-        double4 yj = a.ysrc[j];
+        double4 yj = yload(a.ysrc, j);
         if (mir & 1) yj.y = -yj.y;
         if (mir & 2) yj.z = -yj.z;
         R Gj[3][3];
@@ -1243,7 +1260,7 @@ template <typename R> struct HgBody {
             if (!two_d) A[2 * cstride] = 0;
             return;
         }
-        double4 yj = a.ysrc[j];
+        double4 yj = yload(a.ysrc, j);
         if (mir & 1) yj.y = -yj.y;
         if (mir & 2) yj.z = -yj.z;
         R Gj[3][3];
@@ -1265,28 +1282,48 @@ template <typename R> struct HgBody {
         if (!two_d) {
             A[2 * cstride] = uz;
             f += uz * dyz;
-            hz += f * uz;
         }
-        hx += f * ux; hy += f * uy;
-        bx += c * ex; by += c * ey; bz += c * ez;
+        if (DEL) {
+            // A (M v0_j - v0_i) holds over the stages too: it goes to the source
+            bx += c * ex + f * ux; by += c * ey + f * uy; bz += c * ez + f * uz;
+        } else {
+            hx += f * ux; hy += f * uy;
+            if (!two_d) hz += f * uz;
+            bx += c * ex; by += c * ey; bz += c * ez;
+        }
     }
     // a later stage: A_ij (M y_j - y_i) = u (u . (M y_j - y_i)) from the slot
     __device__ void begin_fast(int i) {
-        yi = a.ysrc[i];
+        if (DEL) {
+            di = ((const float4*)a.ysrc)[i];
+            yi = a.sd.vs[i];
+            yi.x += (double)di.x; yi.y += (double)di.y; yi.z += (double)di.z;
+        } else {
+            yi = a.ysrc[i];
+        }
         hx = hy = hz = 0;
     }
     __device__ void pair_fast(int j, int mir, size_t slot, size_t cstride, bool two_d) {
-        double4 yj = a.ysrc[j];
-        if (mir & 1) yj.y = -yj.y;
-        if (mir & 2) yj.z = -yj.z;
+        R dyx, dyy, dyz = 0;
+        if (DEL) {
+            float4 dj = ((const float4*)a.ysrc)[j];
+            if (mir & 1) dj.y = -dj.y;
+            if (mir & 2) dj.z = -dj.z;
+            dyx = (R)(dj.x - di.x); dyy = (R)(dj.y - di.y); dyz = (R)(dj.z - di.z);
+        } else {
+            double4 yj = a.ysrc[j];
+            if (mir & 1) yj.y = -yj.y;
+            if (mir & 2) yj.z = -yj.z;
+            dyx = (R)(yj.x - yi.x); dyy = (R)(yj.y - yi.y); dyz = (R)(yj.z - yi.z);
+        }
         const R* A = a.hgA + slot;
         R ux = A[0], uy = A[cstride];
-        R f = ux * (R)(yj.x - yi.x) + uy * (R)(yj.y - yi.y);
+        R f = ux * dyx + uy * dyy;
         if (two_d) {
             hx += f * ux; hy += f * uy;
         } else {
             R uz = A[2 * cstride];
-            f += uz * (R)(yj.z - yi.z);
+            f += uz * dyz;
             hx += f * ux; hy += f * uy; hz += f * uz;
         }
     }
@@ -1332,12 +1369,19 @@ template <typename R> struct HgBody {
             const double j = (double)(a.stage + 1);
             const double mu = (2.0 * j - 1.0) / j, nu = (1.0 - j) / j;
             const double mw = mu * (2.0 / (s * s + s)) * dt;
-            const double4 yp = a.ydst[i];
+            const double4 yp = yload(a.ydst, i);
             y.x = mu * y.x + nu * yp.x + mw * (k * (double)hx);
             y.y = mu * y.y + nu * yp.y + mw * (k * (double)hy);
             y.z = mu * y.z + nu * yp.z + mw * (k * (double)hz);
         }
-        a.ydst[i] = y;
+        if (DEL) {
+            const double4 v0 = a.sd.vs[i];
+            float4 d;
+            d.x = (float)(y.x - v0.x); d.y = (float)(y.y - v0.y); d.z = (float)(y.z - v0.z); d.w = 0.0f;
+            ((float4*)a.ydst)[i] = d;
+        } else {
+            a.ydst[i] = y;
+        }
     }
     __device__ void reduce(double*, int) {}
 };
@@ -1553,7 +1597,8 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_sts_kick1(DState<R> st, DSolid<R>
         v.x += h * (double)a.x; v.y += h * (double)a.y; v.z += h * (double)a.z;
         b.v[i] = v;
         sd.vs[i] = v;
-        sd.y0[i] = v;
+        if (sizeof(R) == 4) ((float4*)sd.y0)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);   // d = 0
+        else sd.y0[i] = v;
         b.V[i] = st.V_new[i];
         ke = 0.5 * (double)b.m[i] * (v.x * v.x + v.y * v.y + v.z * v.z);
     }
@@ -1572,7 +1617,15 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_sts_kick2(DGrid g, DFluid F, DSta
     if (i < st.n) {
         const DBuf<R>& b = st.b;
         const int stages = (int)st.scal[SPH_S_STS_STAGES];
-        double4 v = (stages & 1) ? sd.y1[i] : sd.y0[i];
+        double4 v;
+        if (sizeof(R) == 4) {
+            // mixed: the stage buffers hold d = y - v0
+            const float4 d = ((const float4*)((stages & 1) ? sd.y1 : sd.y0))[i];
+            v = sd.vs[i];
+            v.x += (double)d.x; v.y += (double)d.y; v.z += (double)d.z;
+        } else {
+            v = (stages & 1) ? sd.y1[i] : sd.y0[i];
+        }
         double m = (double)b.m[i];
         ke = 0.5 * m * (v.x * v.x + v.y * v.y + v.z * v.z);
         double dt = st.scal[SPH_S_DT], h = 0.5 * dt;

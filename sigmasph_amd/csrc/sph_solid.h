@@ -1840,7 +1840,7 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_build_list(DGrid g, int n, const 
                                                           const int4* __restrict__ cellid,
                                                           const double4* __restrict__ X, double* scal,
                                                           NList nl) {
-    const int t = blockIdx.x * SPH_BLOCK + threadIdx.x;
+    const int t = xcd_block() * SPH_BLOCK + threadIdx.x;
     const int i = t / L, lane = t % L;
     if (i >= n) return;   // whole groups: SPH_BLOCK is a multiple of L
     const int gbase = (threadIdx.x % SPH_WAVE) - lane;
@@ -1871,7 +1871,7 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_list(DGrid g, int n, typename Bod
     __shared__ double red[SPH_BLOCK / SPH_WAVE];
     if (a.skip()) return;   // uniform over the grid
     Body body(a);
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block() * blockDim.x + threadIdx.x;   // XCD-contiguous particle runs
     const int i = t / LPP, lane = t % LPP;
     const bool act = i < n;
     if (act) {
@@ -1908,7 +1908,7 @@ template <typename R, int LPP>
 __global__ __launch_bounds__(SPH_BLOCK) void k_hg_list(DGrid g, int n, SolidArgs<R> a, NList nl) {
     if (a.skip()) return;   // uniform over the grid
     HgBody<R> body(a);
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;   // not XCD-renumbered: ~3 % slower with it (3-D Taylor bar)
     const int i = t / LPP, lane = t % LPP;
     const size_t cstride = (size_t)nl.nmax * nl.npad;
     const bool act = i < n;
@@ -1949,7 +1949,7 @@ template <typename R, int LPP>
 __global__ __launch_bounds__(SPH_BLOCK) void k_hg_apply(int n, SolidArgs<R> a, NList nl) {
     if (a.skip()) return;   // uniform over the grid
     HgBody<R> body(a);
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;   // not XCD-renumbered: ~3 % slower with it (3-D Taylor bar)
     const int i = t / LPP, lane = t % LPP;
     const bool two_d = (a.F.flags & SPH_TWO_D) != 0;
     const size_t cstride = (size_t)nl.nmax * nl.npad;

@@ -421,11 +421,14 @@ int  sph_graphs(sph_ctx* ctx, int32_t enable);
  * stage: neighbour-list passes, any); SPH_TUNE_HG_PAIRS (0/1) switches the stored
  * hourglass operator; SPH_TUNE_LIST_CAP caps the neighbour-list entries a particle may
  * use (0 = the full list): particles with more neighbours take the traversal fallback,
- * which this knob lets a test exercise.  Results do not depend on the knobs beyond
- * summation order.  Setting a knob frees captured graphs. */
+ * which this knob lets a test exercise; SPH_TUNE_HG_FUSE is the most threads (default
+ * 2^20) at which a later hourglass sub-step re-evaluates the overflowed particles in its
+ * own launch rather than a second one (0: always a second launch).  Results do not depend
+ * on the knobs beyond summation order.  Setting a knob frees captured graphs. */
 #define SPH_TUNE_LANES     0
 #define SPH_TUNE_HG_PAIRS  8
 #define SPH_TUNE_LIST_CAP  9
+#define SPH_TUNE_HG_FUSE   10
 int  sph_tune(sph_ctx* ctx, int32_t knob, int32_t value);
 int  sph_tune_get(sph_ctx* ctx, int32_t knob, int32_t* value);
 

@@ -1285,6 +1285,7 @@ struct sph_ctx {
     bool hg_pairs = true;          // later stages read them instead of recomputing
     int list_alloc = 0;            // entries per particle the list was allocated with
     int list_cap = 0;              // SPH_TUNE_LIST_CAP, 0 = none
+    int64_t hg_fuse = (int64_t)1 << 20;   // SPH_TUNE_HG_FUSE
     // captured steps (sph_graphs): replayed when the same arguments come back
     bool graphs = false;
     GraphEntry graph[4];
@@ -1534,6 +1535,9 @@ extern "C" int sph_tune(sph_ctx* c, int32_t knob, int32_t value) {
         c->lpp[kind] = value;
     } else if (knob == SPH_TUNE_HG_PAIRS) {
         c->hg_pairs = value != 0;
+    } else if (knob == SPH_TUNE_HG_FUSE) {
+        if (value < 0) return set_err(c, SPH_EINVAL, "fuse threshold must be >= 0%s", "", 0);
+        c->hg_fuse = value;
     } else if (knob == SPH_TUNE_LIST_CAP) {
         if (value < 0) return set_err(c, SPH_EINVAL, "list cap must be >= 0%s", "", 0);
         c->list_cap = value;
@@ -1550,6 +1554,7 @@ extern "C" int sph_tune_get(sph_ctx* c, int32_t knob, int32_t* value) {
     if (knob >= SPH_TUNE_LANES && knob < SPH_TUNE_LANES + PASS_KINDS) *value = c->lpp[knob - SPH_TUNE_LANES];
     else if (knob == SPH_TUNE_HG_PAIRS) *value = c->hg_pairs ? 1 : 0;
     else if (knob == SPH_TUNE_LIST_CAP) *value = c->list_cap;
+    else if (knob == SPH_TUNE_HG_FUSE) *value = (int32_t)std::min<int64_t>(c->hg_fuse, INT32_MAX);
     else return set_err(c, SPH_EINVAL, "unknown tuning knob%s %lld", "", knob);
     return SPH_OK;
 }

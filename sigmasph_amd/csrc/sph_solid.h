@@ -1944,9 +1944,11 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_hg_list(DGrid g, int n, SolidArgs
 
 // A later hourglass sub-step (mode 2) of the listed particles: per slot the stored u and
 // y_j, y_dst = y + sub k (sum_j u (u . (M y_j - y_i)) + source).  The overflowed
-// particles are k_hg_overflow's, launched after this.
-template <typename R, int LPP>
-__global__ __launch_bounds__(SPH_BLOCK) void k_hg_apply(int n, SolidArgs<R> a, NList nl) {
+// particles are k_hg_overflow's, launched after this -- or, with OVF, this kernel's own
+// (a small system, fewer waves than SIMDs, loses nothing to the traversal's registers
+// and saves the launch).
+template <typename R, int LPP, bool OVF>
+__global__ __launch_bounds__(SPH_BLOCK) void k_hg_apply(DGrid g, int n, SolidArgs<R> a, NList nl) {
     if (a.skip()) return;   // uniform over the grid
     HgBody<R> body(a);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;   // not XCD-renumbered: ~3 % slower with it (3-D Taylor bar)
@@ -1970,6 +1972,19 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_hg_apply(int n, SolidArgs<R> a, N
     if (act && lane == 0) {
         body.source(i, true);
         body.end(i);
+    }
+    if (OVF && a.st.scal[SPH_S_LIST_OVER] != 0.0) {   // uniform over the grid
+        for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+            if (nl.cnt[p] <= nl.nmax) continue;
+            HgBody<R> ob(a);
+            double xi, yi, zi;
+            ob.begin(p, xi, yi, zi);
+            auto f = [&](int j, int mir, double rx, double ry, double rz) { ob.pair(j, mir, rx, ry, rz); };
+            Traverse tr{g, a.s.start, a.s.cellid, a.X, a.any_out()};
+            tr(p, xi, yi, zi, f);
+            ob.source(p, false);
+            ob.end(p);
+        }
     }
 }
 
@@ -2199,11 +2214,21 @@ static void launch_hg_pass(sph_ctx* c, const SolidArgs<R>& a, hipStream_t hs) {
     const int blocks = (int)(((int64_t)n * lpp + SPH_BLOCK - 1) / SPH_BLOCK);
     dim3 gb(blocks), bb(SPH_BLOCK);
     if (a.hg_mode == 2) {
+        // up to 2^20 threads (16 384 waves, 16 per SIMD) the overflow check rides along
+        if ((int64_t)n * lpp <= c->hg_fuse) {
+            switch (lpp) {
+            case 8: hipLaunchKernelGGL((k_hg_apply<R, 8, true>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
+            case 4: hipLaunchKernelGGL((k_hg_apply<R, 4, true>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
+            case 2: hipLaunchKernelGGL((k_hg_apply<R, 2, true>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
+            default: hipLaunchKernelGGL((k_hg_apply<R, 1, true>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
+            }
+            return;
+        }
         switch (lpp) {
-        case 8: hipLaunchKernelGGL((k_hg_apply<R, 8>), gb, bb, 0, hs, n, a, c->nl); break;
-        case 4: hipLaunchKernelGGL((k_hg_apply<R, 4>), gb, bb, 0, hs, n, a, c->nl); break;
-        case 2: hipLaunchKernelGGL((k_hg_apply<R, 2>), gb, bb, 0, hs, n, a, c->nl); break;
-        default: hipLaunchKernelGGL((k_hg_apply<R, 1>), gb, bb, 0, hs, n, a, c->nl); break;
+        case 8: hipLaunchKernelGGL((k_hg_apply<R, 8, false>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
+        case 4: hipLaunchKernelGGL((k_hg_apply<R, 4, false>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
+        case 2: hipLaunchKernelGGL((k_hg_apply<R, 2, false>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
+        default: hipLaunchKernelGGL((k_hg_apply<R, 1, false>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
         }
         hipLaunchKernelGGL((k_hg_overflow<R>), dim3(8), bb, 0, hs, c->grid, n, a, c->nl);
         return;

@@ -71,12 +71,12 @@ def test_autotune_leaves_valid_shapes(gpu):
     ps.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_list_overflow_fallback_matches_the_oracle(gpu, precision):
+@pytest.mark.parametrize("precision,fuse", [("fp64", 1), ("mixed", 1), ("fp64", 0)])
+def test_list_overflow_fallback_matches_the_oracle(gpu, precision, fuse):
     """With the list capped at 16 entries, interior particles (about 28 neighbours with
     their mirror images) overflow and take the traversal in every list pass and in every
-    hourglass sub-step (k_hg_overflow), while corner particles stay listed; the run must
-    still match the oracle step for step."""
+    hourglass sub-step (inside k_hg_apply, or k_hg_overflow's own launch with `fuse` 0),
+    while corner particles stay listed; the run must still match the oracle step for step."""
     import sys, os
     sys.path.insert(0, os.path.dirname(__file__))
     from oracle import oracle as O
@@ -89,6 +89,8 @@ def test_list_overflow_fallback_matches_the_oracle(gpu, precision):
     s = solid_oracle_state(ps, st)
     solver.initialize()
     assert ps._lib.sph_tune(ps.ctx, N.SPH_TUNE_LIST_CAP, 16) == N.SPH_OK
+    if not fuse:                                     # the overflowed particles' own launch
+        assert ps._lib.sph_tune(ps.ctx, N.SPH_TUNE_HG_FUSE, 0) == N.SPH_OK
     v = ctypes.c_int32()
     assert ps._lib.sph_tune_get(ps.ctx, N.SPH_TUNE_LIST_CAP, ctypes.byref(v)) == N.SPH_OK and v.value == 16
     for k in range(3):

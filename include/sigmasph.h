@@ -250,6 +250,9 @@ typedef struct {
     double  erosion_j_min, erosion_j_max;   /* (apply_erosion, SOLID.py:4906-4959)          */
     double  v_cap;           /* VelocityLimit.cap: |v| ceiling of unconstrained particles,
                               * 0 = off (cap_velocity, SOLID.py:4789-4806)                  */
+    int32_t hg_mixed;        /* materials differ in rho0 c_p: the damper's pair terms carry
+                              * 2 Z_j / (Z_i + Z_j) (_hg_impedance_ratio, SOLID.py:4184)    */
+    int32_t pad3_;
 } sph_solid_params;
 
 /* sph_solid_params.sts: how the s passes of a step are spent (hourglass_sts,

@@ -496,6 +496,19 @@ typedef struct {
 static int or_same_material(const or_state* s, int64_t i, int64_t j) {
     return s->eos_id == NULL || s->eos_id[i] == s->eos_id[j];
 }
+/* c0_of: the particle's own EOS sound speed, or the scene's c0 on a fluid state (DSPH.py:513-519,
+ * SOLID.py:4075-4090); pair_c0 is the mean of two (DSPH.py:483-510), local_c0 one's own (520-530) */
+static double or_c0_of(const sph_fluid_params* P, const or_state* s, int64_t i) {
+    return (s->eos_id == NULL || s->mat == NULL) ? P->c0 : s->mat[s->eos_id[i]].c0;
+}
+/* _hg_impedance_ratio (SOLID.py:4184-4240): Z_pair / Z_i with the harmonic mean
+ * Z_pair = 2 Z_i Z_j / (Z_i + Z_j), written 2 Z_j / (Z_i + Z_j), Z = rho0 c_p */
+static double or_hg_impedance_ratio(const or_state* s, int64_t i, int64_t j) {
+    const sph_material* Mi = &s->mat[s->eos_id[i]];
+    const sph_material* Mj = &s->mat[s->eos_id[j]];
+    double zi = Mi->rho0 * Mi->c_p, zj = Mj->rho0 * Mj->c_p, den = zi + zj;
+    return den > 0.0 ? 2.0 * zj / den : 1.0;
+}
 static double or_pair_delta(const sph_fluid_params* P, const or_state* s, int64_t i, int64_t j) {
     if (s->eos_id == NULL) return P->delta_sph;
     double di = s->mat[s->eos_id[i]].delta, dj = s->mat[s->eos_id[j]].delta;
@@ -594,7 +607,8 @@ static void or_depsvol_task(void* c, int64_t i, int64_t j, int mir, const double
         double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
         double pg = 0.0;
         for (int k = 0; k < 3; ++k) pg += (2.0 * (rho_j - rho_i) * (-r[k]) / (r2 + eta2)) * gW[k];
-        double diffusion = delta * h * P->c0 * wj * pg;
+        double c0p = 0.5 * (or_c0_of(P, s, i) + or_c0_of(P, s, j));     /* pair_c0 */
+        double diffusion = delta * h * c0p * wj * pg;
         double diff_rate = diffusion / rho_i;
         t->acc[0] += diff_rate;
         t->acc[1] += diff_rate;
@@ -831,12 +845,12 @@ static void or_apply_pst(const sph_dims* d, const sph_fluid_params* P, or_state*
         double vn = sqrt(vi[0] * vi[0] + vi[1] * vi[1] + vi[2] * vi[2]);
         double Ma;
         if (P->pst_ma_mode == 2) Ma = P->pst_ma_fixed;
-        else if (P->pst_ma_mode == 1) Ma = s->scal[SPH_S_MAX_V] / P->c0;
-        else Ma = vn / P->c0;
+        else if (P->pst_ma_mode == 1) Ma = s->scal[SPH_S_MAX_V] / or_c0_of(P, s, i);   /* local_c0 */
+        else Ma = vn / or_c0_of(P, s, i);
         double h = d->h, dr[3];
         if (P->pst_colle) {
             double dv[3];
-            for (int k = 0; k < 3; ++k) dv[k] = -2.0 * h * P->c0 * Ma * t.acc[k];
+            for (int k = 0; k < 3; ++k) dv[k] = -2.0 * h * or_c0_of(P, s, i) * Ma * t.acc[k];
             if (P->pst_clamp_vel) {
                 double cap = P->pst_m_delta * vn;
                 double dn = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
@@ -1615,6 +1629,7 @@ static void or_stress_task(void* c, int64_t i, int64_t j, int mir, const double*
             double rho_i = s->m[i] / s->V[i];
             double damp = S->hourglass * (M->rho0 / rho_i) * (M->c_p / d->h) * (wj / S->V0)
                           * (Wij / S->w_dx0) * hg;
+            damp *= or_hg_impedance_ratio(s, i, j);
             double ah[3] = { damp * rh[0], damp * rh[1], damp * rh[2] };
             for (int k = 0; k < 3; ++k) t->acc[k] += ah[k];
             t->acc[5] += -0.5 * (ah[0] * vij[0] + ah[1] * vij[1] + ah[2] * vij[2]);
@@ -1744,6 +1759,7 @@ static void or_hg_task(void* c, int64_t i, int64_t j, int mir, const double* r) 
     double wj = or_w(d, t->P, s, j);
     double damp = S->hourglass * (M->rho0 / rho_i) * (M->c_p / d->h) * (wj / S->V0)
                   * (Wij / S->w_dx0) * hg;
+    damp *= or_hg_impedance_ratio(s, i, j);
     for (int k = 0; k < 3; ++k) t->acc[k] += damp * rh[k];
 }
 

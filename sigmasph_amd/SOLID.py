@@ -11,11 +11,14 @@ Johnson-Cook return maps, the Jaumann rate, ALE transport of the deviator, the p
 strain and the internal energy under the shift (with the upwind energy diffusion),
 Morris viscosity, the VNR bulk viscosity and Monaghan's pair viscosity with their dt
 limits, the rate hourglass damper (explicit with its dt cap, or super-time-stepped:
-subcycle or RKL1), the load ramp and displacement grips, axisymmetry and plane strain.
-Everything else the reference offers on this class raises NotImplementedError when a
-scene asks for it: JWL / HJC, damage and erosion, the conservative axisymmetric forms,
-the viscous / adaptive / saturating hourglass forms, the velocity limiters, and a table
-whose materials do not share one EOS sound speed and one acoustic impedance.
+subcycle or RKL1), the load ramp and displacement grips, axisymmetry and plane strain,
+scalar damage (spall threshold, Johnson-Cook fracture, Cocks-Ashby void growth), the
+erosion cap and the hard velocity cap.  Everything else the reference offers on this
+class raises NotImplementedError when a scene asks for it: JWL / HJC, the erode mode,
+the conservative axisymmetric forms, the viscous / adaptive / saturating hourglass
+forms and the conserving velocity limiter.
+Tables of several materials read each particle's own sound speed (the density diffusion
+and the shift) and scale the hourglass damper across an impedance jump.
 """
 import ctypes
 import math
@@ -65,12 +68,10 @@ class SolidSettings:
         c = ps.cfg.get_cfg
         self.materials = ps.materials
         mats = list(self.materials.values())
-        if len({round(m.c_eos, 12) for m in mats}) > 1:
-            raise NotImplementedError("materials with different EOS sound speeds (the c0_mixed "
-                                      "branch, SOLID.py:244-245) are not part of this build")
-        if len({round(m.rho0 * m.c_p, 12) for m in mats}) > 1:
-            raise NotImplementedError("materials with different acoustic impedances (the "
-                                      "hourglass impedance ratio, SOLID.py:229) are not built")
+        # a pair straddling an impedance jump scales the damper by 2 Z_j / (Z_i + Z_j),
+        # Z = rho0 c_p (SOLID.py:222-229, 4184-4230); the per-material sound speeds of the
+        # density diffusion and the shift are read from the table whatever the deck
+        self.hg_mixed = len({m.rho0 * m.c_p for m in mats}) > 1
         for name, what in (("AXI_CONSERVATIVE_X", "Domain.conservativeAxialMomentum"),
                            ("AXI_CONSERVATIVE_R", "Domain.conservativeRadialForce"),
                            ("PST_ALE_E_CONSERVATIVE", "Shifting.aleEnergyConservative")):
@@ -205,6 +206,7 @@ class SolidSettings:
         S.erosion_cap = int(self.erosion_mode == "cap")
         S.erosion_j_min, S.erosion_j_max = self.erosion_j_min, self.erosion_j_max
         S.v_cap = self.v_cap
+        S.hg_mixed = int(self.hg_mixed)
         return S
 
     def load_factor(self, t):

@@ -568,7 +568,11 @@ template <typename R> struct PassArgs {
     const double4* X;
     int pstride;     // stride of one partial-sum slot
     const int32_t* bcf;   // solid: constrained components per particle (null on a fluid)
+    const int32_t* eos;   // solid: material row per particle, for its own c0 (null on a fluid)
+    const sph_material* mat;
     __device__ bool any_out() const { return st.scal[SPH_S_OUT_OF_BOX] > 0.5; }
+    // local_c0 (DSPH.py:520-530): the particle's own EOS sound speed, the scene's on a fluid
+    __device__ double c0_of(int i) const { return (eos && mat) ? mat[eos[i]].c0 : F.c0; }
 };
 
 // ------------------------------------------------------------------------------------
@@ -956,12 +960,13 @@ template <typename R> struct ShiftBody {
         double4 v = b.v[i];
         double vn = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
         double Ma;
+        const double c0i = a.c0_of(i);
         if (F.ma_mode == 2) Ma = F.pst_ma_fixed;
-        else if (F.ma_mode == 1) Ma = st.scal[SPH_S_MAX_V] / F.c0;
-        else Ma = vn / F.c0;
+        else if (F.ma_mode == 1) Ma = st.scal[SPH_S_MAX_V] / c0i;
+        else Ma = vn / c0i;
         double drx, dry, drz;
         if (F.colle) {
-            double kk = -2.0 * F.h * F.c0 * Ma;
+            double kk = -2.0 * F.h * c0i * Ma;
             double dvx = kk * sx, dvy = kk * sy, dvz = kk * sz;
             if (F.clamp_vel) {
                 double cap = F.m_delta * vn;
@@ -1726,6 +1731,8 @@ template <typename R> static PassArgs<R> pass_args(sph_ctx* c, sph_state* st, co
     a.X = (const double4*)st->buf[st->cur].x;
     a.pstride = c->pstride;
     a.bcf = nullptr;
+    a.eos = nullptr;
+    a.mat = nullptr;
     return a;
 }
 

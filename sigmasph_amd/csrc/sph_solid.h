@@ -46,6 +46,7 @@ struct DSolidP {
     double sur_tol;
     int damage_on, jcf_on, transport_jcf;   // damage D live; JC fracture omega live; moved by the shift
     int ca_on, transport_ca;             // Cocks-Ashby porosity live; moved by the shift
+    int hg_mixed;                        // the damper's pair terms carry 2 Z_j / (Z_i + Z_j)
     int ero_cap;                         // erosion "cap": J clamped into [ero_jmin, ero_jmax]
     double ero_jmin, ero_jmax;
     double v_cap;                        // |v| ceiling of unconstrained particles, 0 = off
@@ -401,7 +402,10 @@ template <typename R, bool DMG = false> struct StressBody {
             // same-material pairs only: pair_delta (SOLID.py:4102-4114) of two equal rows
             double di = a.sd.mat[eos_i].delta, den = di + di;
             double delta = den > 1e-12 ? 2.0 * di * di / den : 0.0;
-            kdelta = (R)(delta * a.F.h * a.F.c0);
+            // sized on pair_c0 (DSPH.py:483-510), the mean of the two EOS sound speeds:
+            // the material's own on a same-material pair
+            const double c0i = a.sd.mat[eos_i].c0;
+            kdelta = (R)(delta * a.F.h * (0.5 * (c0i + c0i)));
         }
         si = load_sig(a.sd.sig_in, i, two_d);
         epi = a.sd.ep_in[i];
@@ -965,6 +969,7 @@ template <typename R, bool HGX = true> struct SolidForceBody {
             R q = rn * (R)F.inv_h, om = (R)1 - q;
             R W = (R)F.alpha * om * om * om * om * ((R)4 * q + (R)1);
             R k = wj * W * hg;
+            if (S.hg_mixed) k *= (R)hg_impedance_ratio(a.sd.mat, eos_i, a.sd.eos[j]);
             hx += k * ex; hy += k * ey; hz += k * ez;
             whg += k * (ex * dvx + ey * dvy + ez * dvz);
         }
@@ -1182,6 +1187,15 @@ template <typename R> struct GradVBody {
     __device__ void reduce(double*, int) {}
 };
 
+// _hg_impedance_ratio (SOLID.py:4184-4240): Z_pair / Z_i with Z_pair the harmonic mean of
+// the two acoustic impedances Z = rho0 c_p, written 2 Z_j / (Z_i + Z_j) so that equal
+// impedances give exactly 1
+__device__ __forceinline__ double hg_impedance_ratio(const sph_material* mat, int eos_i, int eos_j) {
+    const double zi = mat[eos_i].rho0 * mat[eos_i].c_p, zj = mat[eos_j].rho0 * mat[eos_j].c_p;
+    const double den = zi + zj;
+    return den > 0.0 ? 2.0 * zj / den : 1.0;
+}
+
 // ------------------------------------------------------------------------------------
 // one forward-Euler hourglass sub-step on the stage velocities (_hg_rate_accel,
 // SOLID.py:4260-4300; sts_stage, 4492-4503: subcycle at (mu, nu) = (1, 0), or the RKL1
@@ -1198,6 +1212,7 @@ template <typename R> struct HgBody {
     const Args& a;
     double4 yi;     // the full stage value y_i
     float4 di;      // d_i (fast path, DEL)
+    int eos_i;      // particle i's material row (the impedance ratio)
     R Gi[3][3];
     R hx, hy, hz;
     R bx, by, bz;   // the stage-invariant part, sum_j w_j W (e . <G> r) e (mode 1)
@@ -1216,6 +1231,7 @@ template <typename R> struct HgBody {
         double4 xi = a.st.b.x[i];
         x = xi.x; y = xi.y; z = xi.z;
         yi = yload(a.ysrc, i);
+        eos_i = a.sd.eos[i];
         load_gv(a.sd.gv, i, (a.F.flags & SPH_TWO_D) != 0, Gi);
         hx = hy = hz = 0;
     }
@@ -1241,6 +1257,7 @@ template <typename R> struct HgBody {
         R q = rn * (R)F.inv_h, om = (R)1 - q;
         R W = (R)F.alpha * om * om * om * om * ((R)4 * q + (R)1);
         R k = wq<R>(F, b, b.V, j) * W * hg;
+        if (a.S.hg_mixed) k *= (R)hg_impedance_ratio(a.sd.mat, eos_i, a.sd.eos[j]);
         hx += k * ex; hy += k * ey; hz += k * ez;
     }
     // The first stage of a list slot: the same term split as A_ij (y_j - y_i) + c_ij e_ij
@@ -1273,6 +1290,7 @@ template <typename R> struct HgBody {
         R gz = (R)0.5 * ((Gi[2][0] + Gj[2][0]) * rx + (Gi[2][1] + Gj[2][1]) * ry + (Gi[2][2] + Gj[2][2]) * rz);
         R q = rn * (R)F.inv_h, om = (R)1 - q;
         R sw = wq<R>(F, b, b.V, j) * ((R)F.alpha * om * om * om * om * ((R)4 * q + (R)1));
+        if (a.S.hg_mixed) sw *= (R)hg_impedance_ratio(a.sd.mat, eos_i, a.sd.eos[j]);
         R dyx = (R)(yj.x - yi.x), dyy = (R)(yj.y - yi.y), dyz = (R)(yj.z - yi.z);
         R c = sw * (gx * ex + gy * ey + gz * ez);
         R su = fsqrt<R>(fmax(sw, (R)0));
@@ -2067,6 +2085,7 @@ static DSolidP make_solid_params(const sph_ctx* c, const sph_fluid_params* P, co
     d.damage_on = S->damage != 0;
     d.jcf_on = S->damage && S->jc_failure;
     d.transport_jcf = d.jcf_on && d.transport;
+    d.hg_mixed = S->hg_mixed != 0;
     d.ero_cap = S->erosion_cap != 0;
     d.v_cap = S->v_cap;
     d.ero_jmin = S->erosion_j_min;
@@ -2322,6 +2341,8 @@ static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F,
     if (F.pst_en) {
         PassArgs<R> pa = pass_args<R>(c, st, F);
         pa.bcf = sd->bc_flag[st->cur];
+        pa.eos = sd->eos_id[st->cur];
+        pa.mat = sd->mat;
         timer_begin(c, 2, hs);
         launch_pair_pass<ShiftBody<R>>(c, pa, hs, PASS_SHIFT);
         timer_end(c, 2, hs);

@@ -166,6 +166,25 @@ def solid_block(n=(20, 8, 1), dx=0.1, two_d="planeStrain", jitter=0.0, seed=0,
     return sc
 
 
+def bimaterial_block(n=(20, 5, 1), dx=0.1, material=None, density=2.79e-06, velocity=(0.0, 0.0, 0.0),
+                     lower_velocity=(0.0, 0.0, 0.0), **kw):
+    """Two `solid_block` lattices of different materials stacked along y and bonded at
+    their shared face (the upper one one spacing above the lower's top row, so the pairs
+    across it are ordinary neighbour pairs): the lower block is `solid_block`'s steel,
+    the upper one `material` (default aluminium 2017-T4 with its Mie-Grueneisen EOS) at
+    `density`.  Both blocks take `solid_block`'s other arguments."""
+    sc = solid_block(n=n, dx=dx, velocity=lower_velocity, **kw)
+    low = sc["SolidBlocks"][0]
+    mat = material or {"name": "aluminium", "library": "al2017_t4"}
+    up = dict(low, objectId=1, material=mat["name"], density=density, velocity=list(velocity),
+              seed=low.get("seed", 0) + 1)
+    up["start"] = [low["start"][0], low["start"][1] + n[1] * dx, low["start"][2]]
+    up["end"] = [low["end"][0], low["end"][1] + n[1] * dx, low["end"][2]]
+    sc["SolidBlocks"].append(up)
+    sc["Materials"].append(mat)
+    return sc
+
+
 def dogbone_c2(dx0=0.1, elongation=3.0, rate=1.0, cfl=0.3):
     """The axisymmetric round-bar tension test: (x, r) half-plane at dx0 (innermost ring
     at r = dx0/2), 10 mm heads 15 mm long, 6 mm gauge 30 mm long, 5 mm fillets; steel

@@ -5,7 +5,8 @@ branch runs within a few steps (an initial velocity field past the yield strain 
 grips on a short ramp): plane strain and axisymmetry with the super-time-stepped
 hourglass damper, Colle shifting and ALE transport, Johnson-Cook; a 3-D block with the
 explicit damper, J2 hardening and the corrected grad v; a Mie-Grueneisen aluminium bar
-under compression with the VNR bulk viscosity and Monaghan's pair viscosity.  fp64: every field to 1e-9 of
+under compression with the VNR bulk viscosity and Monaghan's pair viscosity; steel bonded
+to aluminium (two materials, the damper's pair impedance).  fp64: every field to 1e-9 of
 its scale after one step and 1e-7 after five, the step's scalars and the grips alike.
 Mixed mode (float pair sums and per-particle fields, f64 positions, velocities, EOS,
 return map and reductions): every field within 2e-4 of its scale over three steps.
@@ -19,7 +20,7 @@ from sigmasph_amd import _native as N
 from sigmasph_amd.config import SimConfig
 from sigmasph_amd.particle_system import ParticleSystem
 from sigmasph_amd.SOLID import SolidSettings
-from sigmasph_amd.scenes import solid_block, tensile_ensemble, taylor_bar
+from sigmasph_amd.scenes import bimaterial_block, solid_block, tensile_ensemble, taylor_bar
 from helpers import solid_oracle_state
 
 pytestmark = pytest.mark.gpu
@@ -67,6 +68,14 @@ SCENES = {
         n=(16, 8, 1), div_velocity=(-300.0, 200.0, 0.0), jitter=0.02, grips=0.05, ramp_time=1e-4,
         extra={"Erosion": {"mode": "cap", "minVolumeRatio": 0.995, "maxVolumeRatio": 1.004},
                "VelocityLimit": {"cap": 150.0}}),
+    # steel driven into aluminium across a bonded face (impedance ratio 2.5 : 1, sound
+    # speeds 4.5 / 5.3 km/s): the damper's pair impedance, the shift's own sound speed;
+    # super-time-stepped and explicit dampers
+    "bimaterial": lambda: bimaterial_block(n=(20, 5, 1), jitter=0.02, div_velocity=(300.0, -150.0, 0.0),
+                                           lower_velocity=(0.0, 120.0, 0.0), velocity=(0.0, -120.0, 0.0)),
+    "bimaterial_explicit": lambda: bimaterial_block(n=(20, 5, 1), jitter=0.02, sts=False,
+                                                    div_velocity=(300.0, -150.0, 0.0),
+                                                    lower_velocity=(0.0, 120.0, 0.0), velocity=(0.0, -120.0, 0.0)),
     "plane_strain_noshift": lambda: solid_block(n=(20, 10, 1), pst=False, viscosity=0.0,
                                                 div_velocity=(-200.0, 250.0, 0.0), jitter=0.03),
     # nine specimens with their own Johnson-Cook rows and 18 grips (past 16 the grip
@@ -186,13 +195,15 @@ def test_five_steps_fp64(gpu, name):
         assert s.damage.max() > 0.0                      # the damage model acted
         if name == "jc_fracture":
             assert s.jc_omega.max() > 1.0 and 0.0 < s.damage.max() < 1.0
+    if name.startswith("bimaterial"):
+        assert st.solid_params().hg_mixed == 1 and set(np.unique(s.eos_id)) == {0, 1}
     if name == "plane_strain_rkl":
         assert st.solid_params().sts == N.SPH_STS_RKL and s.scal[N.SPH_S_STS_STAGES] >= 2
 
 
 @pytest.mark.parametrize("name", ["plane_strain_grips", "axisymmetric_grips", "block3d_explicit",
                                   "axisymmetric_shock", "taylor3d", "plane_strain_rkl", "jc_fracture",
-                                  "spall_threshold", "cocks_ashby", "limiters"])
+                                  "spall_threshold", "cocks_ashby", "limiters", "bimaterial"])
 def test_three_steps_mixed(gpu, name):
     ps, solver, st, s = run_pair(SCENES[name](), 3, precision="mixed")
     compare(ps, st, s, 2e-4, eps=2.0 ** -24)

@@ -263,6 +263,75 @@ def test_hourglass_operator():
     assert np.sum(s.m * np.einsum("ij,ij->i", y, a)) < 0.0  # it only removes energy
 
 
+def _bimaterial():
+    from sigmasph_amd.scenes import bimaterial_block
+    lay, st, P, S, s = _sorted_solid(bimaterial_block(n=(12, 5, 1), jitter=0.05, pst=False))
+    rows = [st.materials[k] for k in st.materials]
+    z = [m.rho0 * m.c_p for m in rows]
+    return lay, st, P, S, s, z
+
+
+def test_hourglass_across_an_impedance_jump():
+    """Steel bonded to aluminium (Z = rho0 c_p 2.5 : 1).  A pair straddling the jump
+    carries the harmonic mean 2 Z_i Z_j / (Z_i + Z_j) (SOLID.py:4184-4230): with y a unit
+    impulse on one particle k of the steel and grad v = 0, each aluminium neighbour i
+    feels only the pair (i, k), and its acceleration is the one it feels when the whole
+    table is aluminium times 2 Z_steel / (Z_steel + Z_al); the steel neighbours feel
+    exactly their all-steel value.  Across the jump the damper still conserves
+    momentum, which an impedance of particle i alone (the ratio dropped) or of the
+    other particle alone would not."""
+    lay, st, P, S, s, z = _bimaterial()
+    assert st.hg_mixed and S.hg_mixed == 1 and z[0] > 2.0 * z[1]
+    eos = s.eos_id.copy()
+    s.grad_v[:] = 0.0
+    low = np.where(eos == 0)[0]
+    k = low[np.argmax(s.x[low, 1])]                     # a steel particle on the interface
+    y = np.zeros((s.n, 3)); y[k] = [0.3, -0.8, 0.0]
+    a_mix = O.solid_hg_accel(lay.dims, P, S, s, y)
+    s.eos_id[:] = 1
+    a_al = O.solid_hg_accel(lay.dims, P, S, s, y)
+    s.eos_id[:] = 0
+    a_st = O.solid_hg_accel(lay.dims, P, S, s, y)
+    s.eos_id[:] = eos
+    al = (eos == 1) & (np.abs(a_al).sum(axis=1) > 0)
+    st_nb = (eos == 0) & (np.arange(s.n) != k) & (np.abs(a_st).sum(axis=1) > 0)
+    assert al.sum() >= 3 and st_nb.sum() >= 3
+    np.testing.assert_allclose(a_mix[al], a_al[al] * 2.0 * z[0] / (z[0] + z[1]), rtol=1e-13, atol=0)
+    np.testing.assert_array_equal(a_mix[st_nb], a_st[st_nb])
+    rng = np.random.default_rng(13)
+    y = rng.normal(size=(s.n, 3)); y[:, 2] = 0.0
+    s.grad_v[:] = rng.normal(size=(s.n, 9)) * 0.5
+    gv = s.grad_v.reshape(-1, 3, 3); gv[:, 2, :] = 0.0; gv[:, :, 2] = 0.0
+    a = O.solid_hg_accel(lay.dims, P, S, s, y)
+    mom = (s.m[:, None] * a).sum(axis=0)
+    assert np.all(np.abs(mom) < 1e-12 * np.sum(np.abs(s.m[:, None] * a)))
+
+
+def test_one_material_in_two_rows_is_the_one_material_step():
+    """The mixed-material forms reduce to the single-material step exactly when the two
+    rows are one material under two names: the impedance ratio is 2 Z / (Z + Z) = 1 to
+    the bit and the other terms read the shared values.  (No density diffusion and no
+    shift: their pair sums run over same-material pairs, decided on the row, so two rows
+    of one material are not one material to them.)"""
+    from sigmasph_amd.scenes import bimaterial_block, _steel
+    kw = dict(n=(12, 5, 1), jitter=0.05, div_velocity=(300.0, -150.0, 0.0), velocity=(0.0, -50.0, 0.0),
+              delta=0.0, pst=False)
+    out = []
+    for twin in (False, True):
+        mat = dict(_steel(), name="steel_b") if twin else _steel()
+        sc = bimaterial_block(material=mat, density=7.9e-06, **kw)
+        if not twin:
+            sc["Materials"].pop()
+        lay, st, P, S = solid_layout(sc)
+        s = solid_oracle_state(lay, st)
+        for _ in range(2):
+            O.solid_step(lay.dims, P, S, s)
+        out.append(s)
+    assert out[1].eos_id.max() == 1 and out[0].eos_id.max() == 0
+    for f in ("x", "v", "sig", "V", "e_int", "eps_p"):
+        np.testing.assert_array_equal(getattr(out[1], f), getattr(out[0], f), err_msg=f)
+
+
 @pytest.mark.parametrize("mode", ["subcycle", "rkl"])
 def test_sts_stages_are_the_stability_polynomial(mode):
     """hourglass_sts on a linear damper (grad v = 0, so the damper is y -> A y): s

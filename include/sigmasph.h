@@ -405,7 +405,9 @@ int  sph_reduce(sph_ctx* ctx, const sph_state* st, int32_t which, const double* 
 /* Average device duration in milliseconds of the pair kernels launched since the last
  * reset, measured with HIP events on the stream they run on (which = 0: volume pass,
  * 1: force pass, 2: shift pass, 3: sort, 4: solid stress pass, 5: solid force pass,
- * 6: hourglass sub-step pass).  Enabled by sph_timing(ctx, 1). */
+ * 6: hourglass sub-step pass).  Enabled by sph_timing(ctx, 1).  A step's hourglass
+ * sub-steps are bracketed by one pair of events and counted per sub-step, so their
+ * average carries no event dispatch latency per sub-step. */
 int  sph_timing(sph_ctx* ctx, int32_t enable);
 
 /* Whole-step graphs (no counterpart in the reference): with enable != 0, sph_solid_step

@@ -1236,6 +1236,7 @@ __global__ __launch_bounds__(1024) void k_reduce_final(DScratch s, int nblk, int
 // ====================================================================================
 struct TimerKind {
     std::vector<hipEvent_t> ev;   // pairs
+    std::vector<int> launches;    // kernel launches between each pair
     int used = 0;
 };
 
@@ -1353,11 +1354,16 @@ static void timer_begin(sph_ctx* c, int k, hipStream_t st) {
     if (t.used >= TIMER_SLOTS) return;
     (void)hipEventRecord(t.ev[2 * t.used], st);
 }
-static void timer_end(sph_ctx* c, int k, hipStream_t st) {
+// `launches`: the units (launches, or hourglass sub-steps) the pair brackets.  A run of
+// short launches is bracketed once, so the per-unit average does not carry an event's
+// dispatch latency per unit.
+static void timer_end(sph_ctx* c, int k, hipStream_t st, int launches = 1) {
     if (!c->timing) return;
     TimerKind& t = c->tk[k];
     if (t.used >= TIMER_SLOTS) return;
     (void)hipEventRecord(t.ev[2 * t.used + 1], st);
+    if ((int)t.launches.size() < TIMER_SLOTS) t.launches.assign(TIMER_SLOTS, 1);
+    t.launches[t.used] = launches;
     t.used++;
 }
 
@@ -1575,14 +1581,16 @@ extern "C" int sph_timing_read(sph_ctx* c, int32_t which, double* avg_ms, int64_
     if (!c || which < 0 || which > 6 || !avg_ms || !launches) return SPH_EINVAL;
     TimerKind& t = c->tk[which];
     double tot = 0.0;
+    int64_t cnt = 0;
     for (int k = 0; k < t.used; ++k) {
         HIPCHECK(c, hipEventSynchronize(t.ev[2 * k + 1]));
         float ms = 0.f;
         HIPCHECK(c, hipEventElapsedTime(&ms, t.ev[2 * k], t.ev[2 * k + 1]));
         tot += ms;
+        cnt += t.launches[k];
     }
-    *launches = t.used;
-    *avg_ms = t.used ? tot / t.used : 0.0;
+    *launches = cnt;
+    *avg_ms = cnt ? tot / cnt : 0.0;
     return SPH_OK;
 }
 

@@ -1206,13 +1206,16 @@ __device__ __forceinline__ double hg_impedance_ratio(const sph_material* mat, in
 // start v0 (the synchronised velocity, sd.vs) as float4, not y as double4: the later
 // stages then gather 16 B per neighbour instead of 32, and the operator's v0 part,
 // sum_j A_ij (M v0_j - v0_i), is stage-invariant and joins the stored source.
-template <typename R> struct HgBody {
+// MIX: the pair terms carry the impedance ratio when the deck's flag is up.  The
+// fused overflow path of k_hg_apply is built without it (the ratio's registers would
+// cost that kernel a wave per SIMD), so mixed-impedance decks take k_hg_overflow.
+template <typename R, bool MIX = true> struct HgBody {
     typedef SolidArgs<R> Args;
     static constexpr bool DEL = sizeof(R) == 4;   // stage buffers hold d = y - v0 (float4)
     const Args& a;
     double4 yi;     // the full stage value y_i
     float4 di;      // d_i (fast path, DEL)
-    int eos_i;      // particle i's material row (the impedance ratio)
+    int i_;         // particle i (its material row is read only for the impedance ratio)
     R Gi[3][3];
     R hx, hy, hz;
     R bx, by, bz;   // the stage-invariant part, sum_j w_j W (e . <G> r) e (mode 1)
@@ -1231,7 +1234,7 @@ template <typename R> struct HgBody {
         double4 xi = a.st.b.x[i];
         x = xi.x; y = xi.y; z = xi.z;
         yi = yload(a.ysrc, i);
-        eos_i = a.sd.eos[i];
+        i_ = i;
         load_gv(a.sd.gv, i, (a.F.flags & SPH_TWO_D) != 0, Gi);
         hx = hy = hz = 0;
     }
@@ -1257,7 +1260,7 @@ template <typename R> struct HgBody {
         R q = rn * (R)F.inv_h, om = (R)1 - q;
         R W = (R)F.alpha * om * om * om * om * ((R)4 * q + (R)1);
         R k = wq<R>(F, b, b.V, j) * W * hg;
-        if (a.S.hg_mixed) k *= (R)hg_impedance_ratio(a.sd.mat, eos_i, a.sd.eos[j]);
+        if (MIX && a.S.hg_mixed) k *= (R)hg_impedance_ratio(a.sd.mat, a.sd.eos[i_], a.sd.eos[j]);
         hx += k * ex; hy += k * ey; hz += k * ez;
     }
     // The first stage of a list slot: the same term split as A_ij (y_j - y_i) + c_ij e_ij
@@ -1290,7 +1293,7 @@ template <typename R> struct HgBody {
         R gz = (R)0.5 * ((Gi[2][0] + Gj[2][0]) * rx + (Gi[2][1] + Gj[2][1]) * ry + (Gi[2][2] + Gj[2][2]) * rz);
         R q = rn * (R)F.inv_h, om = (R)1 - q;
         R sw = wq<R>(F, b, b.V, j) * ((R)F.alpha * om * om * om * om * ((R)4 * q + (R)1));
-        if (a.S.hg_mixed) sw *= (R)hg_impedance_ratio(a.sd.mat, eos_i, a.sd.eos[j]);
+        if (a.S.hg_mixed) sw *= (R)hg_impedance_ratio(a.sd.mat, a.sd.eos[i_], a.sd.eos[j]);
         R dyx = (R)(yj.x - yi.x), dyy = (R)(yj.y - yi.y), dyz = (R)(yj.z - yi.z);
         R c = sw * (gx * ex + gy * ey + gz * ez);
         R su = fsqrt<R>(fmax(sw, (R)0));
@@ -1994,7 +1997,7 @@ __global__ __launch_bounds__(SPH_BLOCK) void k_hg_apply(DGrid g, int n, SolidArg
     if (OVF && a.st.scal[SPH_S_LIST_OVER] != 0.0) {   // uniform over the grid
         for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
             if (nl.cnt[p] <= nl.nmax) continue;
-            HgBody<R> ob(a);
+            HgBody<R, false> ob(a);
             double xi, yi, zi;
             ob.begin(p, xi, yi, zi);
             auto f = [&](int j, int mir, double rx, double ry, double rz) { ob.pair(j, mir, rx, ry, rz); };
@@ -2234,7 +2237,7 @@ static void launch_hg_pass(sph_ctx* c, const SolidArgs<R>& a, hipStream_t hs) {
     dim3 gb(blocks), bb(SPH_BLOCK);
     if (a.hg_mode == 2) {
         // up to 2^20 threads (16 384 waves, 16 per SIMD) the overflow check rides along
-        if ((int64_t)n * lpp <= c->hg_fuse) {
+        if ((int64_t)n * lpp <= c->hg_fuse && !a.S.hg_mixed) {
             switch (lpp) {
             case 8: hipLaunchKernelGGL((k_hg_apply<R, 8, true>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
             case 4: hipLaunchKernelGGL((k_hg_apply<R, 4, true>), gb, bb, 0, hs, c->grid, n, a, c->nl); break;
@@ -2315,11 +2318,11 @@ static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F,
         SolidArgs<R> a = solid_args<R>(c, st, sd, F, S);
         a.hgA = (R*)c->hg_pair;
         a.hgb = c->hg_src;
+        timer_begin(c, 6, hs);
         for (int k = 0; k < S.sts_bound; ++k) {
             a.ysrc = (k & 1) ? a.sd.y1 : a.sd.y0;
             a.ydst = (k & 1) ? a.sd.y0 : a.sd.y1;
             a.stage = k;
-            timer_begin(c, 6, hs);
             if (c->hg_pairs) {
                 a.hg_mode = k == 0 ? 1 : 2;
                 launch_hg_pass<R>(c, a, hs);
@@ -2327,8 +2330,8 @@ static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F,
                 a.hg_mode = 0;
                 launch_list_pass<HgBody<R>>(c, a, hs, PASS_HG);
             }
-            timer_end(c, 6, hs);
         }
+        timer_end(c, 6, hs, S.sts_bound);
         hipLaunchKernelGGL((k_sts_kick2<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st),
                            dsolid<R>(st, sd), c->s, c->pstride);
     } else {

@@ -252,7 +252,13 @@ typedef struct {
                               * 0 = off (cap_velocity, SOLID.py:4789-4806)                  */
     int32_t hg_mixed;        /* materials differ in rho0 c_p: the damper's pair terms carry
                               * 2 Z_j / (Z_i + Z_j) (_hg_impedance_ratio, SOLID.py:4184)    */
+    int32_t v_limit;         /* VelocityLimit.limiter: |v - v_pred| <= v_limit_u, v_pred the
+                              * neighbourhood's leave-one-out affine prediction
+                              * (limit_velocity_measure / _apply, SOLID.py:4687-4902)       */
+    int32_t v_limit_conserve;/* ... and the momentum a clamp sheds handed back over the
+                              * donor's neighbours (limit_velocity_redistribute)            */
     int32_t pad3_;
+    double  v_limit_u;       /* the limiter's bound on |v - v_pred|                         */
 } sph_solid_params;
 
 /* sph_solid_params.sts: how the s passes of a step are spent (hourglass_sts,

@@ -207,6 +207,14 @@ def solid_phase(name, dims, params, sparams, s):
                                            ctypes.byref(sparams), ctypes.byref(st))
 
 
+def solid_velocity_limiter(dims, params, sparams, s):
+    """The kinematic velocity limiter (measure, redistribute when conserving, apply) on
+    the state as it stands, with the grid of the last sort."""
+    st = s.c()
+    lib().oracle_solid_velocity_limiter(ctypes.byref(dims), ctypes.byref(params), ctypes.byref(sparams),
+                                        ctypes.byref(st))
+
+
 def solid_hg_accel(dims, params, sparams, s, y):
     y = np.ascontiguousarray(y, dtype=np.float64).reshape(s.n, 3)
     a = np.zeros((s.n, 3))

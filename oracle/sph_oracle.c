@@ -1943,6 +1943,111 @@ static void or_solid_clamps(const sph_dims* d, const sph_fluid_params* P, const 
 
 /* HypoElasticSolver.step (SOLID.py:5227-5231): load factor, SPHBase.step (sort, substep,
  * boundaries), then the clock. */
+/* The kinematic velocity limiter (SOLID.py:4687-4902).
+ * _v_limit_task: neighbour j predicts v_i as v_j + G_j (x_i - x_j) with j's gradient
+ * less particle i's own term, G_j + w_i (v_i - v_j) (x) grad W_ij (leave-one-out);
+ * the prediction is the W w_j weighted mean; acc[4] sums the weights of the
+ * unconstrained neighbours (the ones a donor may pay). */
+typedef struct { or_tctx t; const double* dv; const double* wsum; const int* hit; } or_vlctx;
+
+static void or_vlim_task(void* c, int64_t i, int64_t j, int mir, const double* r) {
+    or_vlctx* q = (or_vlctx*)c;
+    const sph_dims* d = q->t.d;
+    const or_state* s = q->t.s;
+    double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    double gW[3], vj[3] = { s->v[3 * j], s->v[3 * j + 1], s->v[3 * j + 2] }, G[3][3];
+    oracle_wendland_grad(d, r, gW);
+    double w = oracle_wendland(d, rn) * or_w(d, q->t.P, s, j);
+    or_mirror(mir, vj);
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) G[a][b] = s->grad_v[9 * j + 3 * a + b];
+    or_mirror_mat(mir, G);
+    double wi = or_w(d, q->t.P, s, i);
+    for (int a = 0; a < 3; ++a) {
+        double dva = s->v[3 * i + a] - vj[a];
+        for (int b = 0; b < 3; ++b) G[a][b] += wi * dva * gW[b];
+    }
+    for (int a = 0; a < 3; ++a) {
+        double vp = vj[a] + G[a][0] * r[0] + G[a][1] * r[1] + G[a][2] * r[2];
+        q->t.acc[a] += w * vp;
+    }
+    q->t.acc[3] += w;
+    if (s->bc_flag[j] == 0) q->t.acc[4] += w;
+}
+
+/* _v_redistribute_task: donor j's share to i is w_ji / wsum_j of -mw_j dv_j */
+static void or_vredist_task(void* c, int64_t i, int64_t j, int mir, const double* r) {
+    or_vlctx* q = (or_vlctx*)c;
+    const sph_dims* d = q->t.d;
+    const or_state* s = q->t.s;
+    if (!q->hit[j] || !(q->wsum[j] > 1e-30)) return;
+    double dvj[3] = { q->dv[3 * j], q->dv[3 * j + 1], q->dv[3 * j + 2] };
+    or_mirror(mir, dvj);
+    double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    double share = oracle_wendland(d, rn) * or_w(d, q->t.P, s, i) / q->wsum[j];
+    double mj = or_mw(d, q->t.P, s, j);
+    for (int a = 0; a < 3; ++a) q->t.acc[a] += -mj * dvj[a] * share;
+}
+
+/* limit_velocity_measure, _redistribute (when conserving) and _apply: an unconstrained
+ * particle further than u from its prediction is moved back onto the ball of radius u
+ * about it, only if that slows it; the position follows the velocity increment. */
+static void or_velocity_limiter(const sph_dims* d, const sph_fluid_params* P, const sph_solid_params* S,
+                                or_state* s, const or_grid* g) {
+    int64_t n = s->n;
+    double* dv = (double*)calloc((size_t)(3 * n), sizeof(double));
+    double* wsum = (double*)calloc((size_t)n, sizeof(double));
+    double* recv = (double*)calloc((size_t)(3 * n), sizeof(double));
+    int* hit = (int*)calloc((size_t)n, sizeof(int));
+    const double u = S->v_limit_u;
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+        if (s->bc_flag[i] != 0) continue;
+        or_vlctx q = { { d, P, s, { 0 }, 0.0, S }, dv, wsum, hit };
+        or_for_all_neighbors(d, s, g, i, or_vlim_task, &q);
+        wsum[i] = q.t.acc[4];
+        if (!(q.t.acc[3] > 1e-30)) continue;
+        const double* v = s->v + 3 * i;
+        double dd[3], nn = 0.0;
+        for (int a = 0; a < 3; ++a) { dd[a] = v[a] - q.t.acc[a] / q.t.acc[3]; nn += dd[a] * dd[a]; }
+        nn = sqrt(nn);
+        if (!(nn > u)) continue;
+        double inc[3], v2 = 0.0, w2 = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            inc[a] = dd[a] * (u / nn - 1.0);
+            v2 += v[a] * v[a];
+            w2 += (v[a] + inc[a]) * (v[a] + inc[a]);
+        }
+        if (sqrt(w2) < sqrt(v2)) {
+            for (int a = 0; a < 3; ++a) dv[3 * i + a] = inc[a];
+            hit[i] = 1;
+        }
+    }
+    if (S->v_limit_conserve) {
+        #pragma omp parallel for schedule(dynamic, 256)
+        for (int64_t i = 0; i < n; ++i) {
+            if (s->bc_flag[i] != 0) continue;
+            or_vlctx q = { { d, P, s, { 0 }, 0.0, S }, dv, wsum, hit };
+            or_for_all_neighbors(d, s, g, i, or_vredist_task, &q);
+            double mi = or_mw(d, P, s, i);
+            for (int a = 0; a < 3; ++a) recv[3 * i + a] = q.t.acc[a] / mi;
+        }
+    }
+    const double dt = s->scal[SPH_S_DT];
+    for (int64_t i = 0; i < n; ++i) {
+        double inc[3], nn = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            inc[a] = (hit[i] ? dv[3 * i + a] : 0.0) + recv[3 * i + a];
+            nn += inc[a] * inc[a];
+        }
+        if (!(nn > 0.0)) continue;
+        double dr[3];
+        for (int a = 0; a < 3; ++a) { s->v[3 * i + a] += inc[a]; dr[a] = inc[a] * dt; }
+        or_displace(d, P, s, i, dr);
+    }
+    free(dv); free(wsum); free(recv); free(hit);
+}
+
 void oracle_solid_step(const sph_dims* d, const sph_fluid_params* P, const sph_solid_params* S,
                        or_state* s) {
     or_load_factor(S, s);
@@ -1977,6 +2082,7 @@ void oracle_solid_step(const sph_dims* d, const sph_fluid_params* P, const sph_s
         or_advect(d, P, s);
     }
     s->scal[SPH_S_STS_STAGES] = st;
+    if (S->v_limit) or_velocity_limiter(d, P, S, s, &g_grid);
     if (S->v_cap > 0.0) {
         /* cap_velocity (SOLID.py:4789-4806) on unconstrained particles */
         for (int64_t i = 0; i < s->n; ++i) {
@@ -1997,6 +2103,10 @@ void oracle_solid_step(const sph_dims* d, const sph_fluid_params* P, const sph_s
 }
 
 /* Single phases on the state as it stands (grid of the last sort), for unit tests. */
+void oracle_solid_velocity_limiter(const sph_dims* d, const sph_fluid_params* P, const sph_solid_params* S,
+                                   or_state* s) {
+    or_velocity_limiter(d, P, S, s, &g_grid);
+}
 void oracle_solid_velocity_gradient(const sph_dims* d, const sph_fluid_params* P,
                                     const sph_solid_params* S, or_state* s) {
     or_velocity_gradient(d, P, S, s, &g_grid);

@@ -13,10 +13,10 @@ Morris viscosity, the VNR bulk viscosity and Monaghan's pair viscosity with thei
 limits, the rate hourglass damper (explicit with its dt cap, or super-time-stepped:
 subcycle or RKL1), the load ramp and displacement grips, axisymmetry and plane strain,
 scalar damage (spall threshold, Johnson-Cook fracture, Cocks-Ashby void growth), the
-erosion cap and the hard velocity cap.  Everything else the reference offers on this
+erosion cap, the hard velocity cap and the kinematic velocity limiter.  Everything else the reference offers on this
 class raises NotImplementedError when a scene asks for it: JWL / HJC, the erode mode,
 the conservative axisymmetric forms, the viscous / adaptive / saturating hourglass
-forms and the conserving velocity limiter.
+forms.
 Tables of several materials read each particle's own sound speed (the density diffusion
 and the shift) and scale the hourglass damper across an impedance jump.
 """
@@ -156,12 +156,16 @@ class SolidSettings:
                 raise NotImplementedError("material %r: damage on a material without plasticity is "
                                           "not part of this build" % m.name)
         self.jcf_lc = self.dx0                                           # SOLID.py:1675-1682
-        # the runaway limiters (SOLID.py:989-1013, 4687-4976): the hard velocity cap and
-        # erosion's "cap" mode are built; the momentum-conserving limiter and "erode",
-        # which takes particles out of every neighbour sum, are not
+        # the runaway limiters (SOLID.py:889-956, 989-1013, 4687-4976): the hard velocity
+        # cap, the kinematic limiter (by default handing back the momentum it sheds, bound
+        # 2 % of c_p) and erosion's "cap" mode are built; "erode", which takes particles
+        # out of every neighbour sum, is not
         self.v_cap = float(c("VELOCITY_CAP") or 0.0)
-        if c("VELOCITY_LIMIT"):
-            raise NotImplementedError("VelocityLimit.limiter is not part of this build (VelocityLimit.cap is)")
+        self.v_limit = bool(c("VELOCITY_LIMIT") or False)
+        u = c("VELOCITY_LIMIT_U")
+        self.v_limit_u = float(u) if u else 0.02 * float(self.c_p)
+        cons = c("VELOCITY_LIMIT_CONSERVE")
+        self.v_limit_conserve = bool(cons) if cons is not None else True
         self.erosion_mode = str(c("EROSION_MODE") or "off").lower()
         if self.erosion_mode == "erode":
             raise NotImplementedError("Erosion.mode 'erode' is not part of this build ('cap' is)")
@@ -207,6 +211,9 @@ class SolidSettings:
         S.erosion_j_min, S.erosion_j_max = self.erosion_j_min, self.erosion_j_max
         S.v_cap = self.v_cap
         S.hg_mixed = int(self.hg_mixed)
+        S.v_limit = int(self.v_limit)
+        S.v_limit_conserve = int(self.v_limit_conserve)
+        S.v_limit_u = self.v_limit_u
         return S
 
     def load_factor(self, t):

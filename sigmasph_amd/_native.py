@@ -174,7 +174,9 @@ class SphSolidParams(ctypes.Structure):
                     ("jcf_lc", ctypes.c_double), ("damage", ctypes.c_int32), ("jc_failure", ctypes.c_int32),
                     ("cocks_ashby", ctypes.c_int32), ("erosion_cap", ctypes.c_int32),
                     ("erosion_j_min", ctypes.c_double), ("erosion_j_max", ctypes.c_double),
-                    ("v_cap", ctypes.c_double), ("hg_mixed", ctypes.c_int32), ("pad3_", ctypes.c_int32)]
+                    ("v_cap", ctypes.c_double), ("hg_mixed", ctypes.c_int32),
+                    ("v_limit", ctypes.c_int32), ("v_limit_conserve", ctypes.c_int32), ("pad3_", ctypes.c_int32),
+                    ("v_limit_u", ctypes.c_double)]
 
 
 _vp2 = ctypes.c_void_p * 2

@@ -1288,6 +1288,10 @@ struct sph_ctx {
     // the hourglass sub-steps' stage-invariant pair operator, filled by the first stage
     void* hg_pair = nullptr;       // [ncomp][nmax][npad] of R: u = sqrt(w_j W_ij) e
     double4* hg_src = nullptr;     // [npad] sum_j w_j W_ij (e . <G> r) e
+    double4* vl_dv = nullptr;      // [n] the velocity limiter's clamp (w = 1 where it fired)
+    double4* vl_recv = nullptr;    // [n] its hand-back per particle
+    double* vl_wsum = nullptr;     // [n] the payable weight sum of each donor
+    int64_t vl_n = 0;
     bool hg_pairs = true;          // later stages read them instead of recomputing
     int list_alloc = 0;            // entries per particle the list was allocated with
     int list_cap = 0;              // SPH_TUNE_LIST_CAP, 0 = none
@@ -1514,6 +1518,9 @@ extern "C" int sph_ctx_destroy(sph_ctx* c) {
     c->drop_graphs();
     if (c->hg_pair) (void)hipFree(c->hg_pair);
     if (c->hg_src) (void)hipFree(c->hg_src);
+    if (c->vl_dv) (void)hipFree(c->vl_dv);
+    if (c->vl_recv) (void)hipFree(c->vl_recv);
+    if (c->vl_wsum) (void)hipFree(c->vl_wsum);
     for (auto& t : c->tk) for (auto ev : t.ev) (void)hipEventDestroy(ev);
     delete c;
     return SPH_OK;

@@ -50,6 +50,8 @@ struct DSolidP {
     int ero_cap;                         // erosion "cap": J clamped into [ero_jmin, ero_jmax]
     double ero_jmin, ero_jmax;
     double v_cap;                        // |v| ceiling of unconstrained particles, 0 = off
+    int v_limit, v_limit_conserve;       // the kinematic velocity limiter, its hand-back
+    double v_limit_u;                    // its bound on |v - v_pred|
     double jcf_lc;                       // the JC softening length L_c = dx0
 };
 
@@ -2091,6 +2093,9 @@ static DSolidP make_solid_params(const sph_ctx* c, const sph_fluid_params* P, co
     d.hg_mixed = S->hg_mixed != 0;
     d.ero_cap = S->erosion_cap != 0;
     d.v_cap = S->v_cap;
+    d.v_limit = S->v_limit != 0;
+    d.v_limit_conserve = S->v_limit_conserve != 0;
+    d.v_limit_u = S->v_limit_u;
     d.ero_jmin = S->erosion_j_min;
     d.ero_jmax = S->erosion_j_max;
     d.ca_on = S->damage && S->cocks_ashby;
@@ -2121,6 +2126,124 @@ template <typename R> static DSolid<R> dsolid(const sph_state* st, const sph_sol
     d.mat = sd->mat;
     d.cons = sd->cons;
     return d;
+}
+
+// The kinematic velocity limiter (limit_velocity_measure / _redistribute / _apply,
+// SOLID.py:4687-4902), one thread per particle over the cells at the advected positions.
+// Phase 0: the leave-one-out affine prediction v_pred_i = sum_j W w_j (v_j + G_j r) /
+// sum_j W w_j, G_j = grad v_j + w_i (v_i - v_j) (x) grad W_ij, r = x_i - x_j; a clamp back
+// onto |v - v_pred| = u that slows the particle is kept in dv (w = 1), with the weight
+// sum over the unconstrained neighbours.  Phase 1: what each particle is owed by the
+// clamped donors around it, w_ji / wsum_j of -mw_j dv_j, over its own mw.  Phase 2: v
+// and the position take the clamp and the hand-back.  Off unless the scene asks for it,
+// so it is written for clarity, not occupancy.
+template <typename R>
+__global__ __launch_bounds__(SPH_BLOCK) void k_vlimit(DGrid g, int n, SolidArgs<R> a, int phase, double4* dv,
+                                                      double* wsum, double4* recv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DFluid& F = a.F;
+    const DBuf<R>& b = a.st.b;
+    const bool two_d = (F.flags & SPH_TWO_D) != 0;
+    auto wd = [&](int p) {          // the quadrature weight w_p in f64
+        double w = (double)b.V[p];
+        if (F.flags & SPH_AXISYMMETRIC) w /= fmax(b.x[p].y, F.axi_rmin);
+        return w;
+    };
+    auto mwd = [&](int p) {
+        double m = (double)b.m[p];
+        if (F.flags & SPH_AXISYMMETRIC) m /= fmax(b.x[p].y, F.axi_rmin);
+        return m;
+    };
+    auto Wd = [&](double rn) {
+        const double q = rn * F.inv_h, om = 1.0 - q;
+        return q < 1.0 ? F.alpha * om * om * om * om * (4.0 * q + 1.0) : 0.0;
+    };
+    if (phase == 2) {
+        const double4 d = dv[i];
+        const double4 rc = a.S.v_limit_conserve ? recv[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+        double ix = (d.w != 0.0 ? d.x : 0.0) + rc.x, iy = (d.w != 0.0 ? d.y : 0.0) + rc.y,
+               iz = (d.w != 0.0 ? d.z : 0.0) + rc.z;
+        if (!(ix * ix + iy * iy + iz * iz > 0.0)) return;
+        const double dt = a.st.scal[SPH_S_DT];
+        double4 v = b.v[i];
+        v.x += ix; v.y += iy; v.z += iz;
+        b.v[i] = v;
+        double4 x = b.x[i];
+        if (F.disp_pos) {
+            double4 u = b.u[i], x0 = b.x0[i];
+            u.x += ix * dt; u.y += iy * dt; u.z += iz * dt;
+            b.u[i] = u;
+            x.x = x0.x + u.x; x.y = x0.y + u.y; x.z = x0.z + u.z;
+        } else {
+            x.x += ix * dt; x.y += iy * dt; x.z += iz * dt;
+        }
+        x.x = wrap_pos(g, 0, x.x); x.y = wrap_pos(g, 1, x.y); x.z = wrap_pos(g, 2, x.z);
+        b.x[i] = x;
+        return;
+    }
+    const double4 xi = b.x[i];
+    Traverse tr{g, a.s.start, a.s.cellid, a.X, a.any_out()};
+    if (phase == 1) {
+        recv[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (a.sd.bcf[i] != 0) return;
+        const double wi = wd(i);
+        double px = 0.0, py = 0.0, pz = 0.0;
+        auto f = [&](int j, int mir, double rx, double ry, double rz) {
+            const double4 dj = dv[j];
+            if (dj.w == 0.0 || !(wsum[j] > 1e-30)) return;
+            const double sy = (mir & 1) ? -1.0 : 1.0, sz = (mir & 2) ? -1.0 : 1.0;
+            const double share = Wd(sqrt(rx * rx + ry * ry + rz * rz)) * wi / wsum[j];
+            const double k = -mwd(j) * share;
+            px += k * dj.x; py += k * sy * dj.y; pz += k * sz * dj.z;
+        };
+        tr(i, xi.x, xi.y, xi.z, f);
+        const double mi = mwd(i);
+        recv[i] = make_double4(px / mi, py / mi, pz / mi, 0.0);
+        return;
+    }
+    dv[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+    wsum[i] = 0.0;
+    if (a.sd.bcf[i] != 0) return;
+    const double4 vi = b.v[i];
+    const double wi = wd(i);
+    double ax = 0.0, ay = 0.0, az = 0.0, aw = 0.0, ap = 0.0;
+    auto f = [&](int j, int mir, double rx, double ry, double rz) {
+        const double rn = sqrt(rx * rx + ry * ry + rz * rz);
+        const double c = gradw_c<double>(F, rn);
+        const double gx = c * rx, gy = c * ry, gz = c * rz;
+        const double w = Wd(rn) * wd(j);
+        double4 vj = b.v[j];
+        if (mir & 1) vj.y = -vj.y;
+        if (mir & 2) vj.z = -vj.z;
+        R Gr[3][3];
+        load_gv(a.sd.gv, j, two_d, Gr);
+        mirror_gv(Gr, mir);
+        const double dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
+        const double r3[3] = {rx, ry, rz}, dva[3] = {dvx, dvy, dvz}, vja[3] = {vj.x, vj.y, vj.z};
+        double vp[3];
+        for (int p = 0; p < 3; ++p) {
+            const double G0 = (double)Gr[p][0] + wi * dva[p] * gx;
+            const double G1 = (double)Gr[p][1] + wi * dva[p] * gy;
+            const double G2 = (double)Gr[p][2] + wi * dva[p] * gz;
+            vp[p] = vja[p] + G0 * r3[0] + G1 * r3[1] + G2 * r3[2];
+        }
+        ax += w * vp[0]; ay += w * vp[1]; az += w * vp[2];
+        aw += w;
+        if (a.sd.bcf[j] == 0) ap += w;
+    };
+    tr(i, xi.x, xi.y, xi.z, f);
+    wsum[i] = ap;
+    if (!(aw > 1e-30)) return;
+    const double u = a.S.v_limit_u;
+    const double ddx = vi.x - ax / aw, ddy = vi.y - ay / aw, ddz = vi.z - az / aw;
+    const double nn = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+    if (!(nn > u)) return;
+    const double k = u / nn - 1.0;
+    const double ix = ddx * k, iy = ddy * k, iz = ddz * k;
+    const double v2 = vi.x * vi.x + vi.y * vi.y + vi.z * vi.z;
+    const double w2 = (vi.x + ix) * (vi.x + ix) + (vi.y + iy) * (vi.y + iy) + (vi.z + iz) * (vi.z + iz);
+    if (sqrt(w2) < sqrt(v2)) dv[i] = make_double4(ix, iy, iz, 1.0);
 }
 
 template <typename R>
@@ -2338,6 +2461,16 @@ static int solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const DFluid& F,
         hipLaunchKernelGGL((k_advect_solid<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st));
     }
     if ((rc = check_launch(c, "solid advect"))) return rc;
+    if (S.v_limit) {
+        SolidArgs<R> la = solid_args<R>(c, st, sd, F, S);
+        const int np = (int)c->dims.n;
+        for (int ph = 0; ph < 3; ++ph) {
+            if (ph == 1 && !S.v_limit_conserve) continue;   // nothing handed back
+            hipLaunchKernelGGL((k_vlimit<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, np, la, ph, c->vl_dv,
+                               c->vl_wsum, c->vl_recv);
+        }
+        if ((rc = check_launch(c, "velocity limiter"))) return rc;
+    }
     hipLaunchKernelGGL((k_constraints<R>), dim3(nb), dim3(SPH_BLOCK), 0, hs, c->grid, F, dstate<R>(c, st),
                        dsolid<R>(st, sd), c->s, c->cons_part, S.ncons, nb, S.v_cap);
     if (F.ma_mode == 1) hipLaunchKernelGGL(k_publish_vmax, dim3(1), dim3(64), 0, hs, st->scal, c->s);
@@ -2414,6 +2547,18 @@ extern "C" int sph_solid_step(sph_ctx* c, sph_state* st, sph_solid* sd, const sp
         c->list_alloc = nmax;
         c->nl.nmax = c->list_cap > 0 ? std::min(nmax, c->list_cap) : nmax;
         c->nl.npad = npad;
+    }
+    if (S->v_limit && c->vl_n < c->dims.n) {
+        if (c->vl_dv) (void)hipFree(c->vl_dv);
+        if (c->vl_recv) (void)hipFree(c->vl_recv);
+        if (c->vl_wsum) (void)hipFree(c->vl_wsum);
+        c->vl_dv = c->vl_recv = nullptr;
+        c->vl_wsum = nullptr;
+        c->vl_n = 0;
+        HIPCHECK(c, hipMalloc((void**)&c->vl_dv, sizeof(double4) * (size_t)c->dims.n));
+        HIPCHECK(c, hipMalloc((void**)&c->vl_recv, sizeof(double4) * (size_t)c->dims.n));
+        HIPCHECK(c, hipMalloc((void**)&c->vl_wsum, sizeof(double) * (size_t)c->dims.n));
+        c->vl_n = c->dims.n;
     }
     if (S->sts && S->hourglass > 0.0 && c->hg_pairs && !c->hg_pair) {
         const int ncomp = (c->dims.flags & SPH_TWO_D) ? 2 : 3;

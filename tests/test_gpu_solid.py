@@ -76,6 +76,15 @@ SCENES = {
     "bimaterial_explicit": lambda: bimaterial_block(n=(20, 5, 1), jitter=0.02, sts=False,
                                                     div_velocity=(300.0, -150.0, 0.0),
                                                     lower_velocity=(0.0, 120.0, 0.0), velocity=(0.0, -120.0, 0.0)),
+    # the kinematic velocity limiter on a jittered bar stretched past yield (its
+    # leave-one-out prediction leaks a few m/s of the affine stretch, so a 2 m/s bound
+    # fires within two steps): conserving in plane strain, plain in axisymmetry
+    "velocity_limiter": lambda: solid_block(n=(24, 8, 1), div_velocity=(300.0, -150.0, 0.0), jitter=0.05,
+                                            grips=0.05, ramp_time=1e-4,
+                                            extra={"VelocityLimit": {"limiter": {"u": 2.0}}}),
+    "velocity_limiter_axi": lambda: solid_block(n=(24, 8, 1), two_d="axisymmetric", div_velocity=(300.0, 0.0, 0.0),
+                                                jitter=0.05, extra={"VelocityLimit": {"limiter": {
+                                                    "u": 2.0, "conserve": False}}}),
     "plane_strain_noshift": lambda: solid_block(n=(20, 10, 1), pst=False, viscosity=0.0,
                                                 div_velocity=(-200.0, 250.0, 0.0), jitter=0.03),
     # nine specimens with their own Johnson-Cook rows and 18 grips (past 16 the grip

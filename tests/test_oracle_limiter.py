@@ -1,0 +1,58 @@
+"""The oracle's kinematic velocity limiter (limit_velocity_measure / _redistribute /
+_apply, SOLID.py:4687-4902) against what its definition fixes in closed form.
+
+A block at rest with one interior particle k moving at 50 m/s, grad v taken from the
+oracle's own velocity-gradient phase: every neighbour's leave-one-out gradient
+G_j + w_k (v_k - v_j) (x) grad W_kj is exactly zero (k's is the only term of grad v_j,
+and the removal takes it back out), so k's prediction is 0 and the clamp puts v_k on
+|v_k| = u along its own direction, its position moving back by the same increment times
+dt.  The particles at rest read a nonzero residual next to k, but any clamp would speed
+them up, so none fires.  Conserving, the 45 m/s k sheds go to its unconstrained
+neighbours in proportion to W_kj w_j, parallel to v_k, and the total momentum is
+unchanged; not conserving, it is lost.  A mistake in the leave-one-out sign, the
+prediction's weights or the one-sided test moves v_k off (3, -4) or fires a neighbour.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sigmasph_amd.scenes import solid_block
+from helpers import solid_layout, solid_oracle_state
+
+
+def _state(conserve):
+    sc = solid_block(n=(14, 12, 1), jitter=0.04, pst=False,
+                     extra={"VelocityLimit": {"limiter": {"u": 5.0, "conserve": conserve}}})
+    lay, st, P, S = solid_layout(sc)
+    assert S.v_limit == 1 and S.v_limit_u == 5.0 and S.v_limit_conserve == int(conserve)
+    s = solid_oracle_state(lay, st)
+    O.sort_only(lay.dims, s)
+    c = s.x.mean(axis=0)
+    k = int(np.argmin(np.linalg.norm(s.x - c, axis=1)))
+    s.v[:] = 0.0
+    s.v[k] = [30.0, -40.0, 0.0]
+    O.solid_phase("velocity_gradient", lay.dims, P, S, s)
+    s.scal[0] = 1e-6
+    return lay, P, S, s, k
+
+
+@pytest.mark.parametrize("conserve", [False, True])
+def test_outlier_is_clamped_onto_the_ball(conserve):
+    lay, P, S, s, k = _state(conserve)
+    x0, v0, m = s.x.copy(), s.v.copy(), s.m.copy()
+    O.solid_velocity_limiter(lay.dims, P, S, s)
+    np.testing.assert_allclose(s.v[k], [3.0, -4.0, 0.0], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(s.x[k] - x0[k], (s.v[k] - v0[k]) * 1e-6, rtol=0, atol=1e-15)
+    others = np.arange(s.n) != k
+    moved = np.abs(s.v[others]).max(axis=1) > 0.0
+    if not conserve:
+        assert not moved.any()
+        np.testing.assert_array_equal(s.x[others], x0[others])
+        return
+    assert moved.sum() >= 8                                     # k's neighbours are paid
+    p0, p1 = (m[:, None] * v0).sum(axis=0), (m[:, None] * s.v).sum(axis=0)
+    np.testing.assert_allclose(p1, p0, rtol=0, atol=1e-12 * np.abs(m[k] * v0[k]).max())
+    got = s.v[others][moved]
+    e = v0[k] / np.linalg.norm(v0[k])
+    np.testing.assert_allclose(np.cross(got, e)[:, 2], 0.0, atol=1e-12)   # along v_k
+    assert np.all(got @ e > 0.0)

@@ -5,7 +5,8 @@ graphs on a side stream (a steady run alternates between two captured argument s
 six steps replay each graph twice after its capture).  After every step, with the
 program's own synchronisation and host reads in between, every field and every scalar of
 the graph run must equal the eager run's bit for bit: the replayed kernels are the eager
-step's, and the step has no host-side branch on device data.
+step's, and the step has no host-side branch on device data.  Decks: the axisymmetric gripped bar (fp64
+and mixed), and steel bonded to aluminium under the conserving velocity limiter.
 """
 import ctypes
 
@@ -16,7 +17,7 @@ import torch
 from sigmasph_amd import _native as N
 from sigmasph_amd.config import SimConfig
 from sigmasph_amd.particle_system import ParticleSystem
-from sigmasph_amd.scenes import solid_block
+from sigmasph_amd.scenes import bimaterial_block, solid_block
 
 pytestmark = pytest.mark.gpu
 
@@ -28,10 +29,19 @@ def _state(ps):
             ps.cons.cpu().numpy().copy()]
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_graph_replay_equals_eager(gpu, precision):
-    sc = solid_block(n=(24, 8, 1), two_d="axisymmetric", grips=0.05, ramp_time=1e-4,
-                     div_velocity=(300.0, 0.0, 0.0))
+DECKS = {
+    "grips": lambda: solid_block(n=(24, 8, 1), two_d="axisymmetric", grips=0.05, ramp_time=1e-4,
+                                 div_velocity=(300.0, 0.0, 0.0)),
+    # steel bonded to aluminium under the conserving velocity limiter: the limiter's three
+    # launches and the separate overflow launch of a mixed-impedance deck in the graph
+    "limiter_bimaterial": lambda: bimaterial_block(n=(20, 5, 1), jitter=0.05, div_velocity=(300.0, -150.0, 0.0),
+                                                   extra={"VelocityLimit": {"limiter": {"u": 2.0}}}),
+}
+
+
+@pytest.mark.parametrize("deck,precision", [("grips", "fp64"), ("grips", "mixed"), ("limiter_bimaterial", "fp64")])
+def test_graph_replay_equals_eager(gpu, deck, precision):
+    sc = DECKS[deck]()
     a = ParticleSystem(SimConfig(sc), precision=precision)
     b = ParticleSystem(SimConfig(sc), precision=precision)
     sa, sb = a.build_solver(), b.build_solver()
